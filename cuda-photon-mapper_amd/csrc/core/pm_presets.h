@@ -1,0 +1,194 @@
+// pm_presets.h -- host-side configuration: the workload presets and the
+// conversion of the C-ABI pm_config into the per-thread Scene/params structs.
+//
+// Reference counterparts: the compile-time tunables (photonMappingKernel.cu:7-46)
+// and the scene globals (:59-98); sphere animation positionObjects (:1380-1404).
+#pragma once
+
+#include <string.h>
+
+#include <vector>
+
+#include "../../../include/photon_mapper.h"
+#include "pm_transport.h"
+
+namespace pm {
+
+inline void preset_room(pm_config* c) {
+  memset(c, 0, sizeof(*c));
+  c->seed = 1;
+  const float white[3] = {0.7f, 0.7f, 0.7f};
+  const float green[3] = {0.07f, 0.7f, 0.07f};
+  const float red[3] = {0.7f, 0.07f, 0.07f};
+  for (int f = 0; f < 5; ++f)
+    for (int k = 0; k < 3; ++k) c->wall_albedo[f][k] = f == 0 ? green[k] : (f == 2 ? red[k] : white[k]);
+  c->n_spheres = 2;
+  // Lambertian sphere r=0.2 and perfect mirror r=0.25, both resting on the floor.
+  pm_sphere* s0 = &c->spheres[0];
+  s0->center[0] = 0.28f; s0->center[1] = 0.2f; s0->center[2] = 0.62f; s0->radius = 0.2f;
+  s0->albedo[0] = s0->albedo[1] = s0->albedo[2] = 0.7f; s0->material = PM_MAT_DIFFUSE; s0->ior = 1.0f;
+  pm_sphere* s1 = &c->spheres[1];
+  s1->center[0] = 0.72f; s1->center[1] = 0.25f; s1->center[2] = 0.38f; s1->radius = 0.25f;
+  s1->albedo[0] = s1->albedo[1] = s1->albedo[2] = 1.0f; s1->material = PM_MAT_MIRROR; s1->ior = 1.0f;
+  c->light_pos[0] = 0.5f; c->light_pos[1] = 0.98f; c->light_pos[2] = 0.5f;
+  c->light_power[0] = c->light_power[1] = c->light_power[2] = 100.0f;
+  c->medium = PM_MEDIUM_HOMOGENEOUS;
+  c->sigma_s = 0.1f; c->sigma_a = 0.01f; c->hg_g = 0.0f;
+  c->noise_res = 64; c->noise_freq = 8;
+  c->n_emit = 100000; c->cap_surface = 0; c->cap_volume = 0;
+  c->max_bounces = 8; c->rr_survive = 0.8f; c->leaf_size = 32;
+  c->storage = PM_STORE_FP32;
+  c->k_surface = 32; c->k_volume = 32;
+  c->rcap_surface = 0.05f; c->rcap_volume = 0.05f;
+  c->estimator = PM_EST_BOX;
+  c->width = 256; c->height = 256; c->spp = 1;
+  c->march_step = 0.01f; c->max_depth = 8;
+  c->eye[0] = 0.5f; c->eye[1] = 0.5f; c->eye[2] = -0.8f;
+  c->focal = 1.5f; c->exposure = 0.1f;
+  c->fp64_transmittance = 0;
+  c->media_on = 1;
+  c->anim_time = 0.0f;
+}
+
+// Returns 0 on success, -1 for an unknown preset.
+inline int preset_config(int preset, pm_config* c) {
+  preset_room(c);
+  switch (preset) {
+    case 1:
+      return 0;
+    case 2:
+      c->n_emit = 0;
+      c->cap_surface = 1000000u;
+      c->cap_volume = 2000000u;
+      c->medium = PM_MEDIUM_NOISE;
+      c->sigma_s = 0.3f; c->sigma_a = 0.03f; c->hg_g = 0.6f;
+      c->k_surface = 64; c->k_volume = 64;
+      c->rcap_surface = 0.03f; c->rcap_volume = 0.03f;
+      c->leaf_size = 32;
+      c->width = 1920; c->height = 1080; c->spp = 4;
+      return 0;
+    case 3:
+      c->storage = PM_STORE_BF16;
+      c->fp64_transmittance = 1;
+      c->width = 480; c->height = 270;
+      return 0;
+    case 5:
+      return 0;
+    default:
+      return -1;
+  }
+}
+
+// Sphere array with the legacy animation applied: each sphere's centre moves
+// on a small horizontal circle of radius 0.03 with phase anim_time + i*pi
+// (the reference moves spheres 0 and 1 along sin/cos paths, :1380-1392).
+inline std::vector<Sphere> build_spheres(const pm_config& c) {
+  std::vector<Sphere> v((size_t)c.n_spheres);
+  for (int i = 0; i < c.n_spheres; ++i) {
+    const pm_sphere& s = c.spheres[i];
+    Sphere& o = v[(size_t)i];
+    memset(&o, 0, sizeof(o));
+    o.cx = s.center[0]; o.cy = s.center[1]; o.cz = s.center[2]; o.r = s.radius;
+    o.ar = s.albedo[0]; o.ag = s.albedo[1]; o.ab = s.albedo[2];
+    o.mat = s.material; o.ior = s.ior;
+    if (c.anim_time != 0.0f) {
+      float turns = c.anim_time * 0.159154943091895336f + 0.5f * (float)i;
+      turns = turns - floor_(turns);
+      float sn, cs;
+      sincos_turns(turns, &sn, &cs);
+      o.cx = o.cx + 0.03f * sn;
+      o.cz = o.cz + 0.03f * cs;
+    }
+  }
+  return v;
+}
+
+inline std::vector<float> build_density(const pm_config& c) {
+  std::vector<float> d;
+  if (c.medium != PM_MEDIUM_NOISE) return d;
+  const int R = c.noise_res;
+  d.resize((size_t)R * R * R);
+  for (int k = 0; k < R; ++k)
+    for (int j = 0; j < R; ++j)
+      for (int i = 0; i < R; ++i)
+        d[((size_t)k * R + j) * R + i] = noise_vertex(c.seed, R, c.noise_freq, i, j, k);
+  return d;
+}
+
+inline Scene make_scene(const pm_config& c, const Sphere* spheres, const float* density) {
+  Scene s;
+  memset(&s, 0, sizeof(s));
+  for (int f = 0; f < 5; ++f)
+    for (int k = 0; k < 3; ++k) s.wall_albedo[f][k] = c.wall_albedo[f][k];
+  s.nspheres = c.n_spheres;
+  s.spheres = spheres;
+  for (int k = 0; k < 3; ++k) {
+    s.light[k] = c.light_pos[k];
+    s.light_power[k] = c.light_power[k];
+  }
+  s.med.type = c.medium;
+  s.med.sigma_s = c.sigma_s;
+  s.med.sigma_a = c.sigma_a;
+  s.med.g = c.hg_g;
+  s.med.res = c.noise_res;
+  s.med.majorant = c.sigma_s + c.sigma_a;  // density <= 1
+  s.med.density = density;
+  return s;
+}
+
+inline TraceParams trace_params(const pm_config& c) {
+  TraceParams t;
+  t.seed = c.seed;
+  t.max_bounces = c.max_bounces;
+  t.rr_survive = c.rr_survive;
+  t.eps = 1e-4f;
+  t.max_delta_iters = 256;
+  t.media_on = c.media_on && c.medium != PM_MEDIUM_NONE;
+  return t;
+}
+
+inline RenderParams render_params(const pm_config& c) {
+  RenderParams r;
+  r.seed = c.seed;
+  r.width = c.width;
+  r.height = c.height;
+  r.spp = c.spp;
+  r.march_step = c.march_step;
+  r.max_depth = c.max_depth;
+  r.eps = 1e-4f;
+  for (int k = 0; k < 3; ++k) r.eye[k] = c.eye[k];
+  r.focal = c.focal;
+  r.media_on = c.media_on && c.medium != PM_MEDIUM_NONE;
+  r.fp64_transmittance = c.fp64_transmittance;
+  r.max_steps = 1 << 20;
+  return r;
+}
+
+// Validates a configuration; returns 0 or PM_ERR_INVALID.
+inline int validate_config(const pm_config& c) {
+  if (c.n_spheres < 0 || c.n_spheres > PM_MAX_SPHERES) return PM_ERR_INVALID;
+  if (c.max_bounces < 1 || c.max_bounces > 64) return PM_ERR_INVALID;
+  if (!(c.rr_survive > 0.0f) || c.rr_survive > 1.0f) return PM_ERR_INVALID;
+  if (c.leaf_size < 1 || c.leaf_size > 64) return PM_ERR_INVALID;
+  if (c.k_surface < 1 || c.k_surface > 64 || c.k_volume < 1 || c.k_volume > 64) return PM_ERR_INVALID;
+  if (!(c.rcap_surface > 0.0f) || !(c.rcap_volume > 0.0f)) return PM_ERR_INVALID;
+  if (c.width < 1 || c.height < 1 || c.spp < 1 || c.width > 16384 || c.height > 16384 || c.spp > 1024)
+    return PM_ERR_INVALID;
+  if (!(c.march_step > 0.0f) || c.max_depth < 1 || c.max_depth > 64) return PM_ERR_INVALID;
+  if (c.medium < PM_MEDIUM_NONE || c.medium > PM_MEDIUM_NOISE) return PM_ERR_INVALID;
+  if (c.medium == PM_MEDIUM_NOISE && (c.noise_res < 2 || c.noise_res > 512 || c.noise_freq < 1)) return PM_ERR_INVALID;
+  if (c.medium != PM_MEDIUM_NONE && !(c.sigma_s + c.sigma_a > 0.0f)) return PM_ERR_INVALID;
+  if (c.hg_g <= -1.0f || c.hg_g >= 1.0f) return PM_ERR_INVALID;
+  if (c.n_emit == 0 && (c.cap_surface == 0 || c.cap_volume == 0) && c.media_on && c.medium != PM_MEDIUM_NONE)
+    return PM_ERR_INVALID;
+  if (c.n_emit == 0 && c.cap_surface == 0) return PM_ERR_INVALID;
+  for (int i = 0; i < c.n_spheres; ++i) {
+    const pm_sphere& s = c.spheres[i];
+    if (!(s.radius > 0.0f) || s.material < 0 || s.material > 2) return PM_ERR_INVALID;
+    if (s.material == PM_MAT_DIELECTRIC && !(s.ior > 1.0f)) return PM_ERR_INVALID;
+  }
+  if (c.storage != PM_STORE_FP32 && c.storage != PM_STORE_BF16) return PM_ERR_INVALID;
+  return 0;
+}
+
+}  // namespace pm

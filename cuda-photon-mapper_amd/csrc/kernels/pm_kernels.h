@@ -1,0 +1,88 @@
+// pm_kernels.h -- host-side launchers of the gfx950 kernels and the device
+// data layout they share.  Internal to libpm.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../core/pm_kdtree.h"
+#include "../core/pm_transport.h"
+
+namespace pm {
+
+// Photon map on the device, structure of arrays.  Records are in (photon id,
+// event) order; `payload` holds power and travel direction per record:
+//   FP32 storage: 8 floats  (pr, pg, pb, dx | dy, dz, 0, 0)   = 32 B
+//   BF16 storage: 8 bf16    (pr, pg, pb, dx, dy, dz, 0, 0)    = 16 B
+// The kd-tree keeps the positions again in tree order as float4
+// (x, y, z, bits(record index)) so a leaf is one contiguous, coalesced read.
+struct DevMap {
+  uint32_t n;        // records
+  uint32_t cap;      // allocated records
+  uint32_t n_emit;   // photons emitted for this map's normalisation
+  int bf16;
+  float* x;
+  float* y;
+  float* z;
+  void* payload;
+  // kd-tree
+  uint32_t L, nl;
+  float4* tpts;      // n, tree order
+  KdNode* nodes;     // nl - 1
+  bool built;
+};
+
+// Gather parameters of one map (passed by value to kernels).
+struct GatherMap {
+  const float4* tpts;
+  const KdNode* nodes;
+  const void* payload;
+  uint32_t n, nl;
+  int bf16;
+  int k;
+  float rcap2;
+  float scale[3];  // light power / n_emit
+};
+
+// scan / sort (pm_scan_sort.hip)
+size_t scan_temp_words(size_t n);
+hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t* temp, uint32_t* total_dev,
+                          hipStream_t st);
+size_t radix_temp_words(size_t n);
+hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, size_t n, uint32_t* temp, hipStream_t st);
+
+// trace (pm_trace.hip)
+hipError_t launch_trace_count(const Scene& sc, const TraceParams& tp, uint32_t pid0, uint32_t npid,
+                              uint32_t* cnt_s, uint32_t* cnt_v, hipStream_t st);
+hipError_t launch_find_cap(const uint32_t* off, const uint32_t* cnt, uint32_t n, uint32_t cap, uint32_t* out,
+                           hipStream_t st);
+hipError_t launch_trace_write(const Scene& sc, const TraceParams& tp, uint32_t npid, const uint32_t* off_s,
+                              const uint32_t* off_v, DevMap ms, DevMap mv, double* tally_partial, hipStream_t st);
+hipError_t launch_tally_final(const double* partial, uint32_t nblocks, double* out9, hipStream_t st);
+uint32_t trace_write_blocks(uint32_t npid);
+
+// kd-tree (pm_kdbuild.hip)
+size_t kdbuild_temp_words(uint32_t n, uint32_t L);
+// Shape tables (host arrays of the nl-1 internal nodes): point range [lo, hi),
+// left count lc, and lb = left counts of the same level's earlier nodes.
+hipError_t kd_build_impl(DevMap& m, uint32_t* temp, const uint32_t* host_lo, const uint32_t* host_hi,
+                         const uint32_t* host_lc, const uint32_t* host_lb, hipStream_t st);
+
+// gather (pm_gather.hip)
+hipError_t launch_knn(GatherMap g, const float* q, uint32_t nq, uint32_t* idx, float* d2, uint32_t* count,
+                      hipStream_t st);
+struct RenderLaunch {
+  Scene sc;
+  RenderParams rp;
+  GatherMap gs, gv;
+  int estimator;  // 0 box, 1 cone
+  float exposure;
+  int x0, y0, w, h;
+  float* rgb;      // w*h*3 or null
+  uchar4* u8;      // w*h or null
+  unsigned long long* counters;  // [surface queries, volume queries]
+};
+hipError_t launch_render(const RenderLaunch& r, hipStream_t st);
+
+}  // namespace pm

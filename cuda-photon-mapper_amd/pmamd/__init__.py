@@ -1,0 +1,299 @@
+"""Python binding of the MI355X photon mapper's C ABI (libpm.so), via ctypes.
+
+The library is the product: HIP kernels for gfx950 behind the extern "C"
+entry points declared in include/photon_mapper.h.  This module only mirrors
+the C structs and wraps the calls; it never computes anything itself and it
+raises if libpm.so is missing (there is no fallback path).
+
+Reference interface mirrored: the three launchers of photonMappingKernel.cu
+(:1523 launch_emit_photons_kernel, :1549 launch_photon_mapping_kernel,
+:1569 launch_init_random_numbers_kernel), called in the order of
+callbacksPBO.cpp:55-63 (init random numbers once, emit, render).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG_ROOT, "libpm.so")
+
+PM_OK = 0
+PM_ERR_INVALID = -1
+PM_ERR_HIP = -2
+PM_ERR_STATE = -3
+PM_ERR_CAPACITY = -4
+PM_ERR_NOMEM = -5
+
+PM_MAX_SPHERES = 256
+PM_MAT_DIFFUSE, PM_MAT_MIRROR, PM_MAT_DIELECTRIC = 0, 1, 2
+PM_MEDIUM_NONE, PM_MEDIUM_HOMOGENEOUS, PM_MEDIUM_NOISE = 0, 1, 2
+PM_STORE_FP32, PM_STORE_BF16 = 0, 1
+PM_EST_BOX, PM_EST_CONE = 0, 1
+PM_MAP_SURFACE, PM_MAP_VOLUME = 0, 1
+
+
+class PmSphere(ctypes.Structure):
+    _fields_ = [
+        ("center", ctypes.c_float * 3),
+        ("radius", ctypes.c_float),
+        ("albedo", ctypes.c_float * 3),
+        ("material", ctypes.c_int),
+        ("ior", ctypes.c_float),
+    ]
+
+
+class PmConfig(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint32),
+        ("wall_albedo", (ctypes.c_float * 3) * 5),
+        ("n_spheres", ctypes.c_int),
+        ("spheres", PmSphere * PM_MAX_SPHERES),
+        ("light_pos", ctypes.c_float * 3),
+        ("light_power", ctypes.c_float * 3),
+        ("medium", ctypes.c_int),
+        ("sigma_s", ctypes.c_float),
+        ("sigma_a", ctypes.c_float),
+        ("hg_g", ctypes.c_float),
+        ("noise_res", ctypes.c_int),
+        ("noise_freq", ctypes.c_int),
+        ("n_emit", ctypes.c_uint32),
+        ("cap_surface", ctypes.c_uint32),
+        ("cap_volume", ctypes.c_uint32),
+        ("max_bounces", ctypes.c_int),
+        ("rr_survive", ctypes.c_float),
+        ("leaf_size", ctypes.c_int),
+        ("storage", ctypes.c_int),
+        ("k_surface", ctypes.c_int),
+        ("k_volume", ctypes.c_int),
+        ("rcap_surface", ctypes.c_float),
+        ("rcap_volume", ctypes.c_float),
+        ("estimator", ctypes.c_int),
+        ("width", ctypes.c_int),
+        ("height", ctypes.c_int),
+        ("spp", ctypes.c_int),
+        ("march_step", ctypes.c_float),
+        ("max_depth", ctypes.c_int),
+        ("eye", ctypes.c_float * 3),
+        ("focal", ctypes.c_float),
+        ("exposure", ctypes.c_float),
+        ("fp64_transmittance", ctypes.c_int),
+        ("media_on", ctypes.c_int),
+        ("anim_time", ctypes.c_float),
+    ]
+
+
+class PmStats(ctypes.Structure):
+    _fields_ = [
+        ("n_surface", ctypes.c_uint32),
+        ("n_volume", ctypes.c_uint32),
+        ("n_emit_surface", ctypes.c_uint32),
+        ("n_emit_volume", ctypes.c_uint32),
+        ("n_emit_total", ctypes.c_uint32),
+        ("pad0", ctypes.c_uint32),
+        ("emitted", ctypes.c_double * 3),
+        ("absorbed", ctypes.c_double * 3),
+        ("escaped", ctypes.c_double * 3),
+        ("rr_gain", ctypes.c_double * 3),
+        ("ms_emit", ctypes.c_double),
+        ("ms_build", ctypes.c_double),
+        ("ms_render", ctypes.c_double),
+        ("ms_gather_kernel", ctypes.c_double),
+        ("surface_queries", ctypes.c_uint64),
+        ("volume_queries", ctypes.c_uint64),
+    ]
+
+
+class PmError(RuntimeError):
+    def __init__(self, status, what):
+        super().__init__("%s failed with status %d" % (what, status))
+        self.status = status
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "launch_init_random_numbers_kernel": (None, []),
+    "launch_emit_photons_kernel": (None, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_float,
+                                          ctypes.c_bool, ctypes.c_bool]),
+    "launch_photon_mapping_kernel": (None, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_float,
+                                            ctypes.c_bool, ctypes.c_bool]),
+    "pm_config_preset": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(PmConfig)]),
+    "pm_create": (ctypes.c_int, [ctypes.POINTER(PmConfig), ctypes.POINTER(ctypes.c_void_p)]),
+    "pm_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "pm_emit": (ctypes.c_int, [ctypes.c_void_p]),
+    "pm_build": (ctypes.c_int, [ctypes.c_void_p]),
+    "pm_render": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "pm_render_crop": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_void_p, ctypes.c_void_p]),
+    "pm_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PmStats)]),
+    "pm_set_render": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_float, ctypes.c_float]),
+    "pm_get_records": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_uint32]),
+    "pm_get_tree": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
+    "pm_knn": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int,
+                              ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "pm_device_alloc": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]),
+    "pm_device_free": (ctypes.c_int, [ctypes.c_void_p]),
+    "pm_memcpy_htod": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+    "pm_memcpy_dtoh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+    "pm_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "pm_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+}
+
+
+def exported_symbols():
+    """Names of every entry point include/photon_mapper.h declares."""
+    return list(_SIGS)
+
+
+def load(path=LIB_PATH):
+    """Load libpm.so (built in-tree by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError("libpm.so not built: %s (run __graft_entry__.build())" % path)
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status, what):
+    if status != PM_OK:
+        lib = load()
+        msg = lib.pm_status_string(status)
+        raise PmError(status, "%s (%s)" % (what, msg.decode() if msg else "?"))
+
+
+def preset(n):
+    cfg = PmConfig()
+    check(load().pm_config_preset(n, ctypes.byref(cfg)), "pm_config_preset")
+    return cfg
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    st = load().pm_device_count(ctypes.byref(n))
+    return n.value if st == PM_OK else 0
+
+
+class DeviceBuffer:
+    """A device allocation made through the library (hipMalloc)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        check(load().pm_device_alloc(ctypes.byref(p), max(self.nbytes, 1)), "pm_device_alloc")
+        self.ptr = p.value
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes <= self.nbytes
+        check(load().pm_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes), "pm_memcpy_htod")
+
+    def download(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            check(load().pm_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes), "pm_memcpy_dtoh")
+        return out
+
+    def free(self):
+        if self.ptr:
+            load().pm_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class PhotonMapper:
+    """One library context: emit -> build -> render, plus inspection."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        check(load().pm_create(ctypes.byref(cfg), ctypes.byref(h)), "pm_create")
+        self.h = h.value
+
+    def close(self):
+        if self.h:
+            load().pm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def emit(self):
+        check(load().pm_emit(self.h), "pm_emit")
+
+    def build(self):
+        check(load().pm_build(self.h), "pm_build")
+
+    def stats(self):
+        s = PmStats()
+        check(load().pm_stats_get(self.h, ctypes.byref(s)), "pm_stats_get")
+        return s
+
+    def set_render(self, width, height, spp, k_s, k_v, rcap_s, rcap_v):
+        check(load().pm_set_render(self.h, width, height, spp, k_s, k_v, rcap_s, rcap_v), "pm_set_render")
+
+    def render(self, x0=0, y0=0, w=0, h=0, want_u8=False):
+        if w <= 0:
+            w, h, x0, y0 = self.cfg.width, self.cfg.height, 0, 0
+        rgb = DeviceBuffer(w * h * 3 * 4)
+        u8 = DeviceBuffer(w * h * 4) if want_u8 else None
+        check(load().pm_render_crop(self.h, x0, y0, w, h, rgb.ptr, u8.ptr if u8 else None), "pm_render_crop")
+        out = rgb.download(np.float32, w * h * 3).reshape(h, w, 3)
+        if want_u8:
+            return out, u8.download(np.uint8, w * h * 4).reshape(h, w, 4)
+        return out
+
+    def records(self, which):
+        s = self.stats()
+        n = s.n_surface if which == PM_MAP_SURFACE else s.n_volume
+        pos = np.empty((n, 3), np.float32)
+        pw = np.empty((n, 3), np.float32)
+        d = np.empty((n, 3), np.float32)
+        check(load().pm_get_records(self.h, which, pos.ctypes.data, pw.ctypes.data, d.ctypes.data, n),
+              "pm_get_records")
+        return pos, pw, d
+
+    def tree(self, which):
+        s = self.stats()
+        n = s.n_surface if which == PM_MAP_SURFACE else s.n_volume
+        L = self.cfg.leaf_size
+        nl = (n + L - 1) // L if n else 0
+        ni = max(nl - 1, 0)
+        perm = np.empty(n, np.uint32)
+        split = np.empty(ni, np.float32)
+        axis = np.empty(ni, np.uint32)
+        check(load().pm_get_tree(self.h, which, perm.ctypes.data, split.ctypes.data, axis.ctypes.data, n, ni),
+              "pm_get_tree")
+        return perm, split, axis
+
+    def knn(self, which, queries, k, rcap):
+        q = np.ascontiguousarray(queries, dtype=np.float32).reshape(-1, 3)
+        nq = q.shape[0]
+        qb = DeviceBuffer(max(q.nbytes, 4))
+        qb.upload(q)
+        ib = DeviceBuffer(nq * k * 4)
+        db = DeviceBuffer(nq * k * 4)
+        cb = DeviceBuffer(nq * 4)
+        check(load().pm_knn(self.h, which, qb.ptr, nq, k, rcap, ib.ptr, db.ptr, cb.ptr), "pm_knn")
+        return (ib.download(np.uint32, nq * k).reshape(nq, k), db.download(np.float32, nq * k).reshape(nq, k),
+                cb.download(np.uint32, nq))

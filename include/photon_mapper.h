@@ -1,0 +1,183 @@
+/* photon_mapper.h -- C ABI of the MI355X photon mapper (libpm.so).
+ *
+ * Drop-in boundary for the reference's photonMappingKernel.cu launch ABI
+ * (declared by its caller at simplePBO.cpp:27-29, defined at
+ * photonMappingKernel.cu:1523-1582), plus an additive status-returning pm_*
+ * API.  Every pointer named *_dev is a device pointer owned by the caller;
+ * the library never frees or retains it.  Photon-map state persists inside a
+ * library-owned context between calls (the reference keeps it in __device__
+ * globals, photonMappingKernel.cu:76,80).
+ *
+ * Threading: calls on one context are not re-entrant; every call is
+ * synchronous (returns after the GPU work finished), like the reference's
+ * launchers (cudaThreadSynchronize after each launch).
+ */
+#ifndef PHOTON_MAPPER_H
+#define PHOTON_MAPPER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+/* uchar4 {x=r, y=g, z=b, w=0}: same 4-byte layout as HIP's/CUDA's uchar4. */
+#ifndef PM_UCHAR4_DEFINED
+#define PM_UCHAR4_DEFINED
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+typedef uchar4 pm_uchar4;
+#else
+typedef struct pm_uchar4 { unsigned char x, y, z, w; } pm_uchar4;
+#endif
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ legacy ABI
+ * Same symbols, signatures and error behaviour (print + exit(EXIT_FAILURE)) as
+ * the reference.  They drive a default context on the current HIP device
+ * configured as the unit-room scene (100k photons, k=32, r<=0.05, 1 spp).
+ */
+
+/* Replaces photonMappingKernel.cu:1569-1582 (fills the MWC random table).
+ * Here: creates (or resets) the default context; Philox needs no table. */
+void launch_init_random_numbers_kernel(void);
+
+/* Replaces photonMappingKernel.cu:1523-1546 (grid reset + photon emission).
+ * Here: traces photons and builds the kd-trees.  pos/width/height are unused
+ * (as in the reference).  animTime moves the two spheres (reference
+ * positionObjects, :1380-1404); participatingMediaFlag enables the medium
+ * (:1239); interpolateFlag selects the cone-filtered estimate (the
+ * reference's bilinear window blend, :591). */
+void launch_emit_photons_kernel(pm_uchar4* pos, unsigned int image_width, unsigned int image_height,
+                                float animTime, bool interpolateFlag, bool participatingMediaFlag);
+
+/* Replaces photonMappingKernel.cu:1549-1566 (one thread per pixel into the
+ * GL PBO).  Here: renders image_width x image_height into the caller's
+ * device buffer `pos` (W*H uchar4), 1 sample per pixel. */
+void launch_photon_mapping_kernel(pm_uchar4* pos, unsigned int image_width, unsigned int image_height,
+                                  float animTime, bool interpolateFlag, bool participatingMediaFlag);
+
+/* ------------------------------------------------------------------ pm_* API
+ * Additive API (no reference counterpart; the reference configures itself
+ * through compile-time macros at photonMappingKernel.cu:7-46 and __device__
+ * globals at :59-98).  All functions return PM_OK or a negative status. */
+
+enum {
+  PM_OK = 0,
+  PM_ERR_INVALID = -1,   /* null pointer, bad size or out-of-range parameter */
+  PM_ERR_HIP = -2,       /* a HIP runtime call failed */
+  PM_ERR_STATE = -3,     /* call order violated (e.g. render before emit) */
+  PM_ERR_CAPACITY = -4,  /* a bounded buffer would overflow (nothing was written) */
+  PM_ERR_NOMEM = -5
+};
+
+#define PM_MAX_SPHERES 256
+
+enum { PM_MAT_DIFFUSE = 0, PM_MAT_MIRROR = 1, PM_MAT_DIELECTRIC = 2 };
+enum { PM_MEDIUM_NONE = 0, PM_MEDIUM_HOMOGENEOUS = 1, PM_MEDIUM_NOISE = 2 };
+enum { PM_STORE_FP32 = 0, PM_STORE_BF16 = 1 };
+enum { PM_EST_BOX = 0, PM_EST_CONE = 1 };
+enum { PM_MAP_SURFACE = 0, PM_MAP_VOLUME = 1 };
+
+typedef struct pm_sphere {
+  float center[3];
+  float radius;
+  float albedo[3];
+  int material;
+  float ior;
+} pm_sphere;
+
+typedef struct pm_config {
+  uint32_t seed;
+  /* scene: unit room [0,1]^3, walls 0:x=1 1:y=0 2:x=0 3:y=1 4:z=1, front open */
+  float wall_albedo[5][3];
+  int n_spheres;
+  pm_sphere spheres[PM_MAX_SPHERES];
+  float light_pos[3];
+  float light_power[3]; /* W per channel */
+  /* medium */
+  int medium;
+  float sigma_s, sigma_a, hg_g;
+  int noise_res, noise_freq;
+  /* photons */
+  uint32_t n_emit;       /* photons emitted; 0 = emit until both caps are filled */
+  uint32_t cap_surface;  /* record caps (0 = unbounded when n_emit > 0) */
+  uint32_t cap_volume;
+  int max_bounces;
+  float rr_survive;
+  int leaf_size;
+  int storage;           /* PM_STORE_FP32 or PM_STORE_BF16 (power and direction) */
+  /* gather */
+  int k_surface, k_volume;
+  float rcap_surface, rcap_volume;
+  int estimator;
+  /* render */
+  int width, height, spp;
+  float march_step;
+  int max_depth;
+  float eye[3];
+  float focal;
+  float exposure;
+  int fp64_transmittance;
+  int media_on;
+  float anim_time;
+} pm_config;
+
+typedef struct pm_stats {
+  uint32_t n_surface, n_volume;              /* records in each map */
+  uint32_t n_emit_surface, n_emit_volume;    /* photons emitted for each map's normalisation */
+  uint32_t n_emit_total;                     /* photons traced */
+  uint32_t pad0;
+  double emitted[3], absorbed[3], escaped[3], rr_gain[3]; /* FP64 flux tally, watts */
+  double ms_emit, ms_build, ms_render;       /* HIP-event times of the last calls */
+  double ms_gather_kernel;                   /* render kernel alone */
+  uint64_t surface_queries, volume_queries;  /* radiance queries of the last render */
+} pm_stats;
+
+typedef struct pm_context pm_context;
+
+/* Presets: 1 = unit room, 100k photons, homogeneous medium, k=32, 256x256;
+ * 2 = 1M surface + 2M volume records, value-noise medium, k=64, 1920x1080x4;
+ * 3 = preset 1 with BF16 power/direction and FP64 transmittance, 480x270;
+ * 5 = preset 1 scene for the learned radiance cache. */
+int pm_config_preset(int preset, pm_config* out);
+
+int pm_create(const pm_config* cfg, pm_context** out);
+int pm_destroy(pm_context* ctx);
+/* Trace photons and fill both photon maps (records in (photon id, event) order). */
+int pm_emit(pm_context* ctx);
+/* Build the kd-trees of both maps. */
+int pm_build(pm_context* ctx);
+/* Render the configured frame (or the crop x0,y0,w,h when w>0): rgb_dev receives
+ * w*h*3 FP32 radiance (may be NULL), out_dev w*h uchar4 (may be NULL). */
+int pm_render(pm_context* ctx, float* rgb_dev, pm_uchar4* out_dev);
+int pm_render_crop(pm_context* ctx, int x0, int y0, int w, int h, float* rgb_dev, pm_uchar4* out_dev);
+int pm_stats_get(pm_context* ctx, pm_stats* out);
+/* Change the per-render parameters (resolution, spp, k, caps) without re-emitting. */
+int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface, int k_volume,
+                  float rcap_surface, float rcap_volume);
+
+/* Inspection (copies to host memory; for tests and tools). */
+int pm_get_records(pm_context* ctx, int map, float* pos_host, float* power_host, float* dir_host, uint32_t n);
+int pm_get_tree(pm_context* ctx, int map, uint32_t* perm_host, float* split_host, uint32_t* axis_host,
+                uint32_t n, uint32_t n_internal);
+/* kNN over a map's kd-tree for nq device query points (xyz, FP32): idx_dev and
+ * d2_dev receive nq*k entries sorted by (d2, photon index), padded with
+ * 0xffffffff / +inf; count_dev (may be NULL) receives the number found. */
+int pm_knn(pm_context* ctx, int map, const float* queries_dev, uint32_t nq, int k, float rcap,
+           uint32_t* idx_dev, float* d2_dev, uint32_t* count_dev);
+
+/* Device memory helpers for hosts without a HIP runtime binding (ctypes). */
+int pm_device_alloc(void** ptr, size_t bytes);
+int pm_device_free(void* ptr);
+int pm_memcpy_htod(void* dst_dev, const void* src_host, size_t bytes);
+int pm_memcpy_dtoh(void* dst_host, const void* src_dev, size_t bytes);
+int pm_device_count(int* n);
+const char* pm_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHOTON_MAPPER_H */
