@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark of the photon-mapper hot path on one MI355X.
+
+One step = one frame of the 1080p workload (1M surface + 2M volume photon
+records, 64^3 value-noise medium, k=64 kNN radiance gather, 1920x1080 x 4
+eye samples): photon emission/trace, kd-tree build of both maps, render
+(eye rays, ray-march with a volume kNN query per step, surface kNN query at
+the hit).  Inputs are synthetic (the scene is generated from Philox seed 1).
+
+Prints one JSON line.  `value` = radiance queries (surface + volume kNN
+density estimates) per second of whole-frame time.  `roofline` is for the
+dominant kernel (the render/gather kernel), with its duration measured by HIP
+events on the library's own stream inside libpm.so.  `cpu_baseline` times the
+CPU oracle (same per-thread logic, std::thread over the host cores) on a
+stratified sample of the same frame.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(REPO, "cuda-photon-mapper_amd")
+sys.path.insert(0, PKG)
+
+import numpy as np  # noqa: E402
+
+import pmamd  # noqa: E402
+
+METRIC = "radiance queries/s + ms/frame at 1M photons, k=64; batched/cache vs plain FP32"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload_cfg():
+    return pmamd.preset(2)
+
+
+def workload_desc(c):
+    return ("unit room, %dk surface + %dk volume photon records, %d^3 value-noise medium "
+            "(sigma_s %.2g, sigma_a %.2g, HG g %.2g), k=%d kNN, r<=%.3g, %dx%d x %d spp, march step %.3g, FP32"
+            % (c.cap_surface // 1000, c.cap_volume // 1000, c.noise_res, c.sigma_s, c.sigma_a, c.hg_g, c.k_volume,
+               c.rcap_volume, c.width, c.height, c.spp, c.march_step))
+
+
+def gather_bytes_per_query(k, bf16=False):
+    # k neighbours x (12 B position + 12 B power + 12 B direction) + 36 B of
+    # query I/O (position, normal or view direction, RGB out).  BF16 storage:
+    # power and direction 6 B each.
+    per = 12 + (6 + 6 if bf16 else 12 + 12)
+    return k * per + 36
+
+
+def run_gpu(cfg, steps, warmup):
+    lib = pmamd.load()
+    g = pmamd.PhotonMapper(cfg)
+    npx = cfg.width * cfg.height
+    rgb = pmamd.DeviceBuffer(npx * 12)
+    u8 = pmamd.DeviceBuffer(npx * 4)
+
+    def frame():
+        g.emit()
+        g.build()
+        pmamd.check(lib.pm_render(g.h, rgb.ptr, u8.ptr), "pm_render")
+        return g.stats()
+
+    for i in range(warmup):
+        s = frame()
+        log("warmup %d: emit %.1f ms, build %.1f ms, render %.1f ms" % (i, s.ms_emit, s.ms_build, s.ms_render))
+    per = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        s = frame()
+        per.append((s.ms_emit, s.ms_build, s.ms_render, s.ms_gather_kernel, s.surface_queries, s.volume_queries))
+        log("step %d: emit %.1f ms, build %.1f ms, render %.1f ms" % (i, s.ms_emit, s.ms_build, s.ms_render))
+    t1 = time.perf_counter()
+    img = rgb.download(np.float32, npx * 3)
+    return (t1 - t0) / steps, per, g.stats(), img
+
+
+def run_cpu(cfg, budget_s):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+
+    cores = min(16, len(os.sched_getaffinity(0)))
+    o = O.Oracle(cfg, cores)
+    t = time.perf_counter()
+    o.emit()
+    t_emit = time.perf_counter() - t
+    t = time.perf_counter()
+    o.build()
+    t_build = time.perf_counter() - t
+    # Stratified rows of the same frame until the time budget is spent.
+    H = cfg.height
+    order = []
+    stride = 64
+    while stride >= 1 and len(order) < H:
+        for y in range(stride // 2 if stride > 1 else 0, H, stride):
+            if y not in order:
+                order.append(y)
+        stride //= 2
+    rows = queries = 0
+    t_render = 0.0
+    for y in order:
+        t = time.perf_counter()
+        o.render(0, y, cfg.width, 1)
+        t_render += time.perf_counter() - t
+        rows += 1
+        queries += sum(o.last_queries)
+        if t_render > budget_s:
+            break
+    qps = queries / t_render
+    return dict(qps=qps, cores=cores, rows=rows, queries=queries, t_render=t_render, t_emit=t_emit, t_build=t_build,
+                cpu=platform.processor() or platform.machine())
+
+
+def read_traffic():
+    p = os.path.join(REPO, "profiles", "traffic_render_cfg2.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f)
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X only: this build runs on a single GPU (got --gpus %d)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    cfg = workload_cfg()
+    sec_per_frame, per, stats, img = run_gpu(cfg, args.steps, args.warmup)
+    q_frame = stats.surface_queries + stats.volume_queries
+    value = q_frame / sec_per_frame
+    kern_ms = float(np.mean([p[3] for p in per]))
+    bpq = gather_bytes_per_query(cfg.k_volume, cfg.storage == pmamd.PM_STORE_BF16)
+    achieved = bpq * q_frame / (kern_ms * 1e-3) / 1e9
+    tr = read_traffic()
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": (tr.get("bytes_per_launch") if tr else None),
+            "kernel": "k_render (eye ray + march + kNN gather)", "bytes_per_query": bpq,
+            "queries_per_launch": int(q_frame), "kernel_ms": round(kern_ms, 3)}
+    photons_traced = stats.n_emit_total
+    emit_ms = float(np.mean([p[0] for p in per]))
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "radiance queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec_per_frame * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "cfg2 plain FP32 frame: " + workload_desc(cfg),
+                   "photons_emitted": int(photons_traced), "surface_records": int(stats.n_surface),
+                   "volume_records": int(stats.n_volume), "surface_queries": int(stats.surface_queries),
+                   "volume_queries": int(stats.volume_queries), "sigma_a_used": cfg.sigma_a,
+                   "march_step_used": cfg.march_step},
+        "breakdown_ms": {"emit": round(emit_ms, 3), "build": round(float(np.mean([p[1] for p in per])), 3),
+                         "render": round(float(np.mean([p[2] for p in per])), 3)},
+        "photons_traced_per_s": round(photons_traced / (emit_ms * 1e-3), 1),
+        "roofline": roof,
+        "image_mean_rgb": [round(float(x), 5) for x in img.reshape(-1, 3).mean(0)],
+    }
+    if not args.no_cpu:
+        cb = run_cpu(cfg, args.cpu_seconds)
+        out["cpu_baseline"] = {
+            "value": round(cb["qps"], 1), "unit": "radiance queries/s", "cores": cb["cores"], "kind": "port",
+            "sample": "CPU oracle (g++ -O2, std::thread x %d): full cfg2 emit (%.1f s) and kd-build (%.1f s), then "
+                      "render of %d stratified rows of the 1920x1080x4 frame (%d radiance queries in %.1f s)"
+                      % (cb["cores"], cb["t_emit"], cb["t_build"], cb["rows"], cb["queries"], cb["t_render"]),
+            "est_ms_per_frame": round(q_frame / cb["qps"] * 1e3 + (cb["t_emit"] + cb["t_build"]) * 1e3, 1),
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -520,6 +520,14 @@ int pm_device_count(int* n) {
   return PM_OK;
 }
 
+int pm_copy_kernel(void* dst, const void* src, size_t bytes) {
+  if ((!dst || !src) && bytes) return PM_ERR_INVALID;
+  if (bytes % 16) return PM_ERR_INVALID;
+  HIPCHK(launch_copy16(src, dst, bytes, 0));
+  HIPCHK(hipDeviceSynchronize());
+  return PM_OK;
+}
+
 // ---------------------------------------------------------------- legacy ABI
 void launch_init_random_numbers_kernel(void) {
   if (g_legacy) {

@@ -51,6 +51,7 @@ hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t*
                           hipStream_t st);
 size_t radix_temp_words(size_t n);
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, size_t n, uint32_t* temp, hipStream_t st);
+hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t st);
 
 // trace (pm_trace.hip)
 hipError_t launch_trace_count(const Scene& sc, const TraceParams& tp, uint32_t pid0, uint32_t npid,

@@ -202,4 +202,20 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, size_t n, uint32_t* 
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- copy
+__global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * 256;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t st) {
+  size_t n = bytes / 16;
+  if (n == 0) return hipSuccess;
+  size_t nb = (n + 255) / 256;
+  if (nb > 8192) nb = 8192;
+  hipLaunchKernelGGL(k_copy16, dim3((unsigned)nb), dim3(256), 0, st, (const uint4*)src, (uint4*)dst, n);
+  return hipGetLastError();
+}
+
 }  // namespace pm

@@ -143,6 +143,7 @@ _SIGS = {
     "pm_memcpy_dtoh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
     "pm_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
     "pm_status_string": (ctypes.c_char_p, [ctypes.c_int]),
+    "pm_copy_kernel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
 }
 
 

@@ -174,6 +174,10 @@ int pm_device_free(void* ptr);
 int pm_memcpy_htod(void* dst_dev, const void* src_host, size_t bytes);
 int pm_memcpy_dtoh(void* dst_host, const void* src_dev, size_t bytes);
 int pm_device_count(int* n);
+/* Plain device-to-device copy kernel (16-byte vector loads/stores) on the
+ * library's stream; reads and writes exactly `bytes` (a multiple of 16).  Used
+ * to calibrate hardware memory counters against a known byte count. */
+int pm_copy_kernel(void* dst_dev, const void* src_dev, size_t bytes);
 const char* pm_status_string(int status);
 
 #ifdef __cplusplus

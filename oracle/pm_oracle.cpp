@@ -72,61 +72,93 @@ struct Ctx {
   int nthreads = 1;
 };
 
-// Collects one photon's records in event order.
-struct VecSink {
-  std::vector<float> s, v;  // 9 floats per record
-  PM_HD void surface(V3f p, V3f pw, V3f d) { push(s, p, pw, d); }
-  PM_HD void volume(V3f p, V3f pw, V3f d) { push(v, p, pw, d); }
-  static void push(std::vector<float>& a, V3f p, V3f pw, V3f d) {
-    float r[9] = {p.x, p.y, p.z, pw.x, pw.y, pw.z, d.x, d.y, d.z};
-    a.insert(a.end(), r, r + 9);
+// Collects one photon's records in event order (at most 64 events per photon).
+struct ArraySink {
+  float s[64 * 9], v[64 * 9];
+  int ns = 0, nv = 0;
+  bool keep_s = true, keep_v = true;  // false: count only (the map is already full)
+  void surface(V3f p, V3f pw, V3f d) { put(s, ns, keep_s, p, pw, d); }
+  void volume(V3f p, V3f pw, V3f d) { put(v, nv, keep_v, p, pw, d); }
+  static void put(float* a, int& n, bool keep, V3f p, V3f pw, V3f d) {
+    if (keep) {
+      float* r = a + 9 * n;
+      r[0] = p.x; r[1] = p.y; r[2] = p.z; r[3] = pw.x; r[4] = pw.y; r[5] = pw.z; r[6] = d.x; r[7] = d.y; r[8] = d.z;
+    }
+    ++n;
   }
 };
 
-struct PhotonOut {
+struct alignas(64) ThreadOut {
   std::vector<float> s, v;
+};
+
+struct PhotonOut {
+  std::vector<ThreadOut> per;  // one per thread, in photon-range order
   std::vector<uint32_t> s_cnt, v_cnt;
   std::vector<Tally9> tal;
 };
 
-// Trace photons [p0, p1) with `nt` threads; results are concatenated in photon order.
-void trace_range(const Ctx& c, uint32_t p0, uint32_t p1, int nt, PhotonOut& out) {
+// Trace photons [p0, p1) with `nt` threads; thread t owns a contiguous photon
+// range, so concatenating the per-thread outputs gives photon order.
+void trace_range(const Ctx& c, uint32_t p0, uint32_t p1, int nt, bool keep_s, bool keep_v, PhotonOut& out) {
   const uint32_t n = p1 - p0;
   out.s_cnt.assign(n, 0);
   out.v_cnt.assign(n, 0);
   out.tal.assign(n, Tally9());
-  std::vector<std::vector<float>> ss(nt), vv(nt);
+  out.per.assign((size_t)nt, ThreadOut());
   std::vector<std::thread> th;
   for (int t = 0; t < nt; ++t) {
     th.emplace_back([&, t]() {
       uint32_t a = p0 + (uint32_t)((uint64_t)n * t / nt), b = p0 + (uint32_t)((uint64_t)n * (t + 1) / nt);
+      std::vector<float> s, v;
+      ArraySink* sk = new ArraySink();
+      sk->keep_s = keep_s;
+      sk->keep_v = keep_v;
       for (uint32_t pid = a; pid < b; ++pid) {
-        VecSink sk;
+        sk->ns = sk->nv = 0;
         Tally9 tl;
         tally_zero(tl);
-        trace_photon(c.scene, c.tp, pid, sk, tl);
-        out.s_cnt[pid - p0] = (uint32_t)(sk.s.size() / 9);
-        out.v_cnt[pid - p0] = (uint32_t)(sk.v.size() / 9);
+        trace_photon(c.scene, c.tp, pid, *sk, tl);
+        out.s_cnt[pid - p0] = (uint32_t)sk->ns;
+        out.v_cnt[pid - p0] = (uint32_t)sk->nv;
         out.tal[pid - p0] = tl;
-        ss[t].insert(ss[t].end(), sk.s.begin(), sk.s.end());
-        vv[t].insert(vv[t].end(), sk.v.begin(), sk.v.end());
+        if (keep_s) s.insert(s.end(), sk->s, sk->s + 9 * sk->ns);
+        if (keep_v) v.insert(v.end(), sk->v, sk->v + 9 * sk->nv);
       }
+      delete sk;
+      out.per[(size_t)t].s.swap(s);
+      out.per[(size_t)t].v.swap(v);
     });
   }
   for (auto& x : th) x.join();
-  out.s.clear();
-  out.v.clear();
-  for (int t = 0; t < nt; ++t) {
-    out.s.insert(out.s.end(), ss[t].begin(), ss[t].end());
-    out.v.insert(out.v.end(), vv[t].begin(), vv[t].end());
-  }
 }
 
-void append_records(Map& m, const std::vector<float>& r, size_t first, size_t count) {
-  for (size_t i = first; i < first + count; ++i) {
-    const float* q = &r[i * 9];
+// Record j (in photon order) of a PhotonOut map, walking the per-thread chunks.
+struct RecCursor {
+  const PhotonOut* po;
+  bool vol;
+  size_t chunk = 0, off = 0;
+  const float* next() {
+    for (;;) {
+      const std::vector<float>& a = vol ? po->per[chunk].v : po->per[chunk].s;
+      if (off < a.size()) {
+        const float* r = &a[off];
+        off += 9;
+        return r;
+      }
+      ++chunk;
+      off = 0;
+    }
+  }
+};
+
+// Appends `take` records from the cursor and skips `skip` more.
+void append_records(Map& m, RecCursor& cur, uint64_t take, uint64_t skip) {
+  for (uint64_t i = 0; i < take; ++i) {
+    const float* q = cur.next();
     m.push(v3(q[0], q[1], q[2]), v3(q[3], q[4], q[5]), v3(q[6], q[7], q[8]));
   }
+  for (uint64_t i = 0; i < skip; ++i) cur.next();
 }
 
 // Emission.  n_emit > 0: photons [0, n_emit), every record kept (up to the
@@ -149,15 +181,15 @@ int emit(Ctx& c) {
     uint32_t p1 = fixed ? c.cfg.n_emit : (uint32_t)std::min<uint64_t>((uint64_t)pid + chunk, max_photons);
     if (p1 <= pid) break;
     PhotonOut po;
-    trace_range(c, pid, p1, c.nthreads, po);
-    size_t so = 0, vo = 0;
+    trace_range(c, pid, p1, c.nthreads, !done_s, !done_v, po);
+    RecCursor cs{&po, false}, cv{&po, true};
     for (uint32_t i = 0; i < p1 - pid; ++i) {
       uint32_t ns = po.s_cnt[i], nv = po.v_cnt[i];
       uint32_t id = pid + i;
       if (!done_s) {
         uint64_t room = cap_s - c.surf.size();
         uint64_t take = std::min<uint64_t>(room, ns);
-        append_records(c.surf, po.s, so, take);
+        append_records(c.surf, cs, take, ns - take);
         if (!fixed && c.surf.size() >= cap_s) {
           done_s = true;
           c.surf.n_emit = id + 1;
@@ -166,15 +198,13 @@ int emit(Ctx& c) {
       if (!done_v) {
         uint64_t room = cap_v - c.vol.size();
         uint64_t take = std::min<uint64_t>(room, nv);
-        append_records(c.vol, po.v, vo, take);
+        append_records(c.vol, cv, take, nv - take);
         if (!fixed && c.vol.size() >= cap_v) {
           done_v = true;
           c.vol.n_emit = id + 1;
         }
       }
       for (int k = 0; k < 9; ++k) c.tally.v[k] += po.tal[i].v[k];
-      so += ns;
-      vo += nv;
       c.n_emit_total = id + 1;
       if (!fixed && done_s && done_v) break;
     }
@@ -401,9 +431,11 @@ void render_crop(Ctx& c, int x0, int y0, int w, int h, bool bf16, double* rgb, u
       OracleGather<T> g;
       g.c = &c;
       g.bf16 = bf16;
-      int row;
-      while ((row = next.fetch_add(1)) < h) {
-        for (int i = 0; i < w; ++i) {
+      const int total = w * h, block = 8;
+      int b0;
+      while ((b0 = next.fetch_add(block)) < total) {
+        for (int pi = b0; pi < std::min(total, b0 + block); ++pi) {
+          int row = pi / w, i = pi % w;
           uint32_t px = (uint32_t)(x0 + i), py = (uint32_t)(y0 + row);
           NSum3<T> acc;
           nsum3_init(acc);
