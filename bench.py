@@ -59,9 +59,10 @@ def gather_bytes_per_query(k, bf16=False):
     return k * per + 36
 
 
-def run_gpu(cfg, steps, warmup):
+def run_gpu(cfg, steps, warmup, gather_mode):
     lib = pmamd.load()
     g = pmamd.PhotonMapper(cfg)
+    g.set_option(pmamd.PM_OPT_GATHER_MODE, gather_mode)
     npx = cfg.width * cfg.height
     rgb = pmamd.DeviceBuffer(npx * 12)
     u8 = pmamd.DeviceBuffer(npx * 4)
@@ -140,13 +141,27 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--gather-mode", type=int, default=1, help="0: wave per query, 1: lane per query")
+    ap.add_argument("--frame", type=str, default="", help="WxHxSPP override of the frame (profiling runs only)")
+    ap.add_argument("--copy-calibrate", type=int, default=0,
+                    help="run only the library's copy kernel over this many bytes (memory-counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibrate:
+        n = args.copy_calibrate // 16 * 16
+        src, dst = pmamd.DeviceBuffer(n), pmamd.DeviceBuffer(n)
+        for _ in range(3):
+            pmamd.check(pmamd.load().pm_copy_kernel(dst.ptr, src.ptr, n), "pm_copy_kernel")
+        print(json.dumps({"copy_bytes": n, "launches": 3}))
+        return
     if args.gpus != 1:
         print("bench.py measures one MI355X only: this build runs on a single GPU (got --gpus %d)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     cfg = workload_cfg()
-    sec_per_frame, per, stats, img = run_gpu(cfg, args.steps, args.warmup)
+    if args.frame:
+        w, h, spp = (int(v) for v in args.frame.lower().split("x"))
+        cfg.width, cfg.height, cfg.spp = w, h, spp
+    sec_per_frame, per, stats, img = run_gpu(cfg, args.steps, args.warmup, args.gather_mode)
     q_frame = stats.surface_queries + stats.volume_queries
     value = q_frame / sec_per_frame
     kern_ms = float(np.mean([p[3] for p in per]))

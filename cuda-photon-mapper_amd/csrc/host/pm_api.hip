@@ -45,10 +45,10 @@ struct DevBuf {
 };
 
 struct MapBufs {
-  DevBuf x, y, z, payload, tpts, nodes;
+  DevBuf x, y, z, payload, tpts, nodes, nrange;
   DevMap m;
   void release() {
-    x.release(); y.release(); z.release(); payload.release(); tpts.release(); nodes.release();
+    x.release(); y.release(); z.release(); payload.release(); tpts.release(); nodes.release(); nrange.release();
   }
 };
 
@@ -68,6 +68,8 @@ struct pm_context {
   uint32_t n_emit_total = 0;
   double tally_h[9];
   bool emitted = false, built = false;
+  int gather_mode = 1;
+  int blocks_per_cu = 4;
   pm_stats stats;
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
 };
@@ -124,6 +126,7 @@ int alloc_map(MapBufs& mb, uint32_t n, int bf16, int L) {
   HIPCHK(mb.tpts.ensure(16ull * std::max<uint32_t>(n, 1)));
   uint32_t nl = kd_num_leaves(n, (uint32_t)L);
   HIPCHK(mb.nodes.ensure(sizeof(KdNode) * std::max<uint32_t>(nl, 1)));
+  HIPCHK(mb.nrange.ensure(sizeof(uint2) * std::max<uint32_t>(nl, 1)));
   DevMap& m = mb.m;
   memset(&m, 0, sizeof(m));
   m.n = n;
@@ -137,6 +140,7 @@ int alloc_map(MapBufs& mb, uint32_t n, int bf16, int L) {
   m.nl = nl;
   m.tpts = mb.tpts.as<float4>();
   m.nodes = mb.nodes.as<KdNode>();
+  m.nrange = mb.nrange.as<uint2>();
   m.built = false;
   return PM_OK;
 }
@@ -270,6 +274,7 @@ GatherMap gather_map(const pm_context* c, const DevMap& m, int k, float rcap) {
   memset(&g, 0, sizeof(g));
   g.tpts = m.tpts;
   g.nodes = m.nodes;
+  g.nrange = m.nrange;
   g.payload = m.payload;
   g.n = m.n;
   g.nl = m.nl;
@@ -395,11 +400,13 @@ int pm_render_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev, 
   r.x0 = x0; r.y0 = y0; r.w = w; r.h = h;
   r.rgb = rgb_dev;
   r.u8 = (uchar4*)out_dev;
+  r.blocks_per_cu = c->blocks_per_cu;
+  r.gather_mode = c->gather_mode;
   HIPCHK(c->counters.ensure(16));
   r.counters = c->counters.as<unsigned long long>();
   HIPCHK(hipMemsetAsync(c->counters.p, 0, 16, c->stream));
   HIPCHK(hipEventRecord(c->e0, c->stream));
-  HIPCHK(launch_render(r, c->stream));
+  HIPCHK(r.gather_mode == 1 ? launch_render_lane(r, c->stream) : launch_render(r, c->stream));
   HIPCHK(hipEventRecord(c->e1, c->stream));
   unsigned long long q[2] = {0, 0};
   HIPCHK(hipMemcpyAsync(q, c->counters.p, 16, hipMemcpyDeviceToHost, c->stream));
@@ -414,6 +421,22 @@ int pm_render_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev, 
 int pm_render(pm_context* c, float* rgb_dev, pm_uchar4* out_dev) {
   if (!c) return PM_ERR_INVALID;
   return pm_render_crop(c, 0, 0, c->cfg.width, c->cfg.height, rgb_dev, out_dev);
+}
+
+int pm_set_option(pm_context* c, int option, int value) {
+  if (!c) return PM_ERR_INVALID;
+  switch (option) {
+    case PM_OPT_GATHER_MODE:
+      if (value != 0 && value != 1) return PM_ERR_INVALID;
+      c->gather_mode = value;
+      return PM_OK;
+    case PM_OPT_BLOCKS_PER_CU:
+      if (value < 1 || value > 64) return PM_ERR_INVALID;
+      c->blocks_per_cu = value;
+      return PM_OK;
+    default:
+      return PM_ERR_INVALID;
+  }
 }
 
 int pm_stats_get(pm_context* c, pm_stats* out) {

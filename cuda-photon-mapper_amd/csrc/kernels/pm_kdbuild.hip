@@ -43,7 +43,8 @@ __global__ void k_level_nodes(const float* __restrict__ x, const float* __restri
                               const uint32_t* __restrict__ l2, uint32_t f, uint32_t e,
                               const uint32_t* __restrict__ node_lo, const uint32_t* __restrict__ node_hi,
                               const uint32_t* __restrict__ node_lc, KdNode* __restrict__ nodes,
-                              uint32_t* __restrict__ node_med, uint32_t* __restrict__ node_axis) {
+                              uint2* __restrict__ nrange, uint32_t* __restrict__ node_med,
+                              uint32_t* __restrict__ node_axis) {
   uint32_t h = f + blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= e) return;
   uint32_t lo = node_lo[h], hi = node_hi[h], lc = node_lc[h];
@@ -63,6 +64,7 @@ __global__ void k_level_nodes(const float* __restrict__ x, const float* __restri
   nd.split = coord_of(x, y, z, med, axis);
   nd.meta = (uint32_t)axis | (lc << 2);
   nodes[h] = nd;
+  nrange[h] = make_uint2(lo, hi);
   node_med[h] = med;
   node_axis[h] = (uint32_t)axis;
 }
@@ -270,7 +272,8 @@ hipError_t kd_build_impl(DevMap& m, uint32_t* temp, const uint32_t* host_lo, con
     if (eend > ni) eend = ni;
     unsigned nbn = (unsigned)((eend - f + 255) / 256);
     hipLaunchKernelGGL(k_level_nodes, dim3(nbn), dim3(256), 0, st, m.x, m.y, m.z, lists[0], lists[1], lists[2],
-                       (uint32_t)f, (uint32_t)eend, node_lo, node_hi, node_lc, m.nodes, node_med, node_axis);
+                       (uint32_t)f, (uint32_t)eend, node_lo, node_hi, node_lc, m.nodes, m.nrange, node_med,
+                       node_axis);
     LevelArgs A;
     A.x = m.x; A.y = m.y; A.z = m.z;
     for (int b = 0; b < 3; ++b) { A.lin[b] = lists[b]; A.lout[b] = lists2[b]; }

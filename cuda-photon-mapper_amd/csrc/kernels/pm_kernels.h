@@ -30,6 +30,7 @@ struct DevMap {
   uint32_t L, nl;
   float4* tpts;      // n, tree order
   KdNode* nodes;     // nl - 1
+  uint2* nrange;     // nl - 1: point range [lo, hi) of each internal node
   bool built;
 };
 
@@ -37,6 +38,7 @@ struct DevMap {
 struct GatherMap {
   const float4* tpts;
   const KdNode* nodes;
+  const uint2* nrange;
   const void* payload;
   uint32_t n, nl;
   int bf16;
@@ -83,7 +85,10 @@ struct RenderLaunch {
   float* rgb;      // w*h*3 or null
   uchar4* u8;      // w*h or null
   unsigned long long* counters;  // [surface queries, volume queries]
+  int blocks_per_cu;             // persistent grid: workgroups per compute unit
+  int gather_mode;               // 0: one wave per query, 1: one lane per query
 };
 hipError_t launch_render(const RenderLaunch& r, hipStream_t st);
+hipError_t launch_render_lane(const RenderLaunch& r, hipStream_t st);
 
 }  // namespace pm

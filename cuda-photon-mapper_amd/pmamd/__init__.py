@@ -32,6 +32,7 @@ PM_MEDIUM_NONE, PM_MEDIUM_HOMOGENEOUS, PM_MEDIUM_NOISE = 0, 1, 2
 PM_STORE_FP32, PM_STORE_BF16 = 0, 1
 PM_EST_BOX, PM_EST_CONE = 0, 1
 PM_MAP_SURFACE, PM_MAP_VOLUME = 0, 1
+PM_OPT_GATHER_MODE, PM_OPT_BLOCKS_PER_CU = 1, 2
 
 
 class PmSphere(ctypes.Structure):
@@ -144,6 +145,7 @@ _SIGS = {
     "pm_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
     "pm_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "pm_copy_kernel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+    "pm_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
 }
 
 
@@ -238,6 +240,9 @@ class PhotonMapper:
             self.close()
         except Exception:
             pass
+
+    def set_option(self, option, value):
+        check(load().pm_set_option(self.h, option, value), "pm_set_option")
 
     def emit(self):
         check(load().pm_emit(self.h), "pm_emit")

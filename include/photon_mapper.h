@@ -158,6 +158,13 @@ int pm_stats_get(pm_context* ctx, pm_stats* out);
 int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface, int k_volume,
                   float rcap_surface, float rcap_volume);
 
+/* Execution options (no effect on results, only on how the GPU runs them):
+ * PM_OPT_GATHER_MODE   0 = one wave64 per radiance query (cross-lane sorted list),
+ *                      1 = one lane per query, 8x8 pixel tile per wave (default);
+ * PM_OPT_BLOCKS_PER_CU workgroups per compute unit of the persistent mode-0 grid. */
+enum { PM_OPT_GATHER_MODE = 1, PM_OPT_BLOCKS_PER_CU = 2 };
+int pm_set_option(pm_context* ctx, int option, int value);
+
 /* Inspection (copies to host memory; for tests and tools). */
 int pm_get_records(pm_context* ctx, int map, float* pos_host, float* power_host, float* dir_host, uint32_t n);
 int pm_get_tree(pm_context* ctx, int map, uint32_t* perm_host, float* split_host, uint32_t* axis_host,

@@ -129,8 +129,10 @@ def test_knn_bitexact_preset1(pair1, which, k, rcap):
     assert np.array_equal(bits(gd), bits(od))
 
 
-def test_render_crop_preset1(pair1):
+@pytest.mark.parametrize("mode", [0, 1])
+def test_render_crop_preset1(pair1, mode):
     g, o = pair1
+    g.set_option(pmamd.PM_OPT_GATHER_MODE, mode)
     x0, y0, w, h = 96, 96, 64, 64
     gimg = g.render(x0, y0, w, h)
     oimg = o.render(x0, y0, w, h)
@@ -139,12 +141,14 @@ def test_render_crop_preset1(pair1):
     assert (s.surface_queries, s.volume_queries) == o.last_queries
 
 
-def test_render_cone_estimator(ncpu):
+@pytest.mark.parametrize("mode", [0, 1])
+def test_render_cone_estimator(ncpu, mode):
     c = pmamd.preset(1)
     c.estimator = pmamd.PM_EST_CONE
     c.n_emit = 30000
     g, o = run_pair(c, ncpu)
-    assert_radiance_close(g.render(100, 40, 48, 48), o.render(100, 40, 48, 48))
+    g.set_option(pmamd.PM_OPT_GATHER_MODE, mode)
+    assert_radiance_close(g.render(100, 40, 45, 43), o.render(100, 40, 45, 43))
 
 
 def test_no_medium(ncpu):
@@ -185,16 +189,22 @@ def test_tree_and_knn_preset2(pair2):
         assert np.array_equal(gi, oi) and np.array_equal(bits(gd), bits(od)) and np.array_equal(gc, oc)
 
 
-def test_render_preset2_reduced(pair2):
+@pytest.mark.parametrize("mode", [0, 1])
+def test_render_preset2_reduced(pair2, mode):
     g, o = pair2
+    g.set_option(pmamd.PM_OPT_GATHER_MODE, mode)
     assert_radiance_close(g.render(), o.render())
+    s = g.stats()
+    assert (s.surface_queries, s.volume_queries) == o.last_queries
 
 
-def test_bf16_preset3(ncpu):
+@pytest.mark.parametrize("mode", [0, 1])
+def test_bf16_preset3(ncpu, mode):
     c = pmamd.preset(3)
     c.n_emit = 30000
     c.width, c.height = 96, 54
     g, o = run_pair(c, ncpu)
+    g.set_option(pmamd.PM_OPT_GATHER_MODE, mode)
     assert_records_equal(g, o, 0)
     assert_records_equal(g, o, 1)
     assert_radiance_close(g.render(), o.render(bf16=True))
