@@ -49,6 +49,7 @@ _SIGS = {
     "orc_noise_grid": (ctypes.c_int, [_P, _P, ctypes.c_uint32]),
     "orc_collision": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_int, _P, _P, ctypes.c_float, _P]),
     "orc_kd_leaves_under": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
+    "orc_gather": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, ctypes.c_int, _P]),
 }
 
 
@@ -223,6 +224,15 @@ class Oracle:
         out = np.empty(r ** 3, np.float32)
         assert load().orc_noise_grid(self.h, _ptr(out), out.size) == 0
         return out.reshape(r, r, r)
+
+    def gather(self, which, x, v, albedo=(1.0, 1.0, 1.0), fp64=False):
+        x = np.ascontiguousarray(x, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        a = np.ascontiguousarray(albedo, np.float32)
+        out = np.zeros(3, np.float64)
+        st = load().orc_gather(self.h, which, _ptr(x), _ptr(v), _ptr(a), 1 if fp64 else 0, _ptr(out))
+        assert st == 0, st
+        return out
 
     def collision(self, pid, ev, o, d, tmax):
         o = np.ascontiguousarray(o, np.float32)

@@ -684,4 +684,27 @@ int orc_collision(void* h, uint32_t pid, int ev, const float* o, const float* d,
 
 uint32_t orc_kd_leaves_under(uint32_t c, uint32_t nl) { return kd_leaves_under(c, nl); }
 
+// One radiance estimate (FP64 when fp64 != 0) at x: which 0 = surface with
+// normal v and albedo alb, 1 = volume toward direction v.  out = RGB.
+int orc_gather(void* h, int which, const float* x, const float* v, const float* alb, int fp64, double* out) {
+  Ctx* c = (Ctx*)h;
+  if (!c->surf.built || !c->vol.built) return PM_ERR_STATE;
+  if (fp64) {
+    OracleGather<double> g;
+    g.c = c;
+    g.bf16 = false;
+    V3d xx = v3((double)x[0], (double)x[1], (double)x[2]), vv = v3((double)v[0], (double)v[1], (double)v[2]);
+    V3d r = which == 0 ? g.surface(xx, vv, v3((double)alb[0], (double)alb[1], (double)alb[2])) : g.volume(xx, vv);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+  } else {
+    OracleGather<float> g;
+    g.c = c;
+    g.bf16 = false;
+    V3f xx = v3(x[0], x[1], x[2]), vv = v3(v[0], v[1], v[2]);
+    V3f r = which == 0 ? g.surface(xx, vv, v3(alb[0], alb[1], alb[2])) : g.volume(xx, vv);
+    out[0] = r.x; out[1] = r.y; out[2] = r.z;
+  }
+  return 0;
+}
+
 }  // extern "C"

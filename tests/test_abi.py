@@ -1,0 +1,61 @@
+"""The drop-in boundary: libpm.so loads and exports every entry point that
+include/photon_mapper.h declares (the reference's three launchers,
+photonMappingKernel.cu:1523,1549,1569, plus the pm_* API), and argument
+validation works without a GPU."""
+import ctypes
+import os
+import re
+
+import pmamd
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(REPO, "include", "photon_mapper.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w \*]*?\b((?:pm|launch)_\w+)\s*\(", src, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_reference_launchers():
+    names = declared_functions()
+    for n in ("launch_init_random_numbers_kernel", "launch_emit_photons_kernel", "launch_photon_mapping_kernel"):
+        assert n in names
+    assert len(names) >= 20
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(pmamd.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python binding covers the same set
+    assert sorted(pmamd.exported_symbols()) == declared_functions()
+
+
+def test_presets_and_validation_without_gpu():
+    lib = pmamd.load()
+    for p in (1, 2, 3, 5):
+        c = pmamd.preset(p)
+        assert c.width > 0 and c.k_surface in (32, 64)
+    bad = pmamd.PmConfig()
+    assert lib.pm_config_preset(99, ctypes.byref(bad)) == pmamd.PM_ERR_INVALID
+    h = ctypes.c_void_p()
+    c = pmamd.preset(1)
+    c.k_surface = 65  # above the 64-lane gather limit
+    assert lib.pm_create(ctypes.byref(c), ctypes.byref(h)) == pmamd.PM_ERR_INVALID
+    c = pmamd.preset(1)
+    c.leaf_size = 0
+    assert lib.pm_create(ctypes.byref(c), ctypes.byref(h)) == pmamd.PM_ERR_INVALID
+    c = pmamd.preset(1)
+    c.spheres[0].material = 7
+    assert lib.pm_create(ctypes.byref(c), ctypes.byref(h)) == pmamd.PM_ERR_INVALID
+    assert lib.pm_create(None, ctypes.byref(h)) == pmamd.PM_ERR_INVALID
+    assert lib.pm_emit(None) == pmamd.PM_ERR_INVALID
+    assert lib.pm_status_string(pmamd.PM_ERR_STATE) == b"call order violated"
+
+
+def test_struct_layout_matches_c():
+    # sizeof checks: the ctypes mirror must match the C layout of the header.
+    assert ctypes.sizeof(pmamd.PmSphere) == 36
+    assert ctypes.sizeof(pmamd.PmStats) == 24 + 12 * 8 + 4 * 8 + 16

@@ -225,3 +225,32 @@ def test_legacy_abi_matches_pm_api(lib):
     _, u8 = g.render(want_u8=True)
     assert np.array_equal(legacy, u8)
     assert legacy[..., :3].max() > 0 and (legacy[..., 3] == 0).all()
+
+
+def test_golden_small_scene(ncpu):
+    """libpm.so against the committed oracle fixture (tests/golden/small_scene.npz)."""
+    import importlib.util
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    spec = importlib.util.spec_from_file_location("gen_fixture_small", os.path.join(here, "golden",
+                                                                                    "gen_fixture_small.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    ref = np.load(os.path.join(here, "golden", "small_scene.npz"))
+    g = pmamd.PhotonMapper(mod.small_config())
+    g.emit()
+    g.build()
+    for which, name in ((0, "surface"), (1, "volume")):
+        pos, pw, d = g.records(which)
+        assert np.array_equal(bits(pos), bits(ref[name + "_pos"]))
+        assert np.array_equal(bits(pw), bits(ref[name + "_power"]))
+        assert np.array_equal(bits(d), bits(ref[name + "_dir"]))
+        perm, split, axis = g.tree(which)
+        assert np.array_equal(perm, ref[name + "_perm"]) and np.array_equal(bits(split), bits(ref[name + "_split"]))
+        assert np.array_equal(axis, ref[name + "_axis"])
+        idx, d2, cnt = g.knn(which, ref["queries"], 16, 0.1)
+        assert np.array_equal(idx, ref[name + "_knn_idx"]) and np.array_equal(cnt, ref[name + "_knn_count"])
+        assert np.array_equal(bits(d2), bits(ref[name + "_knn_d2"]))
+    for mode in (0, 1):
+        g.set_option(pmamd.PM_OPT_GATHER_MODE, mode)
+        assert_radiance_close(g.render(), ref["render"])
