@@ -73,6 +73,10 @@ inline int preset_config(int preset, pm_config* c) {
       c->width = 480; c->height = 270;
       return 0;
     case 5:
+      // learned-cache scene: preset 1 with the k = 64 gather as the cache's
+      // training target and fallback
+      c->k_surface = 64;
+      c->k_volume = 64;
       return 0;
     default:
       return -1;

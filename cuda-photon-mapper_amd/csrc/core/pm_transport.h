@@ -210,8 +210,9 @@ PM_HD void march_segment(const Scene& sc, const RenderParams& rp, V3<T> o, V3<T>
       Tj = (T)exp_(-tau);
       tau = tau + (TT)sigma_t_at(m, x) * (TT)ds;
     }
-    V3<T> S = g.volume(x, wo);
-    nsum3_add(L, mul(thr, S) * (Tj * ds));
+    V3<T> w = thr * (Tj * ds);
+    V3<T> S = g.volume(x, wo, w);
+    nsum3_add(L, mul(w, S));
   }
   T Tseg;
   if (m.type == MED_HOMOGENEOUS) Tseg = (T)exp_(-st * (TT)(t1 - t0));
@@ -220,8 +221,11 @@ PM_HD void march_segment(const Scene& sc, const RenderParams& rp, V3<T> o, V3<T>
 }
 
 // One eye sample.  The gather object provides
-//   g.surface(x, n, albedo) -> outgoing radiance of a diffuse point (n faces the viewer)
-//   g.volume(x, wo)         -> in-scattered source term sigma_s * int p L_i at x toward wo
+//   g.surface(x, n, albedo, w, wo) -> outgoing radiance of a diffuse point toward wo (n faces the viewer)
+//   g.volume(x, wo, w)         -> in-scattered source term sigma_s * int p L_i at x toward wo
+// where w is the path weight the returned value is multiplied by (throughput,
+// and for volume points transmittance times the step length); the sample's
+// radiance is the sum of w * value over its queries.
 template <class T, class TT, class G>
 PM_HD V3<T> render_sample_t(const Scene& sc, const RenderParams& rp, uint32_t px, uint32_t py, uint32_t s, G& g) {
   V3<T> o, d;
@@ -248,7 +252,7 @@ PM_HD V3<T> render_sample_t(const Scene& sc, const RenderParams& rp, uint32_t px
     V3<T> nf = front ? n : -n;
     int mat = hit_material(sc, h.kind, h.id);
     if (mat == MAT_DIFFUSE) {
-      V3<T> Ls = g.surface(x, nf, hit_albedo<T>(sc, h.kind, h.id));
+      V3<T> Ls = g.surface(x, nf, hit_albedo<T>(sc, h.kind, h.id), thr, -d);
       nsum3_add(L, mul(thr, Ls));
       break;
     } else if (mat == MAT_MIRROR) {

@@ -12,83 +12,19 @@
 #include <algorithm>
 #include <vector>
 
-#include "../../../include/photon_mapper.h"
-#include "../core/pm_presets.h"
-#include "../kernels/pm_kernels.h"
+#include "pm_context.h"
 
 using namespace pm;
 
-namespace {
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  hipError_t ensure(size_t b) {
-    if (b <= bytes && p) return hipSuccess;
-    if (p) {
-      hipError_t e = hipFree(p);
-      p = nullptr;
-      bytes = 0;
-      if (e != hipSuccess) return e;
-    }
-    size_t want = b ? b : 16;
-    hipError_t e = hipMalloc(&p, want);
-    if (e == hipSuccess) bytes = want;
-    return e;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  template <class T> T* as() const { return (T*)p; }
-};
-
-struct MapBufs {
-  DevBuf x, y, z, payload, tpts, nodes, nrange;
-  DevMap m;
-  void release() {
-    x.release(); y.release(); z.release(); payload.release(); tpts.release(); nodes.release(); nrange.release();
-  }
-};
-
-}  // namespace
-
-struct pm_context {
-  pm_config cfg;
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::vector<Sphere> spheres_h;
-  DevBuf spheres_d, density_d;
-  Scene scene;
-  TraceParams tp;
-  RenderParams rp;
-  MapBufs ms, mv;
-  DevBuf cnt_s, cnt_v, off_s, off_v, scratch, tally_part, tally9, small, counters;
-  uint32_t n_emit_total = 0;
-  double tally_h[9];
-  bool emitted = false, built = false;
-  int gather_mode = 1;
-  int blocks_per_cu = 4;
-  pm_stats stats;
-  hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
-};
-
-namespace {
-
-#define HIPCHK(expr)                          \
-  do {                                        \
-    hipError_t _e = (expr);                   \
-    if (_e != hipSuccess) {                   \
-      set_last_hip(_e, #expr);                \
-      return PM_ERR_HIP;                      \
-    }                                         \
-  } while (0)
-
+namespace pm {
+void cache_release(CacheState& k);
 char g_last_err[256] = "";
 void set_last_hip(hipError_t e, const char* what) {
   snprintf(g_last_err, sizeof(g_last_err), "%s: %s", what, hipGetErrorString(e));
 }
+}  // namespace pm
+
+namespace {
 
 int setup_scene(pm_context* c) {
   c->spheres_h = build_spheres(c->cfg);
@@ -269,6 +205,16 @@ int build_map(pm_context* c, MapBufs& mb) {
   return PM_OK;
 }
 
+GatherMap gather_map(const pm_context* c, const DevMap& m, int k, float rcap);
+
+}  // namespace
+
+GatherMap pm::make_gather_map(const pm_context* c, const DevMap& m, int k, float rcap) {
+  return gather_map(c, m, k, rcap);
+}
+
+namespace {
+
 GatherMap gather_map(const pm_context* c, const DevMap& m, int k, float rcap) {
   GatherMap g;
   memset(&g, 0, sizeof(g));
@@ -341,6 +287,7 @@ int pm_create(const pm_config* cfg, pm_context** out) {
 int pm_destroy(pm_context* c) {
   if (!c) return PM_ERR_INVALID;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  cache_release(c->cache);
   c->ms.release();
   c->mv.release();
   for (DevBuf* b : {&c->spheres_d, &c->density_d, &c->cnt_s, &c->cnt_v, &c->off_s, &c->off_v, &c->scratch,
@@ -433,6 +380,10 @@ int pm_set_option(pm_context* c, int option, int value) {
     case PM_OPT_BLOCKS_PER_CU:
       if (value < 1 || value > 64) return PM_ERR_INVALID;
       c->blocks_per_cu = value;
+      return PM_OK;
+    case PM_OPT_CACHE_FUSED_INFER:
+      if (value != 0 && value != 1) return PM_ERR_INVALID;
+      c->cache.fused_infer = value;
       return PM_OK;
     default:
       return PM_ERR_INVALID;

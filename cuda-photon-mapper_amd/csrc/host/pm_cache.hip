@@ -1,0 +1,418 @@
+// pm_cache.hip -- host side of the learned radiance cache: parameter
+// initialisation, online training (Adam), inference and inspection, behind
+// the pm_cache_* entry points of include/photon_mapper.h.
+//
+// No reference counterpart (the reference has no learned component).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "pm_context.h"
+
+using namespace pm;
+
+namespace {
+
+const uint32_t kChunk = 1u << 20;  // rows per inference / training chunk
+
+size_t w_offset(int l) {
+  size_t o = 0;
+  for (int i = 0; i < l; ++i) o += mlp_w_count(i) + mlp_b_count(i);
+  return o;
+}
+size_t wb_offset(int l) {
+  size_t o = 0;
+  for (int i = 0; i < l; ++i) o += mlp_w_count(i);
+  return o;
+}
+
+// Deterministic initialisation: He-uniform weights from Philox (seed, index),
+// zero biases, zero padding (input columns >= 216, output rows >= 3).
+std::vector<float> init_params(uint32_t seed) {
+  std::vector<float> p(mlp_param_count(), 0.0f);
+  for (int l = 0; l < MLP_LAYERS; ++l) {
+    const int kin = mlp_kin(l), nout = mlp_nout(l);
+    const int kreal = l == 0 ? MLP_FEAT : MLP_HID;
+    const int nreal = l == MLP_HIDDEN_LAYERS ? MLP_OUT : MLP_HID;
+    const float s = l == MLP_HIDDEN_LAYERS ? sqrtf(6.0f / (float)(kreal + nreal)) : sqrtf(6.0f / (float)kreal);
+    float* w = p.data() + w_offset(l);
+    for (int o = 0; o < nout; ++o)
+      for (int i = 0; i < kin; ++i) {
+        if (o >= nreal || i >= kreal) continue;
+        uint32_t idx = (uint32_t)(w_offset(l) + (size_t)o * kin + i);
+        float u = u01(rng(seed, idx, 0u, 0u, TAG_CACHE).x);
+        w[(size_t)o * kin + i] = (2.0f * u - 1.0f) * s;
+      }
+  }
+  return p;
+}
+
+int ensure_cache(pm_context* c) {
+  CacheState& k = c->cache;
+  if (k.ready) return PM_OK;
+  const size_t np = mlp_param_count();
+  size_t nwb = 0;
+  for (int l = 0; l < MLP_LAYERS; ++l) nwb += mlp_w_count(l);
+  HIPCHK(k.params.ensure(4 * np));
+  HIPCHK(k.grads.ensure(4 * np));
+  HIPCHK(k.adam_m.ensure(4 * np));
+  HIPCHK(k.adam_v.ensure(4 * np));
+  HIPCHK(k.params_bf16.ensure(2 * nwb));
+  for (int l = 0; l < MLP_LAYERS; ++l) {
+    k.w.w[l] = k.params.as<float>() + w_offset(l);
+    k.w.b[l] = k.w.w[l] + mlp_w_count(l);
+    k.w.mw[l] = k.adam_m.as<float>() + w_offset(l);
+    k.w.mb[l] = k.w.mw[l] + mlp_w_count(l);
+    k.w.vw[l] = k.adam_v.as<float>() + w_offset(l);
+    k.w.vb[l] = k.w.vw[l] + mlp_w_count(l);
+    k.w.wb[l] = k.params_bf16.as<uint16_t>() + wb_offset(l);
+    k.tb.gw[l] = k.grads.as<float>() + w_offset(l);
+    k.tb.gb[l] = k.tb.gw[l] + mlp_w_count(l);
+  }
+  const uint32_t R = kChunk;
+  HIPCHK(k.X.ensure(2ull * R * MLP_KPAD));
+  for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) HIPCHK(k.H[l].ensure(2ull * R * MLP_HID));
+  HIPCHK(k.y.ensure(4ull * R * MLP_OUTPAD));
+  HIPCHK(k.dy.ensure(2ull * R * MLP_OUTPAD));
+  HIPCHK(k.d0.ensure(2ull * R * MLP_HID));
+  HIPCHK(k.d1.ensure(2ull * R * MLP_HID));
+  HIPCHK(k.row_err.ensure(4ull * R));
+  HIPCHK(k.partial.ensure(4ull * 64 * MLP_HID * MLP_KPAD));
+  k.tb.dy = k.dy.as<uint16_t>();
+  k.tb.d0 = k.d0.as<uint16_t>();
+  k.tb.d1 = k.d1.as<uint16_t>();
+  k.tb.row_err = k.row_err.as<float>();
+  k.tb.partial = k.partial.as<float>();
+  k.rows_cap = R;
+  k.ready = true;
+  return PM_OK;
+}
+
+int upload_params(pm_context* c, const float* host) {
+  CacheState& k = c->cache;
+  HIPCHK(hipMemcpy(k.params.p, host, 4 * mlp_param_count(), hipMemcpyHostToDevice));
+  HIPCHK(mlp_refresh_bf16(k.w, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return PM_OK;
+}
+
+uint16_t* const* hidden_ptrs(CacheState& k, uint16_t** tmp) {
+  for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) tmp[l] = k.H[l].as<uint16_t>();
+  return tmp;
+}
+
+}  // namespace
+
+namespace pm {
+void cache_release(CacheState& k) {
+  for (DevBuf* b : {&k.params, &k.grads, &k.adam_m, &k.adam_v, &k.params_bf16, &k.X, &k.y, &k.dy, &k.d0, &k.d1,
+                    &k.row_err, &k.partial, &k.rgb, &k.err_sum, &k.err_cnt, &k.rec_cnt, &k.rec_off, &k.rec_in,
+                    &k.rec_wt, &k.rec_pix, &k.rec_val, &k.sel_in, &k.sel_t, &k.flags, &k.flag_off, &k.list})
+    b->release();
+  for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) k.H[l].release();
+  k.ready = false;
+}
+
+// Encode + forward + decode for n device input rows, in chunks.
+int cache_predict_dev(pm_context* c, const float* in, uint32_t n, float* rgb) {
+  int r = ensure_cache(c);
+  if (r != PM_OK) return r;
+  CacheState& k = c->cache;
+  if (k.fused_infer) {
+    int maxlds = 0;
+    HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    if ((size_t)maxlds >= mlp_infer_lds_bytes()) {
+      HIPCHK(mlp_infer(k.w, in, n, k.H[0].as<uint16_t>(), rgb, c->stream));
+      return PM_OK;
+    }
+  }
+  uint16_t* Hp[MLP_HIDDEN_LAYERS];
+  Hp[0] = k.H[0].as<uint16_t>();
+  Hp[1] = k.H[1].as<uint16_t>();
+  Hp[2] = k.H[0].as<uint16_t>();
+  Hp[3] = k.H[1].as<uint16_t>();
+  for (uint32_t o = 0; o < n; o += kChunk) {
+    uint32_t rows = std::min(kChunk, n - o);
+    HIPCHK(mlp_encode(in + (size_t)o * MLP_IN, rows, k.X.as<uint16_t>(), c->stream));
+    HIPCHK(mlp_forward(k.w, k.X.as<uint16_t>(), rows, Hp, k.y.as<float>(), c->stream));
+    HIPCHK(mlp_decode(k.y.as<float>(), rows, rgb + (size_t)o * MLP_OUT, c->stream));
+  }
+  return PM_OK;
+}
+
+// One Adam step per `batch` rows of (inputs, targets); returns the mean loss.
+int cache_train_dev(pm_context* c, const float* in, const float* target, uint32_t n, uint32_t batch, float lr,
+                    double* loss, unsigned long long* esum = nullptr, unsigned int* ecnt = nullptr) {
+  int r = ensure_cache(c);
+  if (r != PM_OK) return r;
+  CacheState& k = c->cache;
+  if (batch == 0 || batch > kChunk) return PM_ERR_INVALID;
+  uint16_t* tmp[MLP_HIDDEN_LAYERS];
+  uint16_t* const* H = hidden_ptrs(k, tmp);
+  double total = 0.0;
+  uint64_t cnt = 0;
+  std::vector<float> err(batch);
+  for (uint32_t o = 0; o + batch <= n; o += batch) {
+    HIPCHK(mlp_encode(in + (size_t)o * MLP_IN, batch, k.X.as<uint16_t>(), c->stream));
+    HIPCHK(mlp_forward(k.w, k.X.as<uint16_t>(), batch, H, k.y.as<float>(), c->stream));
+    HIPCHK(mlp_backward(k.w, k.X.as<uint16_t>(), batch, H, k.y.as<float>(), target + (size_t)o * MLP_OUT, k.tb,
+                        c->stream));
+    if (esum) HIPCHK(launch_err_accum(in + (size_t)o * MLP_IN, k.tb.row_err, batch, esum, ecnt, c->stream));
+    k.step += 1;
+    HIPCHK(mlp_adam(k.w, k.tb, lr, k.step, c->stream));
+    HIPCHK(hipMemcpyAsync(err.data(), k.row_err.p, 4ull * batch, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (uint32_t i = 0; i < batch; ++i) total += err[i];
+    cnt += (uint64_t)batch * MLP_OUT;
+  }
+  if (loss) *loss = cnt ? total / (double)cnt : 0.0;
+  return PM_OK;
+}
+}  // namespace pm
+
+extern "C" {
+
+uint32_t pm_cache_param_count(void) { return (uint32_t)mlp_param_count(); }
+
+int pm_cache_reset(pm_context* c, uint32_t seed) {
+  if (!c) return PM_ERR_INVALID;
+  int r = ensure_cache(c);
+  if (r != PM_OK) return r;
+  std::vector<float> p = init_params(seed);
+  HIPCHK(hipMemset(c->cache.adam_m.p, 0, 4 * mlp_param_count()));
+  HIPCHK(hipMemset(c->cache.adam_v.p, 0, 4 * mlp_param_count()));
+  c->cache.step = 0;
+  return upload_params(c, p.data());
+}
+
+int pm_cache_params(pm_context* c, float* host, uint32_t count, int set) {
+  if (!c || !host || count != mlp_param_count()) return PM_ERR_INVALID;
+  int r = ensure_cache(c);
+  if (r != PM_OK) return r;
+  if (set) return upload_params(c, host);
+  HIPCHK(hipMemcpy(host, c->cache.params.p, 4ull * count, hipMemcpyDeviceToHost));
+  return PM_OK;
+}
+
+int pm_cache_grad(pm_context* c, const float* in_dev, const float* target_dev, uint32_t n, float* grad_host,
+                  uint32_t count, double* loss) {
+  if (!c || !in_dev || !target_dev || !grad_host || count != mlp_param_count()) return PM_ERR_INVALID;
+  if (n == 0 || n > kChunk) return PM_ERR_INVALID;
+  int r = ensure_cache(c);
+  if (r != PM_OK) return r;
+  CacheState& k = c->cache;
+  uint16_t* tmp[MLP_HIDDEN_LAYERS];
+  uint16_t* const* H = hidden_ptrs(k, tmp);
+  HIPCHK(mlp_encode(in_dev, n, k.X.as<uint16_t>(), c->stream));
+  HIPCHK(mlp_forward(k.w, k.X.as<uint16_t>(), n, H, k.y.as<float>(), c->stream));
+  HIPCHK(mlp_backward(k.w, k.X.as<uint16_t>(), n, H, k.y.as<float>(), target_dev, k.tb, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(grad_host, k.grads.p, 4ull * count, hipMemcpyDeviceToHost));
+  if (loss) {
+    std::vector<float> err(n);
+    HIPCHK(hipMemcpy(err.data(), k.row_err.p, 4ull * n, hipMemcpyDeviceToHost));
+    double s = 0.0;
+    for (uint32_t i = 0; i < n; ++i) s += err[i];
+    *loss = s / ((double)n * MLP_OUT);
+  }
+  return PM_OK;
+}
+
+int pm_cache_train(pm_context* c, const float* in_dev, const float* target_dev, uint32_t n, uint32_t batch,
+                   float lr, double* loss) {
+  if (!c || !in_dev || !target_dev || !(lr > 0.0f)) return PM_ERR_INVALID;
+  if (!c->cache.ready) {
+    int r = pm_cache_reset(c, c->cfg.seed);
+    if (r != PM_OK) return r;
+  }
+  int r = cache_train_dev(c, in_dev, target_dev, n, batch, lr, loss);
+  if (r == PM_OK) HIPCHK(hipStreamSynchronize(c->stream));
+  return r;
+}
+
+int pm_cache_predict(pm_context* c, const float* in_dev, uint32_t n, float* rgb_dev) {
+  if (!c || (n && (!in_dev || !rgb_dev))) return PM_ERR_INVALID;
+  if (!c->cache.ready) {
+    int r = pm_cache_reset(c, c->cfg.seed);
+    if (r != PM_OK) return r;
+  }
+  int r = cache_predict_dev(c, in_dev, n, rgb_dev);
+  if (r == PM_OK) HIPCHK(hipStreamSynchronize(c->stream));
+  return r;
+}
+
+int pm_cache_configure(pm_context* c, float threshold, uint32_t train_samples, uint32_t batch, float lr,
+                       int march_stride, int cache_surface) {
+  if (!c || !(threshold >= 0.0f) || batch == 0 || batch > kChunk || !(lr > 0.0f)) return PM_ERR_INVALID;
+  if (train_samples % batch || march_stride < 1 || march_stride > 64) return PM_ERR_INVALID;
+  c->cache.stride = march_stride;
+  c->cache.cache_surface = cache_surface ? 1 : 0;
+  c->cache.threshold = threshold;
+  c->cache.train_samples = train_samples;
+  c->cache.batch = batch;
+  c->cache.lr = lr;
+  return PM_OK;
+}
+
+int pm_cache_stats_get(pm_context* c, pm_cache_stats* out) {
+  if (!c || !out) return PM_ERR_INVALID;
+  *out = c->cache.st;
+  return PM_OK;
+}
+
+static double ev_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.0f;
+  return hipEventElapsedTime(&ms, a, b) == hipSuccess ? (double)ms : -1.0;
+}
+
+int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev, pm_uchar4* out_dev) {
+  if (!c) return PM_ERR_INVALID;
+  if (!c->built) return PM_ERR_STATE;
+  if (w <= 0 || h <= 0 || x0 < 0 || y0 < 0 || x0 + w > c->cfg.width || y0 + h > c->cfg.height) return PM_ERR_INVALID;
+  CacheState& k = c->cache;
+  if (!k.ready) {
+    int r = pm_cache_reset(c, c->cfg.seed);
+    if (r != PM_OK) return r;
+  }
+  hipStream_t st = c->stream;
+  hipEvent_t ev[7];
+  for (int i = 0; i < 7; ++i) HIPCHK(hipEventCreate(&ev[i]));
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() {
+      for (int i = 0; i < 7; ++i) (void)hipEventDestroy(e[i]);
+    }
+  } guard{ev};
+  const uint32_t npix = (uint32_t)w * (uint32_t)h;
+  HIPCHK(hipEventRecord(ev[0], st));
+  // 1-3: record the frame's queries in pixel order
+  HIPCHK(k.rec_cnt.ensure(4ull * npix));
+  HIPCHK(k.rec_off.ensure(4ull * npix));
+  HIPCHK(c->scratch.ensure(4ull * scan_temp_words(npix)));
+  CacheRecordLaunch rl;
+  memset(&rl, 0, sizeof(rl));
+  rl.sc = c->scene;
+  rl.rp = c->rp;
+  rl.x0 = x0; rl.y0 = y0; rl.w = w; rl.h = h;
+  rl.counts = k.rec_cnt.as<uint32_t>();
+  rl.write = 0;
+  rl.stride = k.stride;
+  HIPCHK(launch_cache_record(rl, st));
+  HIPCHK(exclusive_scan(k.rec_cnt.as<uint32_t>(), k.rec_off.as<uint32_t>(), npix, c->scratch.as<uint32_t>(), nullptr,
+                        st));
+  std::vector<uint32_t> cnt(npix);
+  HIPCHK(hipMemcpyAsync(cnt.data(), k.rec_cnt.p, 4ull * npix, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  uint64_t N = 0;
+  for (uint32_t i = 0; i < npix; ++i) N += cnt[i];
+  if (N >= 0xffffffffull) return PM_ERR_CAPACITY;
+  HIPCHK(k.rec_in.ensure(4ull * MLP_IN * std::max<uint64_t>(N, 1)));
+  HIPCHK(k.rec_wt.ensure(12ull * std::max<uint64_t>(N, 1)));
+  HIPCHK(k.rec_pix.ensure(4ull * std::max<uint64_t>(N, 1)));
+  HIPCHK(k.rec_val.ensure(12ull * std::max<uint64_t>(N, 1)));
+  rl.write = 1;
+  rl.offsets = k.rec_off.as<uint32_t>();
+  rl.in = k.rec_in.as<float>();
+  rl.wt = k.rec_wt.as<float>();
+  rl.pix = k.rec_pix.as<uint32_t>();
+  HIPCHK(launch_cache_record(rl, st));
+  HIPCHK(hipEventRecord(ev[1], st));
+  // 4: online training on a subset with exact kNN targets
+  GatherMap gs = make_gather_map(c, c->ms.m, c->cfg.k_surface, c->cfg.rcap_surface);
+  GatherMap gv = make_gather_map(c, c->mv.m, c->cfg.k_volume, c->cfg.rcap_volume);
+  const uint32_t ncell = 2u * CACHE_GRID * CACHE_GRID * CACHE_GRID;
+  HIPCHK(k.err_sum.ensure(8ull * ncell));
+  HIPCHK(k.err_cnt.ensure(4ull * ncell));
+  HIPCHK(hipMemsetAsync(k.err_sum.p, 0, 8ull * ncell, st));
+  HIPCHK(hipMemsetAsync(k.err_cnt.p, 0, 4ull * ncell, st));
+  double loss = 0.0;
+  const uint32_t S = N > 0 ? k.train_samples : 0u;
+  if (S > 0) {
+    HIPCHK(k.sel_in.ensure(4ull * MLP_IN * S));
+    HIPCHK(k.sel_t.ensure(12ull * S));
+    // index lists of surface and volume records (in record order)
+    HIPCHK(k.flags.ensure(4ull * std::max<uint64_t>(N, 1)));
+    HIPCHK(k.flag_off.ensure(4ull * std::max<uint64_t>(N, 1)));
+    HIPCHK(k.list.ensure(8ull * std::max<uint64_t>(N, 1)));
+    HIPCHK(c->scratch.ensure(4ull * scan_temp_words(std::max<uint64_t>(N, npix))));
+    HIPCHK(c->small.ensure(64));
+    uint32_t* list_s = k.list.as<uint32_t>();
+    uint32_t* list_v = list_s + N;
+    for (int kind = 0; kind < 2; ++kind) {
+      HIPCHK(launch_kind_flags(k.rec_in.as<float>(), N, kind == 0, k.flags.as<uint32_t>(), st));
+      HIPCHK(exclusive_scan(k.flags.as<uint32_t>(), k.flag_off.as<uint32_t>(), N, c->scratch.as<uint32_t>(),
+                            c->small.as<uint32_t>() + 10 + kind, st));
+      HIPCHK(launch_cache_compact(k.flags.as<uint32_t>(), k.flag_off.as<uint32_t>(), N, kind == 0 ? list_s : list_v,
+                                  st));
+    }
+    uint32_t nk[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(nk, c->small.as<uint32_t>() + 10, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    // without surface caching the subset is drawn from the volume records only
+    const uint32_t ns_sel = k.cache_surface ? nk[0] : 0u;
+    HIPCHK(launch_cache_select(k.rec_in.as<float>(), list_s, ns_sel, list_v, nk[1], S, c->cfg.seed, k.frame,
+                               k.sel_in.as<float>(), st));
+    CacheValueLaunch vl;
+    memset(&vl, 0, sizeof(vl));
+    vl.gs = gs; vl.gv = gv; vl.hg_g = c->scene.med.g;
+    vl.in = k.sel_in.as<float>();
+    vl.n = S;
+    vl.out = k.sel_t.as<float>();
+    HIPCHK(launch_cache_values(vl, st));
+    int r = cache_train_dev(c, k.sel_in.as<float>(), k.sel_t.as<float>(), S, k.batch, k.lr, &loss,
+                            k.err_sum.as<unsigned long long>(), k.err_cnt.as<unsigned int>());
+    if (r != PM_OK) return r;
+  }
+  HIPCHK(hipEventRecord(ev[2], st));
+  // 5: inference over every record
+  int r = cache_predict_dev(c, k.rec_in.as<float>(), (uint32_t)N, k.rec_val.as<float>());
+  if (r != PM_OK) return r;
+  HIPCHK(hipEventRecord(ev[3], st));
+  // 6: exact fallback where the cache's cell error is high
+  HIPCHK(k.flags.ensure(4ull * std::max<uint64_t>(N, 1)));
+  HIPCHK(k.flag_off.ensure(4ull * std::max<uint64_t>(N, 1)));
+  HIPCHK(k.list.ensure(8ull * std::max<uint64_t>(N, 1)));
+  HIPCHK(c->scratch.ensure(4ull * scan_temp_words(std::max<uint64_t>(N, npix))));
+  HIPCHK(launch_cache_flags(k.rec_in.as<float>(), N, k.err_sum.as<unsigned long long>(), k.err_cnt.as<unsigned int>(),
+                            k.threshold, k.cache_surface, k.flags.as<uint32_t>(), st));
+  HIPCHK(c->small.ensure(64));
+  HIPCHK(exclusive_scan(k.flags.as<uint32_t>(), k.flag_off.as<uint32_t>(), N, c->scratch.as<uint32_t>(),
+                        c->small.as<uint32_t>() + 8, st));
+  HIPCHK(launch_cache_compact(k.flags.as<uint32_t>(), k.flag_off.as<uint32_t>(), N, k.list.as<uint32_t>(), st));
+  uint32_t nfb = 0;
+  HIPCHK(hipMemcpyAsync(&nfb, c->small.as<uint32_t>() + 8, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (nfb) {
+    CacheValueLaunch vl;
+    memset(&vl, 0, sizeof(vl));
+    vl.gs = gs; vl.gv = gv; vl.hg_g = c->scene.med.g;
+    vl.in = k.rec_in.as<float>();
+    vl.list = k.list.as<uint32_t>();
+    vl.n = nfb;
+    vl.out = k.rec_val.as<float>();
+    vl.out_by_list = 1;
+    HIPCHK(launch_cache_values(vl, st));
+  }
+  HIPCHK(hipEventRecord(ev[4], st));
+  // 7: per-pixel accumulation
+  HIPCHK(launch_cache_accumulate(k.rec_off.as<uint32_t>(), k.rec_cnt.as<uint32_t>(), npix, k.rec_wt.as<float>(),
+                                 k.rec_val.as<float>(), c->rp.spp, c->cfg.exposure, rgb_dev, (uchar4*)out_dev, st));
+  HIPCHK(hipEventRecord(ev[5], st));
+  HIPCHK(hipStreamSynchronize(st));
+  k.frame += 1;
+  k.st.records = N;
+  k.st.fallbacks = nfb;
+  k.st.train_loss = loss;
+  k.st.ms_record = ev_ms(ev[0], ev[1]);
+  k.st.ms_train = ev_ms(ev[1], ev[2]);
+  k.st.ms_infer = ev_ms(ev[2], ev[3]);
+  k.st.ms_fallback = ev_ms(ev[3], ev[4]);
+  k.st.ms_accumulate = ev_ms(ev[4], ev[5]);
+  k.st.ms_total = ev_ms(ev[0], ev[5]);
+  k.st.frame = k.frame;
+  return PM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+// pm_context.h -- the library context behind the C ABI (internal to libpm.so).
+#pragma once
+
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../../include/photon_mapper.h"
+#include "../core/pm_presets.h"
+#include "../kernels/pm_kernels.h"
+#include "../kernels/pm_mlp.h"
+
+struct pm_context;
+
+namespace pm {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t b) {
+    if (b <= bytes && p) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      p = nullptr;
+      bytes = 0;
+      if (e != hipSuccess) return e;
+    }
+    size_t want = b ? b : 16;
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) bytes = want;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return (T*)p; }
+};
+
+struct MapBufs {
+  DevBuf x, y, z, payload, tpts, nodes, nrange;
+  DevMap m;
+  void release() {
+    x.release(); y.release(); z.release(); payload.release(); tpts.release(); nodes.release(); nrange.release();
+  }
+};
+
+// Learned radiance cache state: parameters, Adam moments, training scratch.
+struct CacheState {
+  DevBuf params, grads, adam_m, adam_v, params_bf16;
+  DevBuf X, H[MLP_HIDDEN_LAYERS], y, dy, d0, d1, row_err, partial, rgb;
+  MlpDev w;
+  MlpTrainBufs tb;
+  uint32_t rows_cap = 0;
+  int step = 0;
+  bool ready = false;
+  // per-cell error tracker (fallback to the plain gather where it is high)
+  DevBuf err_sum, err_cnt;
+  // cache render configuration and per-frame buffers
+  float threshold = 0.02f;
+  uint32_t train_samples = 1u << 18;
+  uint32_t batch = 1u << 14;
+  float lr = 1e-3f;
+  uint32_t frame = 0;
+  int fused_infer = 1;
+  int stride = 1;         // march points per volume record
+  int cache_surface = 0;  // 0: surface queries are always gathered exactly  // 1: pm_mlp_infer.hip kernels (when the LDS fits), 0: generic GEMMs
+  DevBuf rec_cnt, rec_off, rec_in, rec_wt, rec_pix, rec_val, sel_in, sel_t, flags, flag_off, list;
+  pm_cache_stats st;
+};
+
+void set_last_hip(hipError_t e, const char* what);
+GatherMap make_gather_map(const pm_context* c, const DevMap& m, int k, float rcap);
+extern char g_last_err[256];
+
+#define HIPCHK(expr)                          \
+  do {                                        \
+    hipError_t _e = (expr);                   \
+    if (_e != hipSuccess) {                   \
+      pm::set_last_hip(_e, #expr);            \
+      return PM_ERR_HIP;                      \
+    }                                         \
+  } while (0)
+
+}  // namespace pm
+
+struct pm_context {
+  pm_config cfg;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<pm::Sphere> spheres_h;
+  pm::DevBuf spheres_d, density_d;
+  pm::Scene scene;
+  pm::TraceParams tp;
+  pm::RenderParams rp;
+  pm::MapBufs ms, mv;
+  pm::DevBuf cnt_s, cnt_v, off_s, off_v, scratch, tally_part, tally9, small, counters;
+  uint32_t n_emit_total = 0;
+  double tally_h[9];
+  bool emitted = false, built = false;
+  int gather_mode = 1;
+  int blocks_per_cu = 4;
+  pm_stats stats;
+  hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+  pm::CacheState cache;
+};

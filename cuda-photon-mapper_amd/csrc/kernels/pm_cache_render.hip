@@ -1,0 +1,293 @@
+// pm_cache_render.hip -- render with the learned radiance cache on gfx950.
+//
+// Same eye paths as the plain render (render_sample, shared with the oracle),
+// but every radiance query is first recorded instead of gathered:
+//   1. k_rec (count)  : per pixel, the number of queries of its samples
+//   2. scan           : per-pixel record offsets (records stay in pixel order)
+//   3. k_rec (write)  : query inputs (position, direction, normal or 0) and
+//                       path weights (throughput x transmittance x step, with
+//                       the surface albedo folded in)
+//   4. training       : a Philox-chosen subset of the records gets exact kNN
+//                       targets (k_values) and trains the MLP (pm_mlp.hip);
+//                       each batch's pre-update errors are binned into a
+//                       16^3 x {surface, volume} grid (k_err_accum)
+//   5. inference      : MLP over all records (pm_mlp.hip)
+//   6. fallback       : records in cells whose mean training error exceeds
+//                       the threshold (or that saw no training sample) are
+//                       re-gathered exactly with kNN (k_flags, k_values)
+//   7. k_accumulate   : per pixel, compensated sum of weight x value / spp.
+// No reference counterpart (the reference has no cache); the plain path it
+// replaces is photon_mapping_kernel, photonMappingKernel.cu:1409-1462.
+#include "pm_lane_knn.h"
+#include "pm_mlp.h"
+
+namespace pm {
+
+// Recording gather: stores each query and its weight, returns zero radiance.
+// With march stride s > 1, one volume record stands for up to s consecutive
+// march points of the same segment (the first point's position, the sum of
+// their weights); a point farther than 1.5 s steps from the open record, or a
+// surface query, starts a new one.
+struct RecordGather {
+  bool write;
+  uint32_t n;        // records of this lane so far
+  uint32_t base;     // first record index of this pixel (write pass)
+  uint32_t pixel;    // crop pixel index
+  float* in;
+  float* wt;
+  uint32_t* pix;
+  int stride;
+  float reach2;      // (1.5 * stride * step)^2
+  int open;          // points merged into the open volume record (0: none)
+  V3f open_x;
+  __device__ void put(V3f x, V3f d, V3f nrm, V3f w) {
+    if (write) {
+      size_t i = (size_t)base + n;
+      float* p = in + i * MLP_IN;
+      p[0] = x.x; p[1] = x.y; p[2] = x.z;
+      p[3] = d.x; p[4] = d.y; p[5] = d.z;
+      p[6] = nrm.x; p[7] = nrm.y; p[8] = nrm.z;
+      wt[3 * i] = w.x; wt[3 * i + 1] = w.y; wt[3 * i + 2] = w.z;
+      pix[i] = pixel;
+    }
+    ++n;
+  }
+  __device__ V3f surface(V3f x, V3f nrm, V3f albedo, V3f w, V3f wo) {
+    open = 0;
+    put(x, wo, nrm, mul(w, albedo));
+    return v3(0.0f, 0.0f, 0.0f);
+  }
+  __device__ V3f volume(V3f x, V3f wo, V3f w) {
+    if (open > 0 && open < stride) {
+      float dx = x.x - open_x.x, dy = x.y - open_x.y, dz = x.z - open_x.z;
+      if ((dx * dx + dy * dy) + dz * dz <= reach2) {
+        if (write) {
+          size_t i = (size_t)base + n - 1u;
+          wt[3 * i] += w.x;
+          wt[3 * i + 1] += w.y;
+          wt[3 * i + 2] += w.z;
+        }
+        ++open;
+        return v3(0.0f, 0.0f, 0.0f);
+      }
+    }
+    put(x, wo, v3(0.0f, 0.0f, 0.0f), w);
+    open = 1;
+    open_x = x;
+    return v3(0.0f, 0.0f, 0.0f);
+  }
+};
+
+// One lane per crop pixel, 8x8 tiles per wave (same pixel order as the lane render).
+template <bool FP64T>
+__global__ void __launch_bounds__(64) k_rec(CacheRecordLaunch r) {
+  const int lane = lane_id();
+  const int tiles_x = (r.w + 7) / 8;
+  const int tile = (int)blockIdx.x;
+  const int lx = (tile % tiles_x) * 8 + (lane & 7), ly = (tile / tiles_x) * 8 + (lane >> 3);
+  if (lx >= r.w || ly >= r.h) return;
+  const uint32_t pidx = (uint32_t)ly * (uint32_t)r.w + (uint32_t)lx;
+  RecordGather g;
+  g.write = r.write != 0;
+  g.n = 0;
+  g.base = r.write ? r.offsets[pidx] : 0u;
+  g.pixel = pidx;
+  g.in = r.in;
+  g.wt = r.wt;
+  g.pix = r.pix;
+  g.stride = r.stride > 0 ? r.stride : 1;
+  const float reach = 1.5f * (float)g.stride * r.rp.march_step;
+  g.reach2 = reach * reach;
+  g.open = 0;
+  const uint32_t px = (uint32_t)(r.x0 + lx), py = (uint32_t)(r.y0 + ly);
+  for (int s = 0; s < r.rp.spp; ++s) {
+    if (FP64T) (void)render_sample_t<float, double>(r.sc, r.rp, px, py, (uint32_t)s, g);
+    else (void)render_sample_t<float, float>(r.sc, r.rp, px, py, (uint32_t)s, g);
+  }
+  if (!r.write) r.counts[pidx] = g.n;
+}
+
+// Exact kNN value (albedo-free for surfaces) of records list[i] (or i).
+__global__ void __launch_bounds__(64) k_values(CacheValueLaunch v) {
+  extern __shared__ uint32_t heap_lds[];
+  const int lane = lane_id();
+  const uint32_t i = blockIdx.x * 64u + (uint32_t)lane;
+  if (i >= v.n) return;
+  const uint32_t rec = v.list ? v.list[i] : i;
+  const float* p = v.in + (size_t)rec * MLP_IN;
+  LaneGather<false> g;
+  g.gs = v.gs;
+  g.gv = v.gv;
+  g.hp.H = heap_lds;
+  g.hp.Lo = heap_lds + 64 * 64;
+  g.hp.lane = lane;
+  g.hp.k = 0;
+  g.hp.size = 0;
+  g.hg_g = v.hg_g;
+  g.nsurf = g.nvol = 0u;
+  g.prev_r = -1.0f;
+  V3f x = v3(p[0], p[1], p[2]), d = v3(p[3], p[4], p[5]), nrm = v3(p[6], p[7], p[8]);
+  const bool surf = !(nrm.x == 0.0f && nrm.y == 0.0f && nrm.z == 0.0f);
+  V3f val = surf ? g.surface(x, nrm, v3(1.0f, 1.0f, 1.0f), v3(1.0f, 1.0f, 1.0f), d) : g.volume(x, d, v3(1.0f, 1.0f, 1.0f));
+  const size_t o = v.out_by_list ? (size_t)rec : (size_t)i;
+  v.out[3 * o] = val.x;
+  v.out[3 * o + 1] = val.y;
+  v.out[3 * o + 2] = val.z;
+}
+
+// Training subset, stratified by query kind: rows j < count/2 are drawn from
+// the surface records (list_s, n_s of them), the rest from the volume records
+// (list_v, n_v), each uniformly with a Philox draw (surface queries are few
+// per pixel but carry most of the image).  An empty kind gives its share to
+// the other.
+__global__ void k_select(const float* __restrict__ in, const uint32_t* __restrict__ list_s, uint32_t n_s,
+                         const uint32_t* __restrict__ list_v, uint32_t n_v, uint32_t count, uint32_t seed,
+                         uint32_t frame, float* __restrict__ out) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  U4 r = rng(seed, j, frame, 0u, TAG_CACHE);
+  uint64_t u = ((uint64_t)r.x << 32) | r.y;
+  bool surf = (j < count / 2u && n_s > 0u) || n_v == 0u;
+  const uint32_t* list = surf ? list_s : list_v;
+  uint64_t n = surf ? n_s : n_v;
+  uint64_t idx = list[(uint64_t)(((unsigned __int128)u * n) >> 64)];
+  for (int c = 0; c < MLP_IN; ++c) out[(size_t)j * MLP_IN + c] = in[idx * MLP_IN + c];
+}
+
+// flags[i] = 1 for surface records (non-zero normal).
+__global__ void k_kind_flags(const float* __restrict__ in, uint64_t n, int want_surface, uint32_t* __restrict__ flags) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* p = in + i * MLP_IN;
+  bool surf = !(p[6] == 0.0f && p[7] == 0.0f && p[8] == 0.0f);
+  flags[i] = (surf == (want_surface != 0)) ? 1u : 0u;
+}
+
+__device__ __forceinline__ uint32_t err_cell(const float* p) {
+  int c[3];
+  for (int a = 0; a < 3; ++a) {
+    float v = p[a] * (float)CACHE_GRID;
+    int iv = (int)v;
+    c[a] = iv < 0 ? 0 : (iv >= CACHE_GRID ? CACHE_GRID - 1 : iv);
+  }
+  const bool surf = !(p[6] == 0.0f && p[7] == 0.0f && p[8] == 0.0f);
+  return (uint32_t)((((surf ? 0 : 1) * CACHE_GRID + c[2]) * CACHE_GRID + c[1]) * CACHE_GRID + c[0]);
+}
+
+// Bins per-row squared errors; integer (fixed-point 2^-24) sums are order-independent.
+__global__ void k_err_accum(const float* __restrict__ in, const float* __restrict__ row_err, uint32_t rows,
+                            unsigned long long* __restrict__ esum, unsigned int* __restrict__ ecnt) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  uint32_t cell = err_cell(in + (size_t)i * MLP_IN);
+  float e = row_err[i];
+  e = e < 1e6f ? e : 1e6f;
+  atomicAdd(&esum[cell], (unsigned long long)(e * 16777216.0f));
+  atomicAdd(&ecnt[cell], 1u);
+}
+
+__global__ void k_flags(const float* __restrict__ in, uint64_t n, const unsigned long long* __restrict__ esum,
+                        const unsigned int* __restrict__ ecnt, float threshold, int cache_surface,
+                        uint32_t* __restrict__ flags) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* p = in + i * MLP_IN;
+  const bool surf = !(p[6] == 0.0f && p[7] == 0.0f && p[8] == 0.0f);
+  uint32_t cell = err_cell(p);
+  uint32_t c = ecnt[cell];
+  bool fb = (surf && !cache_surface) || c == 0u || (double)esum[cell] / 16777216.0 / (double)c > (double)threshold;
+  flags[i] = fb ? 1u : 0u;
+}
+
+__global__ void k_compact(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ off, uint64_t n,
+                          uint32_t* __restrict__ list) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && flags[i]) list[off[i]] = (uint32_t)i;
+}
+
+// Per pixel: compensated sum over its records of weight * value, / spp.
+__global__ void k_accumulate(const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t npix,
+                             const float* __restrict__ wt, const float* __restrict__ val, int spp, float exposure,
+                             float* __restrict__ rgb, uchar4* __restrict__ u8) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  NSum3<float> acc;
+  nsum3_init(acc);
+  const size_t b = off[p], e = b + cnt[p];
+  for (size_t i = b; i < e; ++i)
+    nsum3_add(acc, mul(v3(wt[3 * i], wt[3 * i + 1], wt[3 * i + 2]), v3(val[3 * i], val[3 * i + 1], val[3 * i + 2])));
+  V3f v = nsum3_get(acc) * (1.0f / (float)spp);
+  if (rgb) {
+    rgb[3 * (size_t)p] = v.x;
+    rgb[3 * (size_t)p + 1] = v.y;
+    rgb[3 * (size_t)p + 2] = v.z;
+  }
+  if (u8) {
+    uchar4 q;
+    q.x = tonemap_u8(v.x, exposure);
+    q.y = tonemap_u8(v.y, exposure);
+    q.z = tonemap_u8(v.z, exposure);
+    q.w = 0;
+    u8[p] = q;
+  }
+}
+
+hipError_t launch_cache_record(const CacheRecordLaunch& r, hipStream_t st) {
+  unsigned tiles = (unsigned)(((r.w + 7) / 8) * ((r.h + 7) / 8));
+  if (tiles == 0) return hipSuccess;
+  if (r.rp.fp64_transmittance) hipLaunchKernelGGL((k_rec<true>), dim3(tiles), dim3(64), 0, st, r);
+  else hipLaunchKernelGGL((k_rec<false>), dim3(tiles), dim3(64), 0, st, r);
+  return hipGetLastError();
+}
+
+hipError_t launch_cache_values(const CacheValueLaunch& v, hipStream_t st) {
+  if (v.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_values, dim3((v.n + 63) / 64), dim3(64), 2 * 64 * 64 * sizeof(uint32_t), st, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_cache_select(const float* in, const uint32_t* list_s, uint32_t n_s, const uint32_t* list_v,
+                               uint32_t n_v, uint32_t count, uint32_t seed, uint32_t frame, float* out, hipStream_t st) {
+  if (count == 0 || (n_s == 0 && n_v == 0)) return hipSuccess;
+  hipLaunchKernelGGL(k_select, dim3((count + 255) / 256), dim3(256), 0, st, in, list_s, n_s, list_v, n_v, count, seed,
+                     frame, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_kind_flags(const float* in, uint64_t n, int want_surface, uint32_t* flags, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_kind_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, want_surface, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_err_accum(const float* in, const float* row_err, uint32_t rows, unsigned long long* esum,
+                            unsigned int* ecnt, hipStream_t st) {
+  if (rows == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_err_accum, dim3((rows + 255) / 256), dim3(256), 0, st, in, row_err, rows, esum, ecnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_cache_flags(const float* in, uint64_t n, const unsigned long long* esum, const unsigned int* ecnt,
+                              float threshold, int cache_surface, uint32_t* flags, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, esum, ecnt, threshold,
+                     cache_surface, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_cache_compact(const uint32_t* flags, const uint32_t* off, uint64_t n, uint32_t* list,
+                                hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_compact, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, flags, off, n, list);
+  return hipGetLastError();
+}
+
+hipError_t launch_cache_accumulate(const uint32_t* off, const uint32_t* cnt, uint32_t npix, const float* wt,
+                                   const float* val, int spp, float exposure, float* rgb, uchar4* u8, hipStream_t st) {
+  if (npix == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_accumulate, dim3((npix + 255) / 256), dim3(256), 0, st, off, cnt, npix, wt, val, spp, exposure,
+                     rgb, u8);
+  return hipGetLastError();
+}
+
+}  // namespace pm

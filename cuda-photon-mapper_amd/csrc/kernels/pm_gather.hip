@@ -180,7 +180,7 @@ struct WaveGather {
   V3f prev_x;
   float prev_r;  // < 0: none
 
-  __device__ V3f surface(V3f x, V3f n, V3f albedo) {
+  __device__ V3f surface(V3f x, V3f n, V3f albedo, V3f = V3f(), V3f = V3f()) {
     ++nsurf;
     uint32_t lh, ll;
     uint32_t m = wave_knn(gs, lds_s, nlds_s, x.x, x.y, x.z, f2u(gs.rcap2), lh, ll);
@@ -202,7 +202,7 @@ struct WaveGather {
     return mul(mul(sum, albedo), sc) * surface_norm(r2, CONE);
   }
 
-  __device__ V3f volume(V3f x, V3f wo) {
+  __device__ V3f volume(V3f x, V3f wo, V3f = V3f()) {
     ++nvol;
     uint32_t lh, ll;
     uint32_t caph = f2u(gv.rcap2);

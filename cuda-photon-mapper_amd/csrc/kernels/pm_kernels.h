@@ -91,4 +91,41 @@ struct RenderLaunch {
 hipError_t launch_render(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane(const RenderLaunch& r, hipStream_t st);
 
+// learned-cache render (pm_cache_render.hip)
+enum : int { CACHE_GRID = 16 };  // error-tracker cells per axis (x 2 kinds)
+struct CacheRecordLaunch {
+  Scene sc;
+  RenderParams rp;
+  int x0, y0, w, h;
+  int write;                 // 0: count queries per pixel, 1: write records
+  uint32_t* counts;          // [w*h]
+  const uint32_t* offsets;   // [w*h]
+  float* in;                 // [N x 9]
+  float* wt;                 // [N x 3]
+  uint32_t* pix;             // [N]
+  int stride;                // march points per volume record
+};
+struct CacheValueLaunch {
+  GatherMap gs, gv;
+  float hg_g;
+  const float* in;           // query rows (9 floats)
+  const uint32_t* list;      // rows to evaluate (null: 0..n-1)
+  uint32_t n;
+  float* out;                // values (3 floats)
+  int out_by_list;           // 1: out[list[i]], 0: out[i]
+};
+hipError_t launch_cache_record(const CacheRecordLaunch& r, hipStream_t st);
+hipError_t launch_cache_values(const CacheValueLaunch& v, hipStream_t st);
+hipError_t launch_cache_select(const float* in, const uint32_t* list_s, uint32_t n_s, const uint32_t* list_v,
+                               uint32_t n_v, uint32_t count, uint32_t seed, uint32_t frame, float* out, hipStream_t st);
+hipError_t launch_kind_flags(const float* in, uint64_t n, int want_surface, uint32_t* flags, hipStream_t st);
+hipError_t launch_err_accum(const float* in, const float* row_err, uint32_t rows, unsigned long long* esum,
+                            unsigned int* ecnt, hipStream_t st);
+hipError_t launch_cache_flags(const float* in, uint64_t n, const unsigned long long* esum, const unsigned int* ecnt,
+                              float threshold, int cache_surface, uint32_t* flags, hipStream_t st);
+hipError_t launch_cache_compact(const uint32_t* flags, const uint32_t* off, uint64_t n, uint32_t* list,
+                                hipStream_t st);
+hipError_t launch_cache_accumulate(const uint32_t* off, const uint32_t* cnt, uint32_t npix, const float* wt,
+                                   const float* val, int spp, float exposure, float* rgb, uchar4* u8, hipStream_t st);
+
 }  // namespace pm

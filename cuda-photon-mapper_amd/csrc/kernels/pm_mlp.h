@@ -1,0 +1,72 @@
+// pm_mlp.h -- dimensions and device buffers of the learned radiance cache.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+
+namespace pm {
+
+enum : int {
+  MLP_IN = 9,                         // position, direction, normal (or 0)
+  MLP_BANDS = 12,                     // sin/cos frequency bands per input
+  MLP_FEAT = MLP_IN * 2 * MLP_BANDS,  // 216
+  MLP_KPAD = 224,                     // features padded to a multiple of 32
+  MLP_HID = 128,
+  MLP_HIDDEN_LAYERS = 4,
+  MLP_OUT = 3,
+  MLP_OUTPAD = 16,                    // output rows padded to one MFMA tile
+  MLP_LAYERS = MLP_HIDDEN_LAYERS + 1,
+};
+
+// Weight matrix of layer l is [nout x kin], row-major.
+inline int mlp_kin(int l) { return l == 0 ? MLP_KPAD : MLP_HID; }
+inline int mlp_nout(int l) { return l == MLP_HIDDEN_LAYERS ? MLP_OUTPAD : MLP_HID; }
+inline size_t mlp_w_count(int l) { return (size_t)mlp_nout(l) * (size_t)mlp_kin(l); }
+inline size_t mlp_b_count(int l) { return (size_t)mlp_nout(l); }
+inline size_t mlp_param_count() {
+  size_t n = 0;
+  for (int l = 0; l < MLP_LAYERS; ++l) n += mlp_w_count(l) + mlp_b_count(l);
+  return n;
+}
+// Multiply-adds per forward sample (unpadded sizes): 216*128 + 3*128*128 + 128*3.
+inline double mlp_forward_macs() { return 216.0 * 128 + 3.0 * 128 * 128 + 128.0 * 3; }
+
+struct MlpDev {
+  float* w[MLP_LAYERS];       // FP32 master weights
+  float* b[MLP_LAYERS];
+  uint16_t* wb[MLP_LAYERS];   // BF16 copies used by the MFMA GEMMs
+  float* mw[MLP_LAYERS];      // Adam moments
+  float* vw[MLP_LAYERS];
+  float* mb[MLP_LAYERS];
+  float* vb[MLP_LAYERS];
+};
+
+struct MlpTrainBufs {
+  uint16_t* dy;       // [rows x OUTPAD]
+  uint16_t* d0;       // [rows x HID] deltas (ping-pong)
+  uint16_t* d1;
+  float* row_err;     // [rows]
+  float* partial;     // split-K partials
+  float* gw[MLP_LAYERS];
+  float* gb[MLP_LAYERS];
+};
+
+hipError_t mlp_encode(const float* in, uint32_t n, uint16_t* out, hipStream_t st);
+hipError_t mlp_forward(const MlpDev& w, const uint16_t* X, uint32_t rows, uint16_t* const* H, float* y,
+                       hipStream_t st);
+hipError_t mlp_decode(const float* y, uint32_t rows, float* rgb, hipStream_t st);
+hipError_t mlp_backward(const MlpDev& w, const uint16_t* X, uint32_t rows, uint16_t* const* H, const float* y,
+                        const float* target, MlpTrainBufs& tb, hipStream_t st);
+hipError_t mlp_adam(MlpDev& w, const MlpTrainBufs& tb, float lr, int step, hipStream_t st);
+hipError_t mlp_refresh_bf16(MlpDev& w, hipStream_t st);
+// Fused inference (pm_mlp_infer.hip): h1 holds mlp_infer_h1_rows() x 128 BF16.
+size_t mlp_infer_h1_rows();
+size_t mlp_infer_lds_bytes();
+hipError_t mlp_infer(const MlpDev& w, const float* in, uint32_t n, uint16_t* h1, float* rgb, hipStream_t st);
+
+}  // namespace pm

@@ -1,0 +1,343 @@
+// pm_mlp.hip -- learned radiance cache: frequency encoding, BF16 MFMA GEMMs
+// with FP32 accumulation, backward pass and Adam, for gfx950.
+//
+// No reference counterpart: the reference has no learned component (its
+// radiance comes only from the voxel-grid sums, photonMappingKernel.cu:
+// 314-389, 831-870).  The cache predicts the same quantity as the kNN gather
+// (surface outgoing radiance or volume in-scatter) from
+//   input  = position (3) + direction (3) + normal or zero (3)
+//   encode = sin/cos(2^b pi v), b = 0..11  -> 216 features (padded to 224)
+//   MLP    = 4 x (128, ReLU) -> 3, output = log(1 + radiance)
+//
+// GEMM: C[M x N] = A[M x K] * B[K x N], A/B in BF16 (either may be read
+// transposed), FP32 accumulation on v_mfma_f32_16x16x16_bf16.  A 256-thread
+// workgroup computes a 64x64 tile of C; each wave a 32x32 quarter (2x2 MFMA
+// tiles); K advances 32 at a time through LDS.  The weight-gradient GEMM
+// splits its long K (the batch) over grid.z slices that write FP32 partials;
+// a second kernel adds the slices in a fixed order, so results are
+// deterministic run to run.
+#include "pm_kernels.h"
+#include "pm_mlp.h"
+#include "pm_wave.h"
+
+namespace pm {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) { return f32_to_bf16(f); }
+__device__ __forceinline__ float bf2f(uint16_t h) { return bf16_to_f32(h); }
+
+// ---------------------------------------------------------------- encoding
+// out[r][i*24 + 2b + {0,1}] = {sin, cos}(2 pi * 2^(b-1) * in[r][i]); padded to MLP_KPAD.
+__global__ void k_encode(const float* __restrict__ in, uint32_t n, uint16_t* __restrict__ out) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t r = t / MLP_IN, i = t % MLP_IN;
+  if (r >= n) return;
+  float v = in[(size_t)r * MLP_IN + i];
+  uint16_t* o = out + (size_t)r * MLP_KPAD + i * 2 * MLP_BANDS;
+  float scale = 0.5f;
+  for (int b = 0; b < MLP_BANDS; ++b) {
+    float s, c;
+    sincos_turns(v * scale, &s, &c);
+    o[2 * b] = f2bf(s);
+    o[2 * b + 1] = f2bf(c);
+    scale *= 2.0f;
+  }
+  if (i == MLP_IN - 1)
+    for (int k = MLP_FEAT; k < MLP_KPAD; ++k) out[(size_t)r * MLP_KPAD + k] = 0;
+}
+
+// ---------------------------------------------------------------- GEMM
+static constexpr int GT = 64;   // tile M and N
+static constexpr int GK = 32;   // K per LDS stage
+static constexpr int GPAD = 8;  // LDS row padding (bf16 elements)
+
+struct GemmArgs {
+  int M, N, K;
+  const uint16_t* A;
+  int lda, transA;  // A(m,k) = transA ? A[k*lda + m] : A[m*lda + k]
+  const uint16_t* B;
+  int ldb, transB;  // B(k,n) = transB ? B[n*ldb + k] : B[k*ldb + n]
+  float* Cf;        // FP32 output (or split-K partials, slice z at Cf + z * slice_stride)
+  uint16_t* Cb;     // BF16 output
+  int ldc;
+  size_t slice_stride;
+  const float* bias;       // per column n
+  int relu;                // max(0, .)
+  const uint16_t* mask;    // multiply by (mask(m,n) > 0), mask row-major with ldc
+  int k_per_slice;         // split-K slice length (multiple of GK)
+};
+
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+  __shared__ uint16_t As[GT][GK + GPAD];  // [m][k]
+  __shared__ uint16_t Bs[GK][GT + GPAD];  // [k][n]
+  const int tid = threadIdx.x;
+  const int lane = lane_id();
+  const int wave = tid >> 6;
+  const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+  const int k_begin = blockIdx.z * g.k_per_slice;
+  const int k_end = min(g.K, k_begin + g.k_per_slice);
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x4 acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = k_begin; k0 < k_end; k0 += GK) {
+    // stage A (64 x 32) and B (32 x 64): 2048 elements each, 8 per thread
+    for (int e = tid; e < GT * GK; e += 256) {
+      int mm, kk;
+      if (g.transA) { mm = e % GT; kk = e / GT; } else { kk = e % GK; mm = e / GK; }
+      int gm = m0 + mm, gk = k0 + kk;
+      uint16_t v = 0;
+      if (gm < g.M && gk < k_end) v = g.transA ? g.A[(size_t)gk * g.lda + gm] : g.A[(size_t)gm * g.lda + gk];
+      As[mm][kk] = v;
+    }
+    for (int e = tid; e < GK * GT; e += 256) {
+      int kk, nn;
+      if (g.transB) { kk = e % GK; nn = e / GK; } else { nn = e % GT; kk = e / GT; }
+      int gk = k0 + kk, gn = n0 + nn;
+      uint16_t v = 0;
+      if (gn < g.N && gk < k_end) v = g.transB ? g.B[(size_t)gn * g.ldb + gk] : g.B[(size_t)gk * g.ldb + gn];
+      Bs[kk][nn] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < GK; ks += 16) {
+      bf16x4 af[2], bfr[2];
+      const int kq = ks + 4 * (lane >> 4);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int row = wm + t * 16 + (lane & 15);
+        af[t] = bf16x4{(short)As[row][kq], (short)As[row][kq + 1], (short)As[row][kq + 2], (short)As[row][kq + 3]};
+        const int col = wn + t * 16 + (lane & 15);
+        bfr[t] = bf16x4{(short)Bs[kq][col], (short)Bs[kq + 1][col], (short)Bs[kq + 2][col], (short)Bs[kq + 3][col]};
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bfr[b], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // epilogue: lane holds D[4*(lane/16) + e][lane%16] of each 16x16 tile
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int gm = m0 + wm + a * 16 + 4 * (lane >> 4) + e;
+        const int gn = n0 + wn + b * 16 + (lane & 15);
+        if (gm >= g.M || gn >= g.N) continue;
+        float v = acc[a][b][e];
+        if (g.bias) v += g.bias[gn];
+        if (g.relu) v = v > 0.0f ? v : 0.0f;
+        if (g.mask && !(bf2f(g.mask[(size_t)gm * g.ldc + gn]) > 0.0f)) v = 0.0f;
+        if (g.Cf) g.Cf[blockIdx.z * g.slice_stride + (size_t)gm * g.ldc + gn] = v;
+        if (g.Cb) g.Cb[(size_t)gm * g.ldc + gn] = f2bf(v);
+      }
+}
+
+// out[i] = sum over s of part[s * stride + i], fixed order.
+__global__ void k_sum_slices(const float* __restrict__ part, int slices, size_t stride, size_t n,
+                             float* __restrict__ out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.0f;
+  for (int z = 0; z < slices; ++z) s += part[(size_t)z * stride + i];
+  out[i] = s;
+}
+
+// Column sums of a BF16 matrix [rows x cols] (bias gradients): one block per
+// column, fixed-order tree reduction.
+__global__ void __launch_bounds__(256) k_colsum(const uint16_t* __restrict__ x, int rows, int cols, int ld,
+                                                 float* __restrict__ out) {
+  __shared__ float red[256];
+  const int c = blockIdx.x;
+  float s = 0.0f;
+  for (int r = threadIdx.x; r < rows; r += 256) s += bf2f(x[(size_t)r * ld + c]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = red[0];
+}
+
+// Loss L = mean over (rows, 3 channels) of (y - t)^2 on y = net output,
+// t = log(1 + target).  Writes dL/dy (BF16, padded to MLP_OUTPAD columns)
+// and per-row squared errors.
+__global__ void k_loss_grad(const float* __restrict__ y, const float* __restrict__ target, int rows,
+                            uint16_t* __restrict__ dy, float* __restrict__ row_err) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  float e2 = 0.0f;
+  const float inv = 2.0f / (float)(rows * MLP_OUT);
+  for (int c = 0; c < MLP_OUTPAD; ++c) {
+    float g = 0.0f;
+    if (c < MLP_OUT) {
+      float t = log_(1.0f + fmax_(target[(size_t)r * MLP_OUT + c], 0.0f));
+      float d = y[(size_t)r * MLP_OUTPAD + c] - t;
+      e2 += d * d;
+      g = d * inv;
+    }
+    dy[(size_t)r * MLP_OUTPAD + c] = f2bf(g);
+  }
+  row_err[r] = e2;
+}
+
+// Adam on FP32 master parameters, refreshing the BF16 copy.
+__global__ void k_adam(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+                       const float* __restrict__ grad, uint16_t* __restrict__ pb, size_t n, float lr, float b1,
+                       float b2, float eps, float c1, float c2) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float gr = grad[i];
+  float mi = b1 * m[i] + (1.0f - b1) * gr;
+  float vi = b2 * v[i] + (1.0f - b2) * gr * gr;
+  m[i] = mi;
+  v[i] = vi;
+  float mh = mi / c1, vh = vi / c2;
+  float pi = p[i] - lr * mh / (sqrt_(vh) + eps);
+  p[i] = pi;
+  if (pb) pb[i] = f2bf(pi);
+}
+
+__global__ void k_to_bf16(const float* __restrict__ x, size_t n, uint16_t* __restrict__ y) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(x[i]);
+}
+
+// radiance = max(0, exp(y) - 1) for the first 3 output columns.
+__global__ void k_decode_out(const float* __restrict__ y, size_t rows, float* __restrict__ rgb) {
+  size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  for (int c = 0; c < MLP_OUT; ++c) {
+    float e = exp_(y[r * MLP_OUTPAD + c]) - 1.0f;
+    rgb[r * MLP_OUT + c] = e > 0.0f ? e : 0.0f;
+  }
+}
+
+// ---------------------------------------------------------------- host side
+static hipError_t gemm(const GemmArgs& a, int slices, hipStream_t st) {
+  dim3 grid((unsigned)((a.N + GT - 1) / GT), (unsigned)((a.M + GT - 1) / GT), (unsigned)slices);
+  hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+static GemmArgs gemm_args(int M, int N, int K) {
+  GemmArgs a;
+  memset(&a, 0, sizeof(a));
+  a.M = M;
+  a.N = N;
+  a.K = K;
+  a.k_per_slice = ((K + GK - 1) / GK) * GK;
+  return a;
+}
+
+hipError_t mlp_encode(const float* in, uint32_t n, uint16_t* out, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  uint64_t t = (uint64_t)n * MLP_IN;
+  hipLaunchKernelGGL(k_encode, dim3((unsigned)((t + 255) / 256)), dim3(256), 0, st, in, n, out);
+  return hipGetLastError();
+}
+
+// Forward for `rows` encoded inputs X [rows x KPAD]; H[l] receives the
+// post-ReLU activations (BF16), y the FP32 output [rows x OUTPAD].
+hipError_t mlp_forward(const MlpDev& w, const uint16_t* X, uint32_t rows, uint16_t* const* H, float* y,
+                       hipStream_t st) {
+  hipError_t e;
+  const uint16_t* in = X;
+  int kin = MLP_KPAD;
+  for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) {
+    GemmArgs a = gemm_args((int)rows, MLP_HID, kin);
+    a.A = in; a.lda = kin; a.transA = 0;
+    a.B = w.wb[l]; a.ldb = kin; a.transB = 1;  // W [out x in] row-major -> B(k, n) = W[n][k]
+    a.Cb = H[l]; a.ldc = MLP_HID;
+    a.bias = w.b[l];
+    a.relu = 1;
+    if ((e = gemm(a, 1, st)) != hipSuccess) return e;
+    in = H[l];
+    kin = MLP_HID;
+  }
+  GemmArgs a = gemm_args((int)rows, MLP_OUTPAD, MLP_HID);
+  a.A = in; a.lda = MLP_HID;
+  a.B = w.wb[MLP_HIDDEN_LAYERS]; a.ldb = MLP_HID; a.transB = 1;
+  a.Cf = y; a.ldc = MLP_OUTPAD;
+  a.bias = w.b[MLP_HIDDEN_LAYERS];
+  return gemm(a, 1, st);
+}
+
+hipError_t mlp_decode(const float* y, uint32_t rows, float* rgb, hipStream_t st) {
+  if (rows == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_decode_out, dim3((rows + 255) / 256), dim3(256), 0, st, y, (size_t)rows, rgb);
+  return hipGetLastError();
+}
+
+// Gradients of the loss for one batch (the forward must have filled H and y).
+// grads: FP32 buffers laid out like the parameters; scratch per MlpTrainBufs.
+hipError_t mlp_backward(const MlpDev& w, const uint16_t* X, uint32_t rows, uint16_t* const* H, const float* y,
+                        const float* target, MlpTrainBufs& tb, hipStream_t st) {
+  hipError_t e;
+  hipLaunchKernelGGL(k_loss_grad, dim3((rows + 255) / 256), dim3(256), 0, st, y, target, (int)rows, tb.dy,
+                     tb.row_err);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const int slices = (int)std::max<uint32_t>(1u, std::min<uint32_t>(64u, rows / 256u));
+  const int kps = (int)(((rows + slices - 1) / slices + GK - 1) / GK * GK);
+  // delta of the current layer's pre-activation (BF16): starts as dL/dy.
+  const uint16_t* delta = tb.dy;
+  int dcols = MLP_OUTPAD;
+  for (int l = MLP_HIDDEN_LAYERS; l >= 0; --l) {
+    const uint16_t* act = l == 0 ? X : H[l - 1];
+    const int kin = l == 0 ? MLP_KPAD : MLP_HID;
+    const int nout = l == MLP_HIDDEN_LAYERS ? MLP_OUTPAD : MLP_HID;
+    // dW_l [nout x kin] = delta^T [nout x rows] * act [rows x kin], split over the batch
+    GemmArgs a = gemm_args(nout, kin, (int)rows);
+    a.A = delta; a.lda = dcols; a.transA = 1;
+    a.B = act; a.ldb = kin; a.transB = 0;
+    a.Cf = tb.partial; a.ldc = kin; a.slice_stride = (size_t)nout * kin;
+    a.k_per_slice = kps;
+    if ((e = gemm(a, slices, st)) != hipSuccess) return e;
+    size_t nw = (size_t)nout * kin;
+    hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, tb.partial, slices,
+                       (size_t)nout * kin, nw, tb.gw[l]);
+    hipLaunchKernelGGL(k_colsum, dim3(nout), dim3(256), 0, st, delta, (int)rows, nout, dcols, tb.gb[l]);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (l == 0) break;
+    // delta_{l-1} [rows x 128] = (delta [rows x nout] * W_l [nout x 128]) * (H[l-1] > 0)
+    uint16_t* nd = (l & 1) ? tb.d1 : tb.d0;
+    GemmArgs b = gemm_args((int)rows, MLP_HID, nout);
+    b.A = delta; b.lda = dcols; b.transA = 0;
+    b.B = w.wb[l]; b.ldb = kin; b.transB = 0;  // W [nout x kin] row-major: B(k=out, n=in)
+    b.Cb = nd; b.ldc = MLP_HID;
+    b.mask = H[l - 1];
+    if ((e = gemm(b, 1, st)) != hipSuccess) return e;
+    delta = nd;
+    dcols = MLP_HID;
+  }
+  return hipSuccess;
+}
+
+hipError_t mlp_adam(MlpDev& w, const MlpTrainBufs& tb, float lr, int step, hipStream_t st) {
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  float c1 = 1.0f - (float)std::pow(0.9, (double)step), c2 = 1.0f - (float)std::pow(0.999, (double)step);
+  for (int l = 0; l <= MLP_HIDDEN_LAYERS; ++l) {
+    size_t nw = mlp_w_count(l), nb = mlp_b_count(l);
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, w.w[l], w.mw[l], w.vw[l],
+                       tb.gw[l], w.wb[l], nw, lr, b1, b2, eps, c1, c2);
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, w.b[l], w.mb[l], w.vb[l],
+                       tb.gb[l], (uint16_t*)nullptr, nb, lr, b1, b2, eps, c1, c2);
+  }
+  return hipGetLastError();
+}
+
+hipError_t mlp_refresh_bf16(MlpDev& w, hipStream_t st) {
+  for (int l = 0; l <= MLP_HIDDEN_LAYERS; ++l) {
+    size_t nw = mlp_w_count(l);
+    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, w.w[l], nw, w.wb[l]);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace pm

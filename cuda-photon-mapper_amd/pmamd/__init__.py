@@ -32,7 +32,7 @@ PM_MEDIUM_NONE, PM_MEDIUM_HOMOGENEOUS, PM_MEDIUM_NOISE = 0, 1, 2
 PM_STORE_FP32, PM_STORE_BF16 = 0, 1
 PM_EST_BOX, PM_EST_CONE = 0, 1
 PM_MAP_SURFACE, PM_MAP_VOLUME = 0, 1
-PM_OPT_GATHER_MODE, PM_OPT_BLOCKS_PER_CU = 1, 2
+PM_OPT_GATHER_MODE, PM_OPT_BLOCKS_PER_CU, PM_OPT_CACHE_FUSED_INFER = 1, 2, 3
 
 
 class PmSphere(ctypes.Structure):
@@ -106,6 +106,22 @@ class PmStats(ctypes.Structure):
     ]
 
 
+class PmCacheStats(ctypes.Structure):
+    _fields_ = [
+        ("records", ctypes.c_uint64),
+        ("fallbacks", ctypes.c_uint64),
+        ("train_loss", ctypes.c_double),
+        ("ms_record", ctypes.c_double),
+        ("ms_train", ctypes.c_double),
+        ("ms_infer", ctypes.c_double),
+        ("ms_fallback", ctypes.c_double),
+        ("ms_accumulate", ctypes.c_double),
+        ("ms_total", ctypes.c_double),
+        ("frame", ctypes.c_uint32),
+        ("pad0", ctypes.c_uint32),
+    ]
+
+
 class PmError(RuntimeError):
     def __init__(self, status, what):
         super().__init__("%s failed with status %d" % (what, status))
@@ -146,6 +162,19 @@ _SIGS = {
     "pm_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "pm_copy_kernel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
     "pm_set_option": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "pm_cache_param_count": (ctypes.c_uint32, []),
+    "pm_cache_reset": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "pm_cache_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int]),
+    "pm_cache_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                     ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
+    "pm_cache_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                      ctypes.c_uint32, ctypes.c_float, ctypes.POINTER(ctypes.c_double)]),
+    "pm_cache_predict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
+    "pm_cache_configure": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.c_float, ctypes.c_int, ctypes.c_int]),
+    "pm_cache_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PmCacheStats)]),
+    "pm_render_cache_crop": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_void_p, ctypes.c_void_p]),
 }
 
 
@@ -291,6 +320,73 @@ class PhotonMapper:
         check(load().pm_get_tree(self.h, which, perm.ctypes.data, split.ctypes.data, axis.ctypes.data, n, ni),
               "pm_get_tree")
         return perm, split, axis
+
+    # ---- learned radiance cache
+    def cache_reset(self, seed):
+        check(load().pm_cache_reset(self.h, seed), "pm_cache_reset")
+
+    def cache_params(self, values=None):
+        n = load().pm_cache_param_count()
+        if values is None:
+            out = np.empty(n, np.float32)
+            check(load().pm_cache_params(self.h, out.ctypes.data, n, 0), "pm_cache_params")
+            return out
+        v = np.ascontiguousarray(values, np.float32)
+        assert v.size == n
+        check(load().pm_cache_params(self.h, v.ctypes.data, n, 1), "pm_cache_params")
+
+    def cache_grad(self, inputs, targets):
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 9)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        xb, tb = DeviceBuffer(x.nbytes), DeviceBuffer(t.nbytes)
+        xb.upload(x)
+        tb.upload(t)
+        n = load().pm_cache_param_count()
+        g = np.empty(n, np.float32)
+        loss = ctypes.c_double()
+        check(load().pm_cache_grad(self.h, xb.ptr, tb.ptr, x.shape[0], g.ctypes.data, n, ctypes.byref(loss)),
+              "pm_cache_grad")
+        return g, loss.value
+
+    def cache_train(self, inputs, targets, batch, lr):
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 9)
+        t = np.ascontiguousarray(targets, np.float32).reshape(-1, 3)
+        xb, tb = DeviceBuffer(x.nbytes), DeviceBuffer(t.nbytes)
+        xb.upload(x)
+        tb.upload(t)
+        loss = ctypes.c_double()
+        check(load().pm_cache_train(self.h, xb.ptr, tb.ptr, x.shape[0], batch, lr, ctypes.byref(loss)),
+              "pm_cache_train")
+        return loss.value
+
+    def cache_configure(self, threshold=0.02, train_samples=1 << 18, batch=1 << 14, lr=1e-3, march_stride=1,
+                        cache_surface=0):
+        check(load().pm_cache_configure(self.h, threshold, train_samples, batch, lr, march_stride, cache_surface),
+              "pm_cache_configure")
+
+    def cache_stats(self):
+        s = PmCacheStats()
+        check(load().pm_cache_stats_get(self.h, ctypes.byref(s)), "pm_cache_stats_get")
+        return s
+
+    def render_cache(self, x0=0, y0=0, w=0, h=0, want_u8=False):
+        if w <= 0:
+            w, h, x0, y0 = self.cfg.width, self.cfg.height, 0, 0
+        rgb = DeviceBuffer(w * h * 3 * 4)
+        u8 = DeviceBuffer(w * h * 4) if want_u8 else None
+        check(load().pm_render_cache_crop(self.h, x0, y0, w, h, rgb.ptr, u8.ptr if u8 else None),
+              "pm_render_cache_crop")
+        out = rgb.download(np.float32, w * h * 3).reshape(h, w, 3)
+        if want_u8:
+            return out, u8.download(np.uint8, w * h * 4).reshape(h, w, 4)
+        return out
+
+    def cache_predict(self, inputs):
+        x = np.ascontiguousarray(inputs, np.float32).reshape(-1, 9)
+        xb, ob = DeviceBuffer(max(x.nbytes, 4)), DeviceBuffer(max(x.shape[0] * 12, 4))
+        xb.upload(x)
+        check(load().pm_cache_predict(self.h, xb.ptr, x.shape[0], ob.ptr), "pm_cache_predict")
+        return ob.download(np.float32, x.shape[0] * 3).reshape(-1, 3)
 
     def knn(self, which, queries, k, rcap):
         q = np.ascontiguousarray(queries, dtype=np.float32).reshape(-1, 3)

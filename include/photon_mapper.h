@@ -161,9 +161,60 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
 /* Execution options (no effect on results, only on how the GPU runs them):
  * PM_OPT_GATHER_MODE   0 = one wave64 per radiance query (cross-lane sorted list),
  *                      1 = one lane per query, 8x8 pixel tile per wave (default);
- * PM_OPT_BLOCKS_PER_CU workgroups per compute unit of the persistent mode-0 grid. */
-enum { PM_OPT_GATHER_MODE = 1, PM_OPT_BLOCKS_PER_CU = 2 };
+ * PM_OPT_BLOCKS_PER_CU workgroups per compute unit of the persistent mode-0 grid;
+ * PM_OPT_CACHE_FUSED_INFER 1 = cache inference through the fused LDS-resident
+ *                      kernels (default), 0 = through the generic GEMMs. */
+enum { PM_OPT_GATHER_MODE = 1, PM_OPT_BLOCKS_PER_CU = 2, PM_OPT_CACHE_FUSED_INFER = 3 };
 int pm_set_option(pm_context* ctx, int option, int value);
+
+/* ------------------------------------------------------------------ learned radiance cache
+ * A small MLP (9 inputs -> 12-band sin/cos encoding -> 4 x 128 ReLU -> RGB,
+ * predicting log(1 + radiance)) trained online on the kNN gather's own
+ * estimates.  BF16 weights on MFMA with FP32 accumulation; FP32 master
+ * weights and Adam state.  Inputs are rows of 9 floats: position, direction,
+ * normal (zero for volume queries).  No reference counterpart. */
+uint32_t pm_cache_param_count(void);
+/* (Re)initialise the weights deterministically from `seed`; clears Adam state. */
+int pm_cache_reset(pm_context* ctx, uint32_t seed);
+/* Copy the FP32 parameters to (set = 0) or from (set = 1) host memory; the
+ * flat layout is, per layer l = 0..4, W_l [nout x kin] row-major then b_l,
+ * with (kin, nout) = (224, 128), (128, 128) x 3, (128, 16); padding zero. */
+int pm_cache_params(pm_context* ctx, float* params_host, uint32_t count, int set);
+/* Loss gradient (same layout as the parameters) of mean((y - log(1+t))^2)
+ * over n <= 2^20 device rows, without updating; loss_out may be NULL. */
+int pm_cache_grad(pm_context* ctx, const float* inputs_dev, const float* targets_dev, uint32_t n,
+                  float* grad_host, uint32_t count, double* loss_out);
+/* Adam steps (one per `batch` rows, lr) over n device rows; mean loss out. */
+int pm_cache_train(pm_context* ctx, const float* inputs_dev, const float* targets_dev, uint32_t n,
+                   uint32_t batch, float lr, double* loss_out);
+/* Radiance predictions (n x 3 FP32) for n device input rows. */
+int pm_cache_predict(pm_context* ctx, const float* inputs_dev, uint32_t n, float* rgb_dev);
+
+/* Cache render settings: fallback threshold on the mean squared log-radiance
+ * training error of a 16^3 cell (per query kind), training samples per frame,
+ * Adam batch and learning rate, march points per cached volume query (the
+ * first point's prediction stands for up to `march_stride` consecutive points,
+ * weighted by their summed path weights; 1 = every point), and whether
+ * surface queries use the cache (0: always gathered exactly).
+ * Defaults 0.02, 2^18, 2^14, 1e-3, 1, 0. */
+int pm_cache_configure(pm_context* ctx, float threshold, uint32_t train_samples, uint32_t batch, float lr,
+                       int march_stride, int cache_surface);
+
+typedef struct pm_cache_stats {
+  uint64_t records;        /* radiance queries of the frame */
+  uint64_t fallbacks;      /* of which gathered exactly (kNN) */
+  double train_loss;       /* mean loss of this frame's training steps */
+  double ms_record, ms_train, ms_infer, ms_fallback, ms_accumulate, ms_total;
+  uint32_t frame;          /* frames rendered with this cache */
+  uint32_t pad0;
+} pm_cache_stats;
+
+/* Render the crop (x0, y0, w, h) of the configured frame through the cache:
+ * record all radiance queries, train on train_samples of them (exact kNN
+ * targets), predict the rest, re-gather exactly where the cell error is above
+ * the threshold, accumulate.  Same outputs as pm_render_crop. */
+int pm_render_cache_crop(pm_context* ctx, int x0, int y0, int w, int h, float* rgb_dev, pm_uchar4* out_dev);
+int pm_cache_stats_get(pm_context* ctx, pm_cache_stats* out);
 
 /* Inspection (copies to host memory; for tests and tools). */
 int pm_get_records(pm_context* ctx, int map, float* pos_host, float* power_host, float* dir_host, uint32_t n);

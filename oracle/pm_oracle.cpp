@@ -369,7 +369,7 @@ struct OracleGather {
   uint64_t nsurf = 0, nvol = 0;
   T rd(float v) const { return bf16 ? T(bf16_to_f32(f32_to_bf16(v))) : T(v); }
 
-  V3<T> surface(V3<T> x, V3<T> n, V3<T> albedo) {
+  V3<T> surface(V3<T> x, V3<T> n, V3<T> albedo, V3<T> = V3<T>(), V3<T> = V3<T>()) {
     ++nsurf;
     const Map& m = c->surf;
     const int k = c->cfg.k_surface;
@@ -396,7 +396,7 @@ struct OracleGather {
     return mul(mul(sum, albedo), sc) * norm;
   }
 
-  V3<T> volume(V3<T> x, V3<T> wo) {
+  V3<T> volume(V3<T> x, V3<T> wo, V3<T> = V3<T>()) {
     ++nvol;
     const Map& m = c->vol;
     const int k = c->cfg.k_volume;
