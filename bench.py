@@ -12,7 +12,11 @@ density estimates) per second of whole-frame time.  `roofline` is for the
 dominant kernel (the render/gather kernel), with its duration measured by HIP
 events on the library's own stream inside libpm.so.  `cpu_baseline` times the
 CPU oracle (same per-thread logic, std::thread over the host cores) on a
-stratified sample of the same frame.
+stratified sample of the same frame.  `cache` reports the learned-cache form of
+the same frame (queries recorded, MLP trained online on exact kNN targets,
+MLP inference with exact kNN fallback): ms/frame, speed-up over the plain
+frame, its error against the plain frame on the same eye samples, and the
+MFMA rate of the MLP inference.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -84,7 +88,59 @@ def run_gpu(cfg, steps, warmup, gather_mode):
         log("step %d: emit %.1f ms, build %.1f ms, render %.1f ms" % (i, s.ms_emit, s.ms_build, s.ms_render))
     t1 = time.perf_counter()
     img = rgb.download(np.float32, npx * 3)
-    return (t1 - t0) / steps, per, g.stats(), img
+    return (t1 - t0) / steps, per, g.stats(), img, g
+
+
+def rel_rmse(a, b, exposure=None):
+    if exposure is not None:  # displayed values: exposure, clamp to [0, 1]
+        a, b = np.clip(a * exposure, 0, 1), np.clip(b * exposure, 0, 1)
+    return float(np.sqrt(((a.astype(np.float64) - b) ** 2).mean()) / np.abs(b).mean())
+
+
+def run_cache(g, cfg, steps, train_frames, stride, threshold, plain_img):
+    """Cache frames of the same workload: emit + build + cache render (record
+    queries, train on 2^18 exact kNN targets, MLP inference, exact fallback),
+    after `train_frames` frames of online training from a fresh network."""
+    lib = pmamd.load()
+    npx = cfg.width * cfg.height
+    rgb = pmamd.DeviceBuffer(npx * 12)
+    g.cache_reset(1)
+    g.cache_configure(threshold=threshold, march_stride=stride)
+
+    def frame():
+        g.emit()
+        g.build()
+        pmamd.check(lib.pm_render_cache_crop(g.h, 0, 0, cfg.width, cfg.height, rgb.ptr, None), "pm_render_cache_crop")
+        return g.cache_stats()
+
+    for i in range(train_frames):
+        cs = frame()
+        if i % 4 == 0:
+            log("cache training frame %d: %.1f ms, loss %.4g" % (i, cs.ms_total, cs.train_loss))
+    per = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        cs = frame()
+        per.append((cs.ms_record, cs.ms_train, cs.ms_infer, cs.ms_fallback, cs.ms_accumulate, cs.ms_total))
+        log("cache step %d: %.1f ms (infer %.1f ms), fallback %d of %d" % (i, cs.ms_total, cs.ms_infer, cs.fallbacks,
+                                                                          cs.records))
+    t1 = time.perf_counter()
+    img = rgb.download(np.float32, npx * 3).reshape(-1, 3)
+    m = np.mean(np.array(per), axis=0)
+    infer_flop = 154368.0 * cs.records
+    return {
+        "ms_per_frame": round((t1 - t0) / steps * 1e3, 3),
+        "breakdown_ms": {"record": round(m[0], 3), "train": round(m[1], 3), "infer": round(m[2], 3),
+                         "fallback": round(m[3], 3), "accumulate": round(m[4], 3), "render_total": round(m[5], 3)},
+        "march_stride": stride, "threshold": threshold, "train_frames_before_timing": train_frames,
+        "cache_queries": int(cs.records), "fallback_queries": int(cs.fallbacks), "train_loss": round(cs.train_loss, 5),
+        "rmse_vs_plain_same_samples": {"display_rel": round(rel_rmse(img, plain_img, cfg.exposure), 5),
+                                       "linear_rel": round(rel_rmse(img, plain_img), 5)},
+        "mlp_inference_mfma": {"bound": "mfma", "achieved": round(infer_flop / (m[2] * 1e-3) / 1e12, 2),
+                               "peak": 2500.0, "unit": "TFLOP/s",
+                               "frac": round(infer_flop / (m[2] * 1e-3) / 1e12 / 2500.0, 4),
+                               "flop_per_query": 154368},
+    }
 
 
 def run_cpu(cfg, budget_s):
@@ -143,6 +199,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--gather-mode", type=int, default=1, help="0: wave per query, 1: lane per query")
     ap.add_argument("--frame", type=str, default="", help="WxHxSPP override of the frame (profiling runs only)")
+    ap.add_argument("--cache-steps", type=int, default=3, help="timed cache frames (0: skip the cache leg)")
+    ap.add_argument("--cache-train-frames", type=int, default=12)
+    ap.add_argument("--cache-stride", type=int, default=4)
+    ap.add_argument("--cache-threshold", type=float, default=0.05)
     ap.add_argument("--copy-calibrate", type=int, default=0,
                     help="run only the library's copy kernel over this many bytes (memory-counter calibration)")
     args = ap.parse_args()
@@ -161,7 +221,7 @@ def main():
     if args.frame:
         w, h, spp = (int(v) for v in args.frame.lower().split("x"))
         cfg.width, cfg.height, cfg.spp = w, h, spp
-    sec_per_frame, per, stats, img = run_gpu(cfg, args.steps, args.warmup, args.gather_mode)
+    sec_per_frame, per, stats, img, g = run_gpu(cfg, args.steps, args.warmup, args.gather_mode)
     q_frame = stats.surface_queries + stats.volume_queries
     value = q_frame / sec_per_frame
     kern_ms = float(np.mean([p[3] for p in per]))
@@ -199,6 +259,11 @@ def main():
         "roofline": roof,
         "image_mean_rgb": [round(float(x), 5) for x in img.reshape(-1, 3).mean(0)],
     }
+    if args.cache_steps > 0:
+        ca = run_cache(g, cfg, args.cache_steps, args.cache_train_frames, args.cache_stride, args.cache_threshold,
+                       img.reshape(-1, 3))
+        ca["speedup_vs_plain_frame"] = round(out["ms_per_step"] / ca["ms_per_frame"], 3)
+        out["cache"] = ca
     if not args.no_cpu:
         cb = run_cpu(cfg, args.cpu_seconds)
         out["cpu_baseline"] = {

@@ -83,6 +83,70 @@ inline int preset_config(int preset, pm_config* c) {
   }
 }
 
+// Scene `index` of the batched workload: the unit room of preset 1 (same
+// walls, light, camera, k = 32, r <= 0.05, 1 spp) with, drawn from Philox
+// (key = seed, counter = (index, object, draw, TAG_SCENE)):
+//   2 .. 256 spheres, radius U[0.02, 0.2], material 60% Lambertian (albedo
+//   U[0.2, 0.8] per channel) / 25% mirror / 15% dielectric (IOR 1.5), centres
+//   uniform inside the room, placed without overlapping each other or the
+//   light (up to 32 tries, then the sphere is dropped);
+//   photons emitted log-uniform in [10^4, 2 10^6];
+//   image side from {128, 256, 512, 1024};
+//   homogeneous medium, sigma_s U[0, 0.3], sigma_a = sigma_s / 10.
+// Returns 0, or -1 for a negative index.
+inline int batch_scene_config(uint32_t seed, int index, pm_config* c) {
+  if (index < 0) return -1;
+  preset_room(c);
+  const uint32_t id = (uint32_t)index;
+  auto u = [&](uint32_t obj, uint32_t draw) { return u01(rng(seed, id, obj, draw, TAG_SCENE).x); };
+  c->seed = seed * 0x9E3779B9u + id + 1u;  // independent photon and camera streams per scene
+  const int nsph = 2 + (int)(u(0, 0) * 255.0f);
+  c->n_emit = (uint32_t)(1e4f * exp_(u(0, 1) * 5.298317366548036f));  // ln(200)
+  const int sides[4] = {128, 256, 512, 1024};
+  const int side = sides[(int)(u(0, 2) * 4.0f)];
+  c->width = c->height = side;
+  c->sigma_s = 0.3f * u(0, 3);
+  c->sigma_a = 0.1f * c->sigma_s;
+  if (!(c->sigma_s > 0.0f)) c->medium = PM_MEDIUM_NONE;
+  c->n_spheres = 0;
+  for (int j = 0; j < nsph && c->n_spheres < PM_MAX_SPHERES; ++j) {
+    const uint32_t o = 1u + (uint32_t)j;
+    pm_sphere sp;
+    memset(&sp, 0, sizeof(sp));
+    sp.radius = 0.02f + 0.18f * u(o, 0);
+    const float m = u(o, 1);
+    if (m < 0.6f) {
+      sp.material = PM_MAT_DIFFUSE;
+      for (int k = 0; k < 3; ++k) sp.albedo[k] = 0.2f + 0.6f * u(o, 2u + (uint32_t)k);
+      sp.ior = 1.0f;
+    } else {
+      sp.material = m < 0.85f ? PM_MAT_MIRROR : PM_MAT_DIELECTRIC;
+      sp.albedo[0] = sp.albedo[1] = sp.albedo[2] = 1.0f;
+      sp.ior = sp.material == PM_MAT_DIELECTRIC ? 1.5f : 1.0f;
+    }
+    const float r = sp.radius;
+    bool placed = false;
+    for (uint32_t a = 0; a < 32 && !placed; ++a) {
+      float ctr[3];
+      for (int k = 0; k < 3; ++k) ctr[k] = r + (1.0f - 2.0f * r) * u(o, 8u + 3u * a + (uint32_t)k);
+      float dx = ctr[0] - c->light_pos[0], dy = ctr[1] - c->light_pos[1], dz = ctr[2] - c->light_pos[2];
+      bool ok = (dx * dx + dy * dy) + dz * dz > (r + 0.03f) * (r + 0.03f);
+      for (int i = 0; ok && i < c->n_spheres; ++i) {
+        const pm_sphere& q = c->spheres[i];
+        float ex = ctr[0] - q.center[0], ey = ctr[1] - q.center[1], ez = ctr[2] - q.center[2];
+        float rr = r + q.radius + 1e-3f;
+        ok = (ex * ex + ey * ey) + ez * ez > rr * rr;
+      }
+      if (ok) {
+        for (int k = 0; k < 3; ++k) sp.center[k] = ctr[k];
+        placed = true;
+      }
+    }
+    if (placed) c->spheres[c->n_spheres++] = sp;
+  }
+  return 0;
+}
+
 // Sphere array with the legacy animation applied: each sphere's centre moves
 // on a small horizontal circle of radius 0.03 with phase anim_time + i*pi
 // (the reference moves spheres 0 and 1 along sin/cos paths, :1380-1392).

@@ -171,8 +171,10 @@ int do_emit(pm_context* c) {
   return PM_OK;
 }
 
+}  // namespace
+
 // Host shape tables of the canonical tree (see core/pm_kdtree.h).
-void shape_tables(uint32_t n, uint32_t L, std::vector<uint32_t>& lo, std::vector<uint32_t>& hi,
+void pm::shape_tables(uint32_t n, uint32_t L, std::vector<uint32_t>& lo, std::vector<uint32_t>& hi,
                   std::vector<uint32_t>& lc, std::vector<uint32_t>& lb) {
   uint32_t nl = kd_num_leaves(n, L);
   size_t ni = nl > 0 ? nl - 1 : 0;
@@ -194,6 +196,8 @@ void shape_tables(uint32_t n, uint32_t L, std::vector<uint32_t>& lo, std::vector
     }
   }
 }
+
+namespace {
 
 int build_map(pm_context* c, MapBufs& mb) {
   DevMap& m = mb.m;
@@ -232,6 +236,60 @@ GatherMap gather_map(const pm_context* c, const DevMap& m, int k, float rcap) {
 }
 
 pm_context* g_legacy = nullptr;
+
+}  // namespace
+
+int pm::records_to_host(const DevMap& m, float* pos, float* power, float* dir) {
+  const uint32_t n = m.n;
+  if (n == 0) return PM_OK;
+  std::vector<float> x(n), y(n), z(n);
+  HIPCHK(hipMemcpy(x.data(), m.x, 4ull * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(y.data(), m.y, 4ull * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(z.data(), m.z, 4ull * n, hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < n; ++i) {
+    pos[3 * i] = x[i]; pos[3 * i + 1] = y[i]; pos[3 * i + 2] = z[i];
+  }
+  if (m.bf16) {
+    std::vector<uint16_t> p(8ull * n);
+    HIPCHK(hipMemcpy(p.data(), m.payload, 16ull * n, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i) {
+      for (int k = 0; k < 3; ++k) {
+        power[3 * i + k] = bf16_to_f32(p[8ull * i + k]);
+        dir[3 * i + k] = bf16_to_f32(p[8ull * i + 3 + k]);
+      }
+    }
+  } else {
+    std::vector<float> p(8ull * n);
+    HIPCHK(hipMemcpy(p.data(), m.payload, 32ull * n, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; ++i) {
+      for (int k = 0; k < 3; ++k) {
+        power[3 * i + k] = p[8ull * i + k];
+        dir[3 * i + k] = p[8ull * i + 3 + k];
+      }
+    }
+  }
+  return PM_OK;
+}
+
+int pm::tree_to_host(const DevMap& m, uint32_t* perm, float* split, uint32_t* axis) {
+  const uint32_t n = m.n, ni = m.nl > 0 ? m.nl - 1 : 0;
+  if (n == 0) return PM_OK;
+  std::vector<float4> t(n);
+  HIPCHK(hipMemcpy(t.data(), m.tpts, 16ull * n, hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < n; ++i) perm[i] = f2u(t[i].w);
+  if (ni) {
+    if (!split || !axis) return PM_ERR_INVALID;
+    std::vector<KdNode> nd(ni);
+    HIPCHK(hipMemcpy(nd.data(), m.nodes, sizeof(KdNode) * ni, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < ni; ++i) {
+      split[i] = nd[i].split;
+      axis[i] = (uint32_t)kd_axis(nd[i].meta);
+    }
+  }
+  return PM_OK;
+}
+
+namespace {
 
 void legacy_fail(const char* what, int st) {
   fprintf(stderr, "photon mapper error: %s failed: %s (%s).\n", what, pm_status_string(st), g_last_err);
@@ -401,34 +459,7 @@ int pm_get_records(pm_context* c, int map, float* pos, float* power, float* dir,
   if (!c->emitted) return PM_ERR_STATE;
   const DevMap& m = map == PM_MAP_VOLUME ? c->mv.m : c->ms.m;
   if (n != m.n) return PM_ERR_INVALID;
-  if (n == 0) return PM_OK;
-  std::vector<float> x(n), y(n), z(n);
-  HIPCHK(hipMemcpy(x.data(), m.x, 4ull * n, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(y.data(), m.y, 4ull * n, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(z.data(), m.z, 4ull * n, hipMemcpyDeviceToHost));
-  for (uint32_t i = 0; i < n; ++i) {
-    pos[3 * i] = x[i]; pos[3 * i + 1] = y[i]; pos[3 * i + 2] = z[i];
-  }
-  if (m.bf16) {
-    std::vector<uint16_t> p(8ull * n);
-    HIPCHK(hipMemcpy(p.data(), m.payload, 16ull * n, hipMemcpyDeviceToHost));
-    for (uint32_t i = 0; i < n; ++i) {
-      for (int k = 0; k < 3; ++k) {
-        power[3 * i + k] = bf16_to_f32(p[8ull * i + k]);
-        dir[3 * i + k] = bf16_to_f32(p[8ull * i + 3 + k]);
-      }
-    }
-  } else {
-    std::vector<float> p(8ull * n);
-    HIPCHK(hipMemcpy(p.data(), m.payload, 32ull * n, hipMemcpyDeviceToHost));
-    for (uint32_t i = 0; i < n; ++i) {
-      for (int k = 0; k < 3; ++k) {
-        power[3 * i + k] = p[8ull * i + k];
-        dir[3 * i + k] = p[8ull * i + 3 + k];
-      }
-    }
-  }
-  return PM_OK;
+  return records_to_host(m, pos, power, dir);
 }
 
 int pm_get_tree(pm_context* c, int map, uint32_t* perm, float* split, uint32_t* axis, uint32_t n, uint32_t ni) {
@@ -437,20 +468,7 @@ int pm_get_tree(pm_context* c, int map, uint32_t* perm, float* split, uint32_t* 
   const DevMap& m = map == PM_MAP_VOLUME ? c->mv.m : c->ms.m;
   uint32_t want_ni = m.nl > 0 ? m.nl - 1 : 0;
   if (n != m.n || ni != want_ni) return PM_ERR_INVALID;
-  if (n == 0) return PM_OK;
-  std::vector<float4> t(n);
-  HIPCHK(hipMemcpy(t.data(), m.tpts, 16ull * n, hipMemcpyDeviceToHost));
-  for (uint32_t i = 0; i < n; ++i) perm[i] = f2u(t[i].w);
-  if (ni) {
-    if (!split || !axis) return PM_ERR_INVALID;
-    std::vector<KdNode> nd(ni);
-    HIPCHK(hipMemcpy(nd.data(), m.nodes, sizeof(KdNode) * ni, hipMemcpyDeviceToHost));
-    for (uint32_t i = 0; i < ni; ++i) {
-      split[i] = nd[i].split;
-      axis[i] = (uint32_t)kd_axis(nd[i].meta);
-    }
-  }
-  return PM_OK;
+  return tree_to_host(m, perm, split, axis);
 }
 
 int pm_knn(pm_context* c, int map, const float* q, uint32_t nq, int k, float rcap, uint32_t* idx, float* d2,

@@ -1,0 +1,645 @@
+// pm_batch.hip -- batched scenes behind the pm_batch_* C ABI.
+//
+// A frame of a batch is one hipGraph launch of:
+//   k_trace_count_batch -> scans -> k_trace_write_batch -> k_tally_final_batch
+//   (every scene's photons; scene s owns a block-aligned range of trace
+//   blocks, so its records, offsets and tally are those of its own run),
+//   kd_build_forest over the 2 x n_scenes photon maps (one level loop for all
+//   trees, per-tree sorted lists from one global sort plus a stable pass on
+//   the tree id), and one persistent k_render_batch over a device work queue
+//   of (scene, tile) items, most expensive scenes first.
+// The sizing pass at creation runs the counting part once to learn the record
+// counts, so the arenas, the forest tables and the work list are fixed before
+// the graph is captured; every replay recomputes the whole frame from scratch.
+// No reference counterpart (the reference renders one scene per frame,
+// photonMappingKernel.cu:1523-1566).
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "pm_context.h"
+
+using namespace pm;
+
+struct pm_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int ns = 0;
+  std::vector<pm_config> cfg;
+  DevBuf spheres_d, density_d;
+  // emission
+  uint32_t tblocks = 0;
+  std::vector<BatchTraceScene> tsc;
+  DevBuf tsc_d, block_scene_d, cnt_s, cnt_v, off_s, off_v, scan_tmp, small, tally_part, tally9;
+  // records: surface arena [0, S_tot) then volume arena [S_tot, S_tot + V_tot)
+  uint32_t S_tot = 0, V_tot = 0;
+  std::vector<uint32_t> nrec_s, nrec_v, base_s, base_v;
+  int bf16 = 0;
+  DevBuf x, y, z, payload, tpts;
+  DevMap ms, mv;
+  // forest: tree t < ns is scene t's surface map, tree ns + t its volume map
+  int L = 0;
+  uint32_t nnodes = 0;
+  std::vector<uint32_t> level_off, tree_nbase, tree_nl, tree_pbase_h, tree_n;
+  DevBuf nodes, nrange, point_tree, tree_pbase, tree_root, node_lo, node_hi, node_lc, node_lb, node_child,
+      node_pbase, level_nodes, build_tmp;
+  // render
+  std::vector<BatchRenderScene> rsc;
+  std::vector<uint64_t> img_off;
+  uint64_t npix = 0;
+  std::vector<uint2> items_h;
+  uint32_t nitems_tune = 0;
+  DevBuf rsc_d, items_d, items_tune_d, head, rgb, counters;
+  int fp64 = 0, cone = 0, kmax = 1, cus = 1, grid = 1;
+  pm_batch_tuning tune;
+  // graph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  // last frame
+  pm_batch_stats st;
+  std::vector<double> tally;
+};
+
+namespace {
+
+void release_graph(pm_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+int setup_scenes(pm_batch* b) {
+  std::vector<Sphere> sph;
+  std::vector<size_t> sph_off(b->ns), dens_off(b->ns, SIZE_MAX);
+  std::vector<float> dens;
+  for (int i = 0; i < b->ns; ++i) {
+    std::vector<Sphere> s = build_spheres(b->cfg[i]);
+    sph_off[i] = sph.size();
+    sph.insert(sph.end(), s.begin(), s.end());
+    std::vector<float> d = build_density(b->cfg[i]);
+    if (!d.empty()) {
+      dens_off[i] = dens.size();
+      dens.insert(dens.end(), d.begin(), d.end());
+    }
+  }
+  HIPCHK(b->spheres_d.ensure(sizeof(Sphere) * std::max<size_t>(sph.size(), 1)));
+  if (!sph.empty()) HIPCHK(hipMemcpy(b->spheres_d.p, sph.data(), sizeof(Sphere) * sph.size(), hipMemcpyHostToDevice));
+  if (!dens.empty()) {
+    HIPCHK(b->density_d.ensure(sizeof(float) * dens.size()));
+    HIPCHK(hipMemcpy(b->density_d.p, dens.data(), sizeof(float) * dens.size(), hipMemcpyHostToDevice));
+  }
+  const uint32_t TT = trace_threads();
+  b->tsc.resize(b->ns);
+  b->tblocks = 0;
+  std::vector<uint32_t> block_scene;
+  for (int i = 0; i < b->ns; ++i) {
+    const pm_config& c = b->cfg[i];
+    BatchTraceScene& t = b->tsc[i];
+    memset(&t, 0, sizeof(t));
+    t.sc = make_scene(c, b->spheres_d.as<Sphere>() + sph_off[i],
+                      dens_off[i] == SIZE_MAX ? nullptr : b->density_d.as<float>() + dens_off[i]);
+    t.tp = trace_params(c);
+    t.n_emit = c.n_emit;
+    t.block0 = b->tblocks;
+    t.nblocks = ceil_div(c.n_emit, TT);
+    b->tblocks += t.nblocks;
+    block_scene.insert(block_scene.end(), t.nblocks, (uint32_t)i);
+  }
+  HIPCHK(b->tsc_d.ensure(sizeof(BatchTraceScene) * b->ns));
+  HIPCHK(hipMemcpy(b->tsc_d.p, b->tsc.data(), sizeof(BatchTraceScene) * b->ns, hipMemcpyHostToDevice));
+  HIPCHK(b->block_scene_d.ensure(4ull * b->tblocks));
+  HIPCHK(hipMemcpy(b->block_scene_d.p, block_scene.data(), 4ull * b->tblocks, hipMemcpyHostToDevice));
+  return PM_OK;
+}
+
+// Counting pass outside the graph: per-scene record counts and bases; arenas.
+int size_records(pm_batch* b) {
+  hipStream_t st = b->stream;
+  const size_t np = (size_t)b->tblocks * trace_threads();
+  HIPCHK(b->cnt_s.ensure(4 * np));
+  HIPCHK(b->cnt_v.ensure(4 * np));
+  HIPCHK(b->off_s.ensure(4 * np));
+  HIPCHK(b->off_v.ensure(4 * np));
+  HIPCHK(b->scan_tmp.ensure(4 * scan_temp_words(np)));
+  HIPCHK(b->small.ensure(64));
+  uint32_t* small = b->small.as<uint32_t>();
+  HIPCHK(launch_trace_count_batch(b->tsc_d.as<BatchTraceScene>(), b->block_scene_d.as<uint32_t>(), b->tblocks,
+                                  b->cnt_s.as<uint32_t>(), b->cnt_v.as<uint32_t>(), st));
+  HIPCHK(exclusive_scan(b->cnt_s.as<uint32_t>(), b->off_s.as<uint32_t>(), np, b->scan_tmp.as<uint32_t>(), small, st));
+  HIPCHK(exclusive_scan(b->cnt_v.as<uint32_t>(), b->off_v.as<uint32_t>(), np, b->scan_tmp.as<uint32_t>(), small + 1,
+                        st));
+  std::vector<uint32_t> first(b->ns);
+  for (int i = 0; i < b->ns; ++i) first[i] = b->tsc[i].block0 * trace_threads();
+  DevBuf idx, got;
+  HIPCHK(idx.ensure(4ull * b->ns));
+  HIPCHK(got.ensure(8ull * b->ns));
+  HIPCHK(hipMemcpyAsync(idx.p, first.data(), 4ull * b->ns, hipMemcpyHostToDevice, st));
+  HIPCHK(launch_gather_u32(b->off_s.as<uint32_t>(), idx.as<uint32_t>(), b->ns, got.as<uint32_t>(), st));
+  HIPCHK(launch_gather_u32(b->off_v.as<uint32_t>(), idx.as<uint32_t>(), b->ns, got.as<uint32_t>() + b->ns, st));
+  std::vector<uint32_t> h(2 * b->ns);
+  uint32_t tot[2];
+  HIPCHK(hipMemcpyAsync(h.data(), got.p, 8ull * b->ns, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(tot, small, 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  idx.release();
+  got.release();
+  if ((uint64_t)tot[0] + tot[1] >= (1ull << 31)) return PM_ERR_CAPACITY;
+  b->S_tot = tot[0];
+  b->V_tot = tot[1];
+  b->base_s.assign(h.begin(), h.begin() + b->ns);
+  b->base_v.assign(h.begin() + b->ns, h.end());
+  b->nrec_s.resize(b->ns);
+  b->nrec_v.resize(b->ns);
+  for (int i = 0; i < b->ns; ++i) {
+    b->nrec_s[i] = (i + 1 < b->ns ? b->base_s[i + 1] : b->S_tot) - b->base_s[i];
+    b->nrec_v[i] = (i + 1 < b->ns ? b->base_v[i + 1] : b->V_tot) - b->base_v[i];
+  }
+  const size_t N = std::max<size_t>((size_t)b->S_tot + b->V_tot, 1);
+  const size_t stride = b->bf16 ? 16 : 32;
+  HIPCHK(b->x.ensure(4 * N));
+  HIPCHK(b->y.ensure(4 * N));
+  HIPCHK(b->z.ensure(4 * N));
+  HIPCHK(b->payload.ensure(stride * N));
+  HIPCHK(b->tpts.ensure(16 * N));
+  for (int v = 0; v < 2; ++v) {
+    DevMap& m = v ? b->mv : b->ms;
+    memset(&m, 0, sizeof(m));
+    const size_t base = v ? b->S_tot : 0;
+    m.n = m.cap = v ? b->V_tot : b->S_tot;
+    m.bf16 = b->bf16;
+    m.x = b->x.as<float>() + base;
+    m.y = b->y.as<float>() + base;
+    m.z = b->z.as<float>() + base;
+    m.payload = (char*)b->payload.p + stride * base;
+    m.tpts = b->tpts.as<float4>() + base;
+  }
+  HIPCHK(b->tally_part.ensure(sizeof(double) * 9 * std::max<uint32_t>(b->tblocks, 1)));
+  HIPCHK(b->tally9.ensure(sizeof(double) * 9 * b->ns));
+  return PM_OK;
+}
+
+GatherMap tree_gather_map(const pm_batch* b, int scene, int vol, uint32_t tree) {
+  const pm_config& c = b->cfg[scene];
+  GatherMap g;
+  memset(&g, 0, sizeof(g));
+  const size_t stride = b->bf16 ? 16 : 32;
+  g.tpts = b->tpts.as<float4>() + b->tree_pbase_h[tree];
+  g.nodes = b->nodes.as<KdNode>() + b->tree_nbase[tree];
+  g.nrange = b->nrange.as<uint2>() + b->tree_nbase[tree];
+  g.payload = (const char*)b->payload.p + stride * b->tree_pbase_h[tree];
+  g.n = b->tree_n[tree];
+  g.nl = b->tree_nl[tree];
+  g.bf16 = b->bf16;
+  g.k = vol ? c.k_volume : c.k_surface;
+  const float rc = vol ? c.rcap_volume : c.rcap_surface;
+  g.rcap2 = rc * rc;
+  for (int j = 0; j < 3; ++j) g.scale[j] = c.light_power[j] / (float)c.n_emit;
+  return g;
+}
+
+// Forest tables for leaf size L, the per-scene gather maps, the build scratch.
+int setup_forest(pm_batch* b, int L) {
+  const uint32_t nt = 2u * (uint32_t)b->ns;
+  b->tree_n.resize(nt);
+  b->tree_pbase_h.resize(nt);
+  b->tree_nl.resize(nt);
+  b->tree_nbase.resize(nt);
+  std::vector<std::vector<uint32_t>> lo(nt), hi(nt), lc(nt), lbl(nt);
+  uint32_t nn = 0, depth = 0;
+  for (uint32_t t = 0; t < nt; ++t) {
+    const int s = (int)(t % (uint32_t)b->ns);
+    const bool vol = t >= (uint32_t)b->ns;
+    b->tree_n[t] = vol ? b->nrec_v[s] : b->nrec_s[s];
+    b->tree_pbase_h[t] = vol ? b->S_tot + b->base_v[s] : b->base_s[s];
+    b->tree_nl[t] = kd_num_leaves(b->tree_n[t], (uint32_t)L);
+    b->tree_nbase[t] = nn;
+    shape_tables(b->tree_n[t], (uint32_t)L, lo[t], hi[t], lc[t], lbl[t]);
+    const uint32_t ni = (uint32_t)lo[t].size();
+    nn += ni;
+    uint32_t d = 0;
+    for (uint64_t f = 0; f < ni; f = 2 * f + 1) ++d;
+    depth = std::max(depth, d);
+  }
+  b->nnodes = nn;
+  std::vector<uint32_t> nlo(nn), nhi(nn), nlc(nn), nlb(nn), nchild(2ull * nn), npb(nn), lvl, root(nt), pbase(nt);
+  b->level_off.assign(1, 0u);
+  for (uint32_t l = 0; l < depth; ++l) {
+    uint32_t run = 0;
+    const uint64_t f = (1ull << l) - 1ull, e = (2ull << l) - 1ull;
+    for (uint32_t t = 0; t < nt; ++t) {  // trees in point order
+      const uint32_t ni = (uint32_t)lo[t].size(), nb = b->tree_nbase[t], pb = b->tree_pbase_h[t];
+      for (uint64_t h = f; h < std::min<uint64_t>(e, ni); ++h) {
+        const uint32_t id = nb + (uint32_t)h;
+        nlo[id] = pb + lo[t][h];
+        nhi[id] = pb + hi[t][h];
+        nlc[id] = lc[t][h];
+        nlb[id] = run;
+        run += lc[t][h];
+        const uint64_t cl = 2 * h + 1, cr = 2 * h + 2;
+        nchild[2ull * id] = cl < ni ? nb + (uint32_t)cl : KD_LEAF;
+        nchild[2ull * id + 1] = cr < ni ? nb + (uint32_t)cr : KD_LEAF;
+        npb[id] = pb;
+        lvl.push_back(id);
+      }
+    }
+    b->level_off.push_back((uint32_t)lvl.size());
+  }
+  for (uint32_t t = 0; t < nt; ++t) {
+    root[t] = lo[t].empty() ? KD_LEAF : b->tree_nbase[t];
+    pbase[t] = b->tree_pbase_h[t];
+  }
+  const size_t N = (size_t)b->S_tot + b->V_tot;
+  const size_t n1 = std::max<size_t>(nn, 1);
+  HIPCHK(b->nodes.ensure(sizeof(KdNode) * n1));
+  HIPCHK(b->nrange.ensure(sizeof(uint2) * n1));
+  DevBuf* tabs[6] = {&b->node_lo, &b->node_hi, &b->node_lc, &b->node_lb, &b->node_pbase, &b->level_nodes};
+  const std::vector<uint32_t>* src[6] = {&nlo, &nhi, &nlc, &nlb, &npb, &lvl};
+  for (int i = 0; i < 6; ++i) {
+    HIPCHK(tabs[i]->ensure(4 * n1));
+    if (!src[i]->empty())
+      HIPCHK(hipMemcpy(tabs[i]->p, src[i]->data(), 4 * src[i]->size(), hipMemcpyHostToDevice));
+  }
+  HIPCHK(b->node_child.ensure(8 * n1));
+  if (nn) HIPCHK(hipMemcpy(b->node_child.p, nchild.data(), 8ull * nn, hipMemcpyHostToDevice));
+  HIPCHK(b->tree_root.ensure(4ull * nt));
+  HIPCHK(b->tree_pbase.ensure(4ull * nt));
+  HIPCHK(hipMemcpy(b->tree_root.p, root.data(), 4ull * nt, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->tree_pbase.p, pbase.data(), 4ull * nt, hipMemcpyHostToDevice));
+  HIPCHK(b->point_tree.ensure(4 * std::max<size_t>(N, 1)));
+  for (uint32_t t = 0; t < nt; ++t)
+    HIPCHK(launch_fill_u32(b->point_tree.as<uint32_t>() + b->tree_pbase_h[t], t, b->tree_n[t], b->stream));
+  HIPCHK(b->build_tmp.ensure(4 * kdforest_temp_words((uint32_t)N, nn)));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->L = L;
+  // per-scene render descriptors (the gather maps point into the forest)
+  b->rsc.resize(b->ns);
+  for (int s = 0; s < b->ns; ++s) {
+    const pm_config& c = b->cfg[s];
+    BatchRenderScene& r = b->rsc[s];
+    memset(&r, 0, sizeof(r));
+    r.sc = b->tsc[s].sc;
+    r.rp = render_params(c);
+    r.gs = tree_gather_map(b, s, 0, (uint32_t)s);
+    r.gv = tree_gather_map(b, s, 1, (uint32_t)(b->ns + s));
+    r.exposure = c.exposure;
+    r.w = c.width;
+    r.h = c.height;
+    r.img_off = b->img_off[s];
+  }
+  HIPCHK(b->rsc_d.ensure(sizeof(BatchRenderScene) * b->ns));
+  HIPCHK(hipMemcpy(b->rsc_d.p, b->rsc.data(), sizeof(BatchRenderScene) * b->ns, hipMemcpyHostToDevice));
+  return PM_OK;
+}
+
+// Image arena and the work list: the 8x8 tiles of every scene, scenes in
+// decreasing order of expected cost per tile (march length x log of the map
+// sizes x k), so the persistent grid ends on the cheapest tiles.
+int setup_render(pm_batch* b) {
+  b->img_off.resize(b->ns);
+  b->npix = 0;
+  std::vector<double> cost(b->ns);
+  std::vector<int> order(b->ns);
+  for (int s = 0; s < b->ns; ++s) {
+    const pm_config& c = b->cfg[s];
+    b->img_off[s] = b->npix;
+    b->npix += (uint64_t)c.width * (uint64_t)c.height;
+    const bool media = c.media_on && c.medium != PM_MEDIUM_NONE;
+    cost[s] = c.spp * ((double)c.k_surface * std::log2(b->nrec_s[s] + 2.0) +
+                       (media ? (double)c.k_volume * std::log2(b->nrec_v[s] + 2.0) * 1.5 / c.march_step : 0.0));
+    order[s] = s;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+  b->items_h.clear();
+  for (int s : order) {
+    const pm_config& c = b->cfg[s];
+    const uint32_t tiles = ceil_div(c.width, 8) * ceil_div(c.height, 8);
+    for (uint32_t t = 0; t < tiles; ++t) b->items_h.push_back(make_uint2((uint32_t)s, t));
+  }
+  HIPCHK(b->items_d.ensure(sizeof(uint2) * std::max<size_t>(b->items_h.size(), 1)));
+  HIPCHK(hipMemcpy(b->items_d.p, b->items_h.data(), sizeof(uint2) * b->items_h.size(), hipMemcpyHostToDevice));
+  HIPCHK(b->rgb.ensure(12 * std::max<uint64_t>(b->npix, 1)));
+  HIPCHK(b->head.ensure(16));
+  HIPCHK(b->counters.ensure(16));
+  return PM_OK;
+}
+
+int set_grid(pm_batch* b) {
+  const int occ = render_batch_occupancy(b->fp64, b->cone, b->tune.block_threads, b->tune.heap_slots);
+  if (occ < 1) return PM_ERR_INVALID;
+  b->grid = b->cus * occ;
+  return PM_OK;
+}
+
+// Captures one frame over `nitems` work items into b->graph / b->exec.
+int capture(pm_batch* b, const uint2* items, uint32_t nitems) {
+  release_graph(b);
+  hipStream_t st = b->stream;
+  const size_t np = (size_t)b->tblocks * trace_threads();
+  uint32_t* small = b->small.as<uint32_t>();
+  KdForest F;
+  memset(&F, 0, sizeof(F));
+  F.n = b->S_tot + b->V_tot;
+  F.x = b->x.as<float>(); F.y = b->y.as<float>(); F.z = b->z.as<float>();
+  F.tpts = b->tpts.as<float4>();
+  F.nodes = b->nodes.as<KdNode>();
+  F.nrange = b->nrange.as<uint2>();
+  F.ntrees = 2u * (uint32_t)b->ns;
+  F.nnodes = b->nnodes;
+  F.point_tree = b->point_tree.as<uint32_t>();
+  F.tree_pbase = b->tree_pbase.as<uint32_t>();
+  F.tree_root = b->tree_root.as<uint32_t>();
+  F.node_lo = b->node_lo.as<uint32_t>();
+  F.node_hi = b->node_hi.as<uint32_t>();
+  F.node_lc = b->node_lc.as<uint32_t>();
+  F.node_lb = b->node_lb.as<uint32_t>();
+  F.node_child = b->node_child.as<uint32_t>();
+  F.node_pbase = b->node_pbase.as<uint32_t>();
+  F.level_nodes = b->level_nodes.as<uint32_t>();
+  BatchRenderLaunch R;
+  memset(&R, 0, sizeof(R));
+  R.scenes = b->rsc_d.as<BatchRenderScene>();
+  R.items = items;
+  R.nitems = nitems;
+  R.head = b->head.as<uint32_t>();
+  R.rgb = b->rgb.as<float>();
+  R.counters = b->counters.as<unsigned long long>();
+  R.heap_slots = b->tune.heap_slots;
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipSuccess;
+  auto step = [&](hipError_t r) {
+    if (e == hipSuccess) e = r;
+  };
+  step(launch_trace_count_batch(b->tsc_d.as<BatchTraceScene>(), b->block_scene_d.as<uint32_t>(), b->tblocks,
+                                b->cnt_s.as<uint32_t>(), b->cnt_v.as<uint32_t>(), st));
+  step(exclusive_scan(b->cnt_s.as<uint32_t>(), b->off_s.as<uint32_t>(), np, b->scan_tmp.as<uint32_t>(), small, st));
+  step(exclusive_scan(b->cnt_v.as<uint32_t>(), b->off_v.as<uint32_t>(), np, b->scan_tmp.as<uint32_t>(), small + 1,
+                      st));
+  step(launch_trace_write_batch(b->tsc_d.as<BatchTraceScene>(), b->block_scene_d.as<uint32_t>(), b->tblocks,
+                                b->off_s.as<uint32_t>(), b->off_v.as<uint32_t>(), b->ms, b->mv,
+                                b->tally_part.as<double>(), st));
+  step(launch_tally_final_batch(b->tally_part.as<double>(), b->tsc_d.as<BatchTraceScene>(), (uint32_t)b->ns,
+                                b->tally9.as<double>(), st));
+  step(kd_build_forest(F, b->level_off.data(), (uint32_t)b->level_off.size() - 1u, b->build_tmp.as<uint32_t>(), st));
+  step(launch_fill_u32(b->head.as<uint32_t>(), 0u, 4, st));
+  step(launch_fill_u32(b->counters.as<uint32_t>(), 0u, 4, st));
+  step(launch_render_batch(R, b->fp64, b->cone, b->tune.block_threads, b->grid, st));
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(st, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    set_last_hip(e != hipSuccess ? e : e2, "batch graph capture");
+    return PM_ERR_HIP;
+  }
+  b->graph = g;
+  HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  size_t n = 0;
+  HIPCHK(hipGraphGetNodes(b->graph, nullptr, &n));
+  b->st.graph_nodes = (uint32_t)n;
+  return PM_OK;
+}
+
+int run_frame(pm_batch* b, double* ms) {
+  HIPCHK(hipEventRecord(b->e0, b->stream));
+  HIPCHK(hipGraphLaunch(b->exec, b->stream));
+  HIPCHK(hipEventRecord(b->e1, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  float t = 0.0f;
+  HIPCHK(hipEventElapsedTime(&t, b->e0, b->e1));
+  *ms = t;
+  return PM_OK;
+}
+
+int apply_tuning(pm_batch* b, const pm_batch_tuning& t, const uint2* items, uint32_t nitems) {
+  if (t.leaf_size < 1 || t.leaf_size > 64) return PM_ERR_INVALID;
+  if (t.block_threads != 64 && t.block_threads != 128 && t.block_threads != 192 && t.block_threads != 256)
+    return PM_ERR_INVALID;
+  if ((t.heap_slots != 32 && t.heap_slots != 64) || t.heap_slots < b->kmax) return PM_ERR_INVALID;
+  int r;
+  b->tune = t;
+  if ((r = set_grid(b)) != PM_OK) return r;
+  if (t.leaf_size != b->L && (r = setup_forest(b, t.leaf_size)) != PM_OK) return r;
+  return capture(b, items, nitems);
+}
+
+void destroy(pm_batch* b) {
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  release_graph(b);
+  for (DevBuf* d : {&b->spheres_d, &b->density_d, &b->tsc_d, &b->block_scene_d, &b->cnt_s, &b->cnt_v, &b->off_s,
+                    &b->off_v, &b->scan_tmp, &b->small, &b->tally_part, &b->tally9, &b->x, &b->y, &b->z,
+                    &b->payload, &b->tpts, &b->nodes, &b->nrange, &b->point_tree, &b->tree_pbase, &b->tree_root,
+                    &b->node_lo, &b->node_hi, &b->node_lc, &b->node_lb, &b->node_child, &b->node_pbase,
+                    &b->level_nodes, &b->build_tmp, &b->rsc_d, &b->items_d, &b->items_tune_d, &b->head, &b->rgb,
+                    &b->counters})
+    d->release();
+  if (b->e0) (void)hipEventDestroy(b->e0);
+  if (b->e1) (void)hipEventDestroy(b->e1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+bool scene_ok(const pm_batch* b, int scene) { return b && scene >= 0 && scene < b->ns; }
+
+DevMap scene_map(const pm_batch* b, int scene, int map) {
+  const uint32_t t = (uint32_t)scene + (map == PM_MAP_VOLUME ? (uint32_t)b->ns : 0u);
+  const DevMap& a = map == PM_MAP_VOLUME ? b->mv : b->ms;
+  const size_t off = map == PM_MAP_VOLUME ? b->base_v[scene] : b->base_s[scene];
+  const size_t stride = b->bf16 ? 16 : 32;
+  DevMap m = a;
+  m.n = m.cap = b->tree_n[t];
+  m.x = a.x + off;
+  m.y = a.y + off;
+  m.z = a.z + off;
+  m.payload = (char*)a.payload + stride * off;
+  m.tpts = b->tpts.as<float4>() + b->tree_pbase_h[t];
+  m.nodes = b->nodes.as<KdNode>() + b->tree_nbase[t];
+  m.nrange = b->nrange.as<uint2>() + b->tree_nbase[t];
+  m.L = (uint32_t)b->L;
+  m.nl = b->tree_nl[t];
+  return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pm_batch_scene_config(uint32_t seed, int index, pm_config* out) {
+  if (!out) return PM_ERR_INVALID;
+  return batch_scene_config(seed, index, out) == 0 ? PM_OK : PM_ERR_INVALID;
+}
+
+int pm_batch_create(const pm_config* cfgs, int n, pm_batch** out) {
+  if (!cfgs || !out || n < 1 || n > 128) return PM_ERR_INVALID;
+  for (int i = 0; i < n; ++i) {
+    const pm_config& c = cfgs[i];
+    if (validate_config(c) != 0 || c.n_emit == 0 || c.cap_surface || c.cap_volume) return PM_ERR_INVALID;
+    if (c.storage != cfgs[0].storage || c.estimator != cfgs[0].estimator ||
+        c.fp64_transmittance != cfgs[0].fp64_transmittance)
+      return PM_ERR_INVALID;
+  }
+  pm_batch* b = new pm_batch();
+  b->ns = n;
+  b->cfg.assign(cfgs, cfgs + n);
+  b->bf16 = cfgs[0].storage == PM_STORE_BF16;
+  b->fp64 = cfgs[0].fp64_transmittance != 0;
+  b->cone = cfgs[0].estimator == PM_EST_CONE;
+  for (int i = 0; i < n; ++i) b->kmax = std::max(b->kmax, std::max(cfgs[i].k_surface, cfgs[i].k_volume));
+  memset(&b->st, 0, sizeof(b->st));
+  hipError_t e = hipGetDevice(&b->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&b->e0);
+  if (e == hipSuccess) e = hipEventCreate(&b->e1);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device);
+  if (e != hipSuccess) {
+    set_last_hip(e, "pm_batch_create");
+    destroy(b);
+    return PM_ERR_HIP;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  pm_batch_tuning t;
+  t.leaf_size = 32;
+  t.block_threads = 64;
+  t.heap_slots = b->kmax <= 32 ? 32 : 64;
+  int r = setup_scenes(b);
+  if (r == PM_OK) r = size_records(b);
+  if (r == PM_OK) r = setup_render(b);
+  if (r == PM_OK) r = apply_tuning(b, t, b->items_d.as<uint2>(), (uint32_t)b->items_h.size());
+  if (r != PM_OK) {
+    destroy(b);
+    return r;
+  }
+  b->st.ms_setup = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  b->tally.assign(9ull * n, 0.0);
+  *out = b;
+  return PM_OK;
+}
+
+int pm_batch_destroy(pm_batch* b) {
+  if (!b) return PM_ERR_INVALID;
+  destroy(b);
+  return PM_OK;
+}
+
+int pm_batch_set_tuning(pm_batch* b, const pm_batch_tuning* t) {
+  if (!b || !t) return PM_ERR_INVALID;
+  return apply_tuning(b, *t, b->items_d.as<uint2>(), (uint32_t)b->items_h.size());
+}
+
+int pm_batch_autotune(pm_batch* b, int sample_stride, pm_batch_tuning* best_out, double* best_ms_out) {
+  if (!b || sample_stride < 1) return PM_ERR_INVALID;
+  std::vector<uint2> sub;
+  for (size_t i = 0; i < b->items_h.size(); i += (size_t)sample_stride) sub.push_back(b->items_h[i]);
+  HIPCHK(b->items_tune_d.ensure(sizeof(uint2) * std::max<size_t>(sub.size(), 1)));
+  if (!sub.empty())
+    HIPCHK(hipMemcpy(b->items_tune_d.p, sub.data(), sizeof(uint2) * sub.size(), hipMemcpyHostToDevice));
+  const pm_batch_tuning start = b->tune;
+  pm_batch_tuning best = start;
+  double best_ms = -1.0;
+  const int leaves[4] = {4, 8, 16, 32}, blocks[3] = {64, 128, 256}, slots[2] = {32, 64};
+  for (int li = 0; li < 4; ++li)
+    for (int bi = 0; bi < 3; ++bi)
+      for (int si = 0; si < 2; ++si) {
+        pm_batch_tuning t;
+        t.leaf_size = leaves[li];
+        t.block_threads = blocks[bi];
+        t.heap_slots = slots[si];
+        if (t.heap_slots < b->kmax) continue;
+        int r = apply_tuning(b, t, b->items_tune_d.as<uint2>(), (uint32_t)sub.size());
+        if (r != PM_OK) return r;
+        double ms = 0.0;
+        if ((r = run_frame(b, &ms)) != PM_OK) return r;  // warm-up
+        if ((r = run_frame(b, &ms)) != PM_OK) return r;
+        if (best_ms < 0.0 || ms < best_ms) {
+          best_ms = ms;
+          best = t;
+        }
+      }
+  int r = apply_tuning(b, best, b->items_d.as<uint2>(), (uint32_t)b->items_h.size());
+  if (r != PM_OK) return r;
+  if (best_out) *best_out = best;
+  if (best_ms_out) *best_ms_out = best_ms;
+  return PM_OK;
+}
+
+int pm_batch_submit(pm_batch* b) {
+  if (!b || !b->exec) return PM_ERR_INVALID;
+  double ms = 0.0;
+  int r = run_frame(b, &ms);
+  if (r != PM_OK) return r;
+  unsigned long long q[2];
+  HIPCHK(hipMemcpy(q, b->counters.p, 16, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(b->tally.data(), b->tally9.p, sizeof(double) * 9 * b->ns, hipMemcpyDeviceToHost));
+  pm_batch_stats& s = b->st;
+  s.n_scenes = (uint32_t)b->ns;
+  s.work_items = (uint32_t)b->items_h.size();
+  s.photons = 0;
+  for (int i = 0; i < b->ns; ++i) s.photons += b->cfg[i].n_emit;
+  s.surface_records = b->S_tot;
+  s.volume_records = b->V_tot;
+  s.pixels = b->npix;
+  s.surface_queries = q[0];
+  s.volume_queries = q[1];
+  s.ms_frame = ms;
+  s.tuning = b->tune;
+  return PM_OK;
+}
+
+int pm_batch_image(pm_batch* b, int scene, float* rgb_dev, pm_uchar4* out_dev) {
+  if (!scene_ok(b, scene)) return PM_ERR_INVALID;
+  const pm_config& c = b->cfg[scene];
+  const size_t np = (size_t)c.width * (size_t)c.height;
+  const float* src = b->rgb.as<float>() + 3 * b->img_off[scene];
+  if (rgb_dev) HIPCHK(hipMemcpyAsync(rgb_dev, src, 12 * np, hipMemcpyDeviceToDevice, b->stream));
+  if (out_dev) HIPCHK(launch_tonemap(src, np, c.exposure, (uchar4*)out_dev, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return PM_OK;
+}
+
+int pm_batch_scene_stats(pm_batch* b, int scene, pm_stats* out) {
+  if (!scene_ok(b, scene) || !out) return PM_ERR_INVALID;
+  const pm_config& c = b->cfg[scene];
+  memset(out, 0, sizeof(*out));
+  out->n_surface = b->nrec_s[scene];
+  out->n_volume = b->nrec_v[scene];
+  out->n_emit_surface = out->n_emit_volume = out->n_emit_total = c.n_emit;
+  const double* t = b->tally.data() + 9ull * scene;
+  for (int k = 0; k < 3; ++k) {
+    const double w = (double)c.light_power[k] / (double)c.n_emit;
+    out->emitted[k] = (double)c.light_power[k];
+    out->absorbed[k] = t[k] * w;
+    out->escaped[k] = t[3 + k] * w;
+    out->rr_gain[k] = t[6 + k] * w;
+  }
+  return PM_OK;
+}
+
+int pm_batch_stats_get(pm_batch* b, pm_batch_stats* out) {
+  if (!b || !out) return PM_ERR_INVALID;
+  *out = b->st;
+  out->tuning = b->tune;
+  return PM_OK;
+}
+
+int pm_batch_get_records(pm_batch* b, int scene, int map, float* pos, float* power, float* dir, uint32_t n) {
+  if (!scene_ok(b, scene) || !pos || !power || !dir) return PM_ERR_INVALID;
+  DevMap m = scene_map(b, scene, map);
+  if (n != m.n) return PM_ERR_INVALID;
+  return records_to_host(m, pos, power, dir);
+}
+
+int pm_batch_get_tree(pm_batch* b, int scene, int map, uint32_t* perm, float* split, uint32_t* axis, uint32_t n,
+                      uint32_t ni) {
+  if (!scene_ok(b, scene) || !perm) return PM_ERR_INVALID;
+  DevMap m = scene_map(b, scene, map);
+  if (n != m.n || ni != (m.nl > 0 ? m.nl - 1 : 0)) return PM_ERR_INVALID;
+  return tree_to_host(m, perm, split, axis);
+}
+
+}  // extern "C"

@@ -151,9 +151,10 @@ int cache_train_dev(pm_context* c, const float* in, const float* target, uint32_
   if (batch == 0 || batch > kChunk) return PM_ERR_INVALID;
   uint16_t* tmp[MLP_HIDDEN_LAYERS];
   uint16_t* const* H = hidden_ptrs(k, tmp);
-  double total = 0.0;
   uint64_t cnt = 0;
-  std::vector<float> err(batch);
+  HIPCHK(c->small.ensure(64));
+  double* dloss = reinterpret_cast<double*>(c->small.as<uint32_t>() + 12);
+  HIPCHK(hipMemsetAsync(dloss, 0, sizeof(double), c->stream));
   for (uint32_t o = 0; o + batch <= n; o += batch) {
     HIPCHK(mlp_encode(in + (size_t)o * MLP_IN, batch, k.X.as<uint16_t>(), c->stream));
     HIPCHK(mlp_forward(k.w, k.X.as<uint16_t>(), batch, H, k.y.as<float>(), c->stream));
@@ -162,11 +163,12 @@ int cache_train_dev(pm_context* c, const float* in, const float* target, uint32_
     if (esum) HIPCHK(launch_err_accum(in + (size_t)o * MLP_IN, k.tb.row_err, batch, esum, ecnt, c->stream));
     k.step += 1;
     HIPCHK(mlp_adam(k.w, k.tb, lr, k.step, c->stream));
-    HIPCHK(hipMemcpyAsync(err.data(), k.row_err.p, 4ull * batch, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    for (uint32_t i = 0; i < batch; ++i) total += err[i];
+    HIPCHK(launch_sum_rows(k.row_err.as<float>(), batch, dloss, c->stream));
     cnt += (uint64_t)batch * MLP_OUT;
   }
+  double total = 0.0;
+  HIPCHK(hipMemcpyAsync(&total, dloss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   if (loss) *loss = cnt ? total / (double)cnt : 0.0;
   return PM_OK;
 }

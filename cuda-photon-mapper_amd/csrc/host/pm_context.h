@@ -59,20 +59,25 @@ struct CacheState {
   // per-cell error tracker (fallback to the plain gather where it is high)
   DevBuf err_sum, err_cnt;
   // cache render configuration and per-frame buffers
-  float threshold = 0.02f;
+  float threshold = 0.05f;  // per-channel mean squared log error of a cell
   uint32_t train_samples = 1u << 18;
   uint32_t batch = 1u << 14;
   float lr = 1e-3f;
   uint32_t frame = 0;
-  int fused_infer = 1;
-  int stride = 1;         // march points per volume record
-  int cache_surface = 0;  // 0: surface queries are always gathered exactly  // 1: pm_mlp_infer.hip kernels (when the LDS fits), 0: generic GEMMs
+  int fused_infer = 1;     // 1: pm_mlp_infer.hip kernels (when the LDS fits), 0: generic GEMMs
+  int stride = 1;          // march points per volume record
+  int cache_surface = 0;   // 0: surface queries are always gathered exactly
   DevBuf rec_cnt, rec_off, rec_in, rec_wt, rec_pix, rec_val, sel_in, sel_t, flags, flag_off, list;
   pm_cache_stats st;
 };
 
 void set_last_hip(hipError_t e, const char* what);
 GatherMap make_gather_map(const pm_context* c, const DevMap& m, int k, float rcap);
+void shape_tables(uint32_t n, uint32_t L, std::vector<uint32_t>& lo, std::vector<uint32_t>& hi,
+                  std::vector<uint32_t>& lc, std::vector<uint32_t>& lb);
+// Host copies of a map's records / tree (the layouts of pm_get_records / pm_get_tree).
+int records_to_host(const DevMap& m, float* pos, float* power, float* dir);
+int tree_to_host(const DevMap& m, uint32_t* perm, float* split, uint32_t* axis);
 extern char g_last_err[256];
 
 #define HIPCHK(expr)                          \

@@ -27,7 +27,7 @@ namespace pm {
 // With march stride s > 1, one volume record stands for up to s consecutive
 // march points of the same segment (the first point's position, the sum of
 // their weights); a point farther than 1.5 s steps from the open record, or a
-// surface query, starts a new one.
+// surface query, or a point on a ray of another direction, starts a new one.
 struct RecordGather {
   bool write;
   uint32_t n;        // records of this lane so far
@@ -39,7 +39,7 @@ struct RecordGather {
   int stride;
   float reach2;      // (1.5 * stride * step)^2
   int open;          // points merged into the open volume record (0: none)
-  V3f open_x;
+  V3f open_x, open_d;
   __device__ void put(V3f x, V3f d, V3f nrm, V3f w) {
     if (write) {
       size_t i = (size_t)base + n;
@@ -58,7 +58,7 @@ struct RecordGather {
     return v3(0.0f, 0.0f, 0.0f);
   }
   __device__ V3f volume(V3f x, V3f wo, V3f w) {
-    if (open > 0 && open < stride) {
+    if (open > 0 && open < stride && wo.x == open_d.x && wo.y == open_d.y && wo.z == open_d.z) {
       float dx = x.x - open_x.x, dy = x.y - open_x.y, dz = x.z - open_x.z;
       if ((dx * dx + dy * dy) + dz * dz <= reach2) {
         if (write) {
@@ -74,6 +74,7 @@ struct RecordGather {
     put(x, wo, v3(0.0f, 0.0f, 0.0f), w);
     open = 1;
     open_x = x;
+    open_d = wo;
     return v3(0.0f, 0.0f, 0.0f);
   }
 };
@@ -195,8 +196,25 @@ __global__ void k_flags(const float* __restrict__ in, uint64_t n, const unsigned
   const bool surf = !(p[6] == 0.0f && p[7] == 0.0f && p[8] == 0.0f);
   uint32_t cell = err_cell(p);
   uint32_t c = ecnt[cell];
-  bool fb = (surf && !cache_surface) || c == 0u || (double)esum[cell] / 16777216.0 / (double)c > (double)threshold;
+  // mean over the cell's training rows of the per-channel squared log error
+  const double mse = (double)esum[cell] / 16777216.0 / ((double)c * (double)MLP_OUT);
+  bool fb = (surf && !cache_surface) || c == 0u || mse > (double)threshold;
   flags[i] = fb ? 1u : 0u;
+}
+
+// acc += sum of v[0..n) in a fixed order (one workgroup): the frame's training
+// loss without a host round trip per step.
+__global__ void __launch_bounds__(256) k_sum_rows(const float* __restrict__ v, uint32_t n, double* __restrict__ acc) {
+  __shared__ double part[256];
+  double s = 0.0;
+  for (uint32_t i = threadIdx.x; i < n; i += 256u) s += (double)v[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *acc += part[0];
 }
 
 __global__ void k_compact(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ off, uint64_t n,
@@ -264,6 +282,11 @@ hipError_t launch_err_accum(const float* in, const float* row_err, uint32_t rows
                             unsigned int* ecnt, hipStream_t st) {
   if (rows == 0) return hipSuccess;
   hipLaunchKernelGGL(k_err_accum, dim3((rows + 255) / 256), dim3(256), 0, st, in, row_err, rows, esum, ecnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_rows(const float* v, uint32_t n, double* acc, hipStream_t st) {
+  hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, st, v, n, acc);
   return hipGetLastError();
 }
 

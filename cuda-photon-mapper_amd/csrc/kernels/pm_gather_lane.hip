@@ -20,6 +20,52 @@
 
 namespace pm {
 
+// Radiance of pixel (px, py): the mean of its spp samples, compensated sum.
+// heap: this wave's LDS heap, `slots` keys per lane (k <= slots).
+template <bool FP64T, bool CONE>
+__device__ __forceinline__ V3f lane_pixel(const Scene& sc, const RenderParams& rp, const GatherMap& gs,
+                                          const GatherMap& gv, uint32_t* heap, int slots, int lane, uint32_t px,
+                                          uint32_t py, uint32_t& nsurf, uint32_t& nvol) {
+  LaneGather<CONE> g;
+  g.gs = gs;
+  g.gv = gv;
+  g.hp.H = heap;
+  g.hp.Lo = heap + slots * 64;
+  g.hp.lane = lane;
+  g.hp.k = 0;
+  g.hp.size = 0;
+  g.hg_g = sc.med.g;
+  g.nsurf = 0u;
+  g.nvol = 0u;
+  g.prev_r = -1.0f;
+  NSum3<float> acc;
+  nsum3_init(acc);
+  for (int s = 0; s < rp.spp; ++s) {
+    V3f v = FP64T ? render_sample_t<float, double>(sc, rp, px, py, (uint32_t)s, g)
+                  : render_sample_t<float, float>(sc, rp, px, py, (uint32_t)s, g);
+    nsum3_add(acc, v);
+  }
+  nsurf = g.nsurf;
+  nvol = g.nvol;
+  return nsum3_get(acc) * (1.0f / (float)rp.spp);
+}
+
+__device__ __forceinline__ void store_pixel(float* rgb, uchar4* u8, size_t o, V3f v, float exposure) {
+  if (rgb) {
+    rgb[3 * o] = v.x;
+    rgb[3 * o + 1] = v.y;
+    rgb[3 * o + 2] = v.z;
+  }
+  if (u8) {
+    uchar4 q;
+    q.x = tonemap_u8(v.x, exposure);
+    q.y = tonemap_u8(v.y, exposure);
+    q.z = tonemap_u8(v.z, exposure);
+    q.w = 0;
+    u8[o] = q;
+  }
+}
+
 // One wave (one workgroup) per 8x8 tile of the crop; lane = pixel.
 template <bool FP64T, bool CONE>
 __global__ void __launch_bounds__(64) k_render_lane(RenderLaunch r) {
@@ -31,43 +77,12 @@ __global__ void __launch_bounds__(64) k_render_lane(RenderLaunch r) {
   const int lx = tx * 8 + (lane & 7), ly = ty * 8 + (lane >> 3);
   if (lx >= r.w || ly >= r.h) return;
   const uint32_t px = (uint32_t)(r.x0 + lx), py = (uint32_t)(r.y0 + ly);
-  LaneGather<CONE> g;
-  g.gs = r.gs;
-  g.gv = r.gv;
-  g.hp.H = heap_lds;
-  g.hp.Lo = heap_lds + 64 * 64;
-  g.hp.lane = lane;
-  g.hp.k = 0;
-  g.hp.size = 0;
-  g.hg_g = r.sc.med.g;
-  g.nsurf = 0u;
-  g.nvol = 0u;
-  g.prev_r = -1.0f;
-  NSum3<float> acc;
-  nsum3_init(acc);
-  for (int s = 0; s < r.rp.spp; ++s) {
-    V3f v = FP64T ? render_sample_t<float, double>(r.sc, r.rp, px, py, (uint32_t)s, g)
-                  : render_sample_t<float, float>(r.sc, r.rp, px, py, (uint32_t)s, g);
-    nsum3_add(acc, v);
-  }
-  V3f v = nsum3_get(acc) * (1.0f / (float)r.rp.spp);
-  const size_t o = (size_t)ly * (size_t)r.w + (size_t)lx;
-  if (r.rgb) {
-    r.rgb[3 * o] = v.x;
-    r.rgb[3 * o + 1] = v.y;
-    r.rgb[3 * o + 2] = v.z;
-  }
-  if (r.u8) {
-    uchar4 q;
-    q.x = tonemap_u8(v.x, r.exposure);
-    q.y = tonemap_u8(v.y, r.exposure);
-    q.z = tonemap_u8(v.z, r.exposure);
-    q.w = 0;
-    r.u8[o] = q;
-  }
+  uint32_t ns, nv;
+  V3f v = lane_pixel<FP64T, CONE>(r.sc, r.rp, r.gs, r.gv, heap_lds, 64, lane, px, py, ns, nv);
+  store_pixel(r.rgb, r.u8, (size_t)ly * (size_t)r.w + (size_t)lx, v, r.exposure);
   if (r.counters) {
-    atomicAdd(&r.counters[0], (unsigned long long)g.nsurf);
-    atomicAdd(&r.counters[1], (unsigned long long)g.nvol);
+    atomicAdd(&r.counters[0], (unsigned long long)ns);
+    atomicAdd(&r.counters[1], (unsigned long long)nv);
   }
 }
 
@@ -80,6 +95,92 @@ hipError_t launch_render_lane(const RenderLaunch& r, hipStream_t st) {
   else if (fp64) hipLaunchKernelGGL((k_render_lane<true, false>), dim3(tiles), dim3(64), lds, st, r);
   else if (cone) hipLaunchKernelGGL((k_render_lane<false, true>), dim3(tiles), dim3(64), lds, st, r);
   else hipLaunchKernelGGL((k_render_lane<false, false>), dim3(tiles), dim3(64), lds, st, r);
+  return hipGetLastError();
+}
+
+// Batched render: one persistent grid over the 8x8 tiles of every scene of a
+// batch.  Each wave takes the next (scene, tile) item from a device work
+// queue (an atomic counter over an item list ordered by decreasing expected
+// cost) until the list is exhausted, so scenes of very different sizes keep
+// all waves busy.  Per pixel the arithmetic is lane_pixel's, as in the
+// one-scene render, so every scene's image is bit-identical to its own
+// render.  heap_slots (32 or 64 keys per lane) sizes each wave's LDS heap.
+template <bool FP64T, bool CONE>
+__global__ void __launch_bounds__(256) k_render_batch(BatchRenderLaunch r) {
+  extern __shared__ uint32_t lds[];
+  const int lane = lane_id();
+  const int wave = (int)(threadIdx.x >> 6);
+  uint32_t* heap = lds + (size_t)wave * 2u * (uint32_t)r.heap_slots * 64u;
+  unsigned long long ns_tot = 0, nv_tot = 0;
+  for (;;) {
+    uint32_t it = 0;
+    if (lane == 0) it = atomicAdd(r.head, 1u);
+    it = read_lane(it, 0);
+    if (it >= r.nitems) break;
+    const uint2 item = r.items[it];
+    const BatchRenderScene& S = r.scenes[item.x];
+    const int tiles_x = (S.w + 7) / 8;
+    const int tx = (int)item.y % tiles_x, ty = (int)item.y / tiles_x;
+    const int lx = tx * 8 + (lane & 7), ly = ty * 8 + (lane >> 3);
+    if (lx < S.w && ly < S.h) {
+      uint32_t ns, nv;
+      V3f v = lane_pixel<FP64T, CONE>(S.sc, S.rp, S.gs, S.gv, heap, r.heap_slots, lane, (uint32_t)lx, (uint32_t)ly,
+                                       ns, nv);
+      store_pixel(r.rgb + 3 * S.img_off, nullptr, (size_t)ly * (size_t)S.w + (size_t)lx, v, S.exposure);
+      ns_tot += ns;
+      nv_tot += nv;
+    }
+  }
+  if (r.counters) {
+    atomicAdd(&r.counters[0], ns_tot);
+    atomicAdd(&r.counters[1], nv_tot);
+  }
+}
+
+size_t render_batch_lds_bytes(int block_threads, int heap_slots) {
+  return (size_t)(block_threads / 64) * 2u * (size_t)heap_slots * 64u * sizeof(uint32_t);
+}
+
+hipError_t launch_render_batch(const BatchRenderLaunch& r, int fp64, int cone, int block_threads, int grid,
+                               hipStream_t st) {
+  if (r.nitems == 0) return hipSuccess;
+  const size_t lds = render_batch_lds_bytes(block_threads, r.heap_slots);
+  if (fp64 && cone) hipLaunchKernelGGL((k_render_batch<true, true>), dim3(grid), dim3(block_threads), lds, st, r);
+  else if (fp64) hipLaunchKernelGGL((k_render_batch<true, false>), dim3(grid), dim3(block_threads), lds, st, r);
+  else if (cone) hipLaunchKernelGGL((k_render_batch<false, true>), dim3(grid), dim3(block_threads), lds, st, r);
+  else hipLaunchKernelGGL((k_render_batch<false, false>), dim3(grid), dim3(block_threads), lds, st, r);
+  return hipGetLastError();
+}
+
+int render_batch_occupancy(int fp64, int cone, int block_threads, int heap_slots) {
+  const size_t lds = render_batch_lds_bytes(block_threads, heap_slots);
+  int nb = 0;
+  hipError_t e;
+  if (fp64 && cone)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<true, true>, block_threads, lds);
+  else if (fp64)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<true, false>, block_threads, lds);
+  else if (cone)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<false, true>, block_threads, lds);
+  else
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<false, false>, block_threads, lds);
+  return e == hipSuccess ? nb : 0;
+}
+
+__global__ void k_tonemap(const float* __restrict__ rgb, size_t npix, float exposure, uchar4* __restrict__ out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  uchar4 q;
+  q.x = tonemap_u8(rgb[3 * i], exposure);
+  q.y = tonemap_u8(rgb[3 * i + 1], exposure);
+  q.z = tonemap_u8(rgb[3 * i + 2], exposure);
+  q.w = 0;
+  out[i] = q;
+}
+
+hipError_t launch_tonemap(const float* rgb, size_t npix, float exposure, uchar4* out, hipStream_t st) {
+  if (npix == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tonemap, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st, rgb, npix, exposure, out);
   return hipGetLastError();
 }
 

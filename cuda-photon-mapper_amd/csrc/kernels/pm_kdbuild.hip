@@ -16,6 +16,11 @@
 //     needed.
 //  3. After the last level the x-list gives the leaf order (x key, index);
 //     positions are gathered into float4 (x, y, z, index) in tree order.
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
 #include "pm_kernels.h"
 #include "pm_wave.h"
 
@@ -37,22 +42,30 @@ __global__ void k_make_keys(const float* __restrict__ c, uint32_t n, uint32_t* _
   vals[i] = i;
 }
 
-// One thread per internal node of the level [f, e).
-__global__ void k_level_nodes(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
-                              const uint32_t* __restrict__ l0, const uint32_t* __restrict__ l1,
-                              const uint32_t* __restrict__ l2, uint32_t f, uint32_t e,
-                              const uint32_t* __restrict__ node_lo, const uint32_t* __restrict__ node_hi,
-                              const uint32_t* __restrict__ node_lc, KdNode* __restrict__ nodes,
-                              uint2* __restrict__ nrange, uint32_t* __restrict__ node_med,
-                              uint32_t* __restrict__ node_axis) {
-  uint32_t h = f + blockIdx.x * blockDim.x + threadIdx.x;
-  if (h >= e) return;
-  uint32_t lo = node_lo[h], hi = node_hi[h], lc = node_lc[h];
+// A forest of canonical trees over one concatenated point array (one tree
+// for the single-map build; one per photon map of every scene in a batch).
+// Node ids are global; seg[p] holds the node id of point position p at the
+// current level, or KD_LEAF once its subtree is a leaf.
+__device__ __forceinline__ uint32_t kd_child(const KdForest& F, uint32_t h, bool left) {
+  if (F.node_child) return F.node_child[2u * h + (left ? 0u : 1u)];
+  uint32_t c = 2u * h + (left ? 1u : 2u);
+  return c < F.nnodes ? c : KD_LEAF;
+}
+
+// One thread per internal node of the level: level_nodes[first .. first+count)
+// (identity ids when level_nodes is null).
+__global__ void k_level_nodes(KdForest F, const uint32_t* __restrict__ l0, const uint32_t* __restrict__ l1,
+                              const uint32_t* __restrict__ l2, uint32_t first, uint32_t count,
+                              uint32_t* __restrict__ node_med, uint32_t* __restrict__ node_axis) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  uint32_t h = F.level_nodes ? F.level_nodes[first + t] : first + t;
+  uint32_t lo = F.node_lo[h], hi = F.node_hi[h], lc = F.node_lc[h];
   const uint32_t* L[3] = {l0, l1, l2};
   float ext[3];
   for (int a = 0; a < 3; ++a) {
-    float mn = coord_of(x, y, z, L[a][lo], a);
-    float mx = coord_of(x, y, z, L[a][hi - 1], a);
+    float mn = coord_of(F.x, F.y, F.z, L[a][lo], a);
+    float mx = coord_of(F.x, F.y, F.z, L[a][hi - 1], a);
     ext[a] = mx - mn;
   }
   int axis = 0;
@@ -61,29 +74,23 @@ __global__ void k_level_nodes(const float* __restrict__ x, const float* __restri
   if (ext[2] > best) { best = ext[2]; axis = 2; }
   uint32_t med = L[axis][lo + lc];
   KdNode nd;
-  nd.split = coord_of(x, y, z, med, axis);
+  nd.split = coord_of(F.x, F.y, F.z, med, axis);
   nd.meta = (uint32_t)axis | (lc << 2);
-  nodes[h] = nd;
-  nrange[h] = make_uint2(lo, hi);
+  F.nodes[h] = nd;
+  const uint32_t pb = F.node_pbase ? F.node_pbase[h] : 0u;
+  F.nrange[h] = make_uint2(lo - pb, hi - pb);
   node_med[h] = med;
   node_axis[h] = (uint32_t)axis;
 }
 
 struct LevelArgs {
-  const float* x;
-  const float* y;
-  const float* z;
+  KdForest F;
   const uint32_t* lin[3];
   uint32_t* lout[3];
   const uint32_t* seg;
   uint32_t* seg_out;
-  uint32_t f, e;  // internal nodes of this level
-  const uint32_t* node_lo;
-  const uint32_t* node_lc;
-  const uint32_t* node_lb;
   const uint32_t* node_med;
   const uint32_t* node_axis;
-  uint32_t n;
   uint32_t nblocks;
   uint32_t* blockcnt;  // [3][nblocks]
 };
@@ -93,11 +100,11 @@ struct LevelArgs {
 // list is already split at the median, so the flag is "p < median position".
 __device__ __forceinline__ bool part_flag(const LevelArgs& A, int b, uint32_t p, uint32_t h) {
   uint32_t a = A.node_axis[h];
-  if ((uint32_t)b == a) return p < A.node_lo[h] + A.node_lc[h];
+  if ((uint32_t)b == a) return p < A.F.node_lo[h] + A.F.node_lc[h];
   uint32_t i = A.lin[b][p];
   uint32_t m = A.node_med[h];
-  uint32_t ki = float_key(coord_of(A.x, A.y, A.z, i, (int)a));
-  uint32_t km = float_key(coord_of(A.x, A.y, A.z, m, (int)a));
+  uint32_t ki = float_key(coord_of(A.F.x, A.F.y, A.F.z, i, (int)a));
+  uint32_t km = float_key(coord_of(A.F.x, A.F.y, A.F.z, m, (int)a));
   return ki < km || (ki == km && i < m);
 }
 
@@ -107,9 +114,9 @@ __global__ void __launch_bounds__(KB_THREADS) k_level_count(LevelArgs A) {
   size_t base = (size_t)blockIdx.x * KB_TILE + (size_t)threadIdx.x * KB_ITEMS;
   for (int r = 0; r < KB_ITEMS; ++r) {
     size_t p = base + r;
-    if (p >= A.n) break;
+    if (p >= A.F.n) break;
     uint32_t h = A.seg[p];
-    if (h < A.f || h >= A.e) continue;
+    if (h == KD_LEAF) continue;
     for (int b = 0; b < 3; ++b) cnt[b] += part_flag(A, b, (uint32_t)p, h) ? 1u : 0u;
   }
   for (int b = 0; b < 3; ++b) {
@@ -148,6 +155,8 @@ __device__ __forceinline__ uint32_t kb_block_excl(uint32_t v, uint32_t* lw) {
 
 // blockoff: exclusive scan of blockcnt over the concatenated [3][nblocks]
 // array; list b's own offsets are blockoff[b*nb + blk] - blockoff[b*nb].
+// The flags of the level's earlier nodes number exactly their left counts
+// (node_lb), so the left part of position p's node starts at lo + (run - lb).
 __global__ void __launch_bounds__(KB_THREADS) k_level_scatter(LevelArgs A, const uint32_t* __restrict__ blockoff) {
   __shared__ uint32_t lw[KB_THREADS / 64];
   size_t base = (size_t)blockIdx.x * KB_TILE + (size_t)threadIdx.x * KB_ITEMS;
@@ -157,11 +166,11 @@ __global__ void __launch_bounds__(KB_THREADS) k_level_scatter(LevelArgs A, const
   for (int r = 0; r < KB_ITEMS; ++r) {
     size_t p = base + r;
     fl[r] = 0;
-    hs[r] = 0;
-    if (p >= A.n) continue;
+    hs[r] = KD_LEAF;
+    if (p >= A.F.n) continue;
     uint32_t h = A.seg[p];
     hs[r] = h;
-    if (h < A.f || h >= A.e) continue;
+    if (h == KD_LEAF) continue;
     uint8_t m = 0x80;
     for (int b = 0; b < 3; ++b)
       if (part_flag(A, b, (uint32_t)p, h)) {
@@ -177,50 +186,127 @@ __global__ void __launch_bounds__(KB_THREADS) k_level_scatter(LevelArgs A, const
   }
   for (int r = 0; r < KB_ITEMS; ++r) {
     size_t p = base + r;
-    if (p >= A.n) break;
+    if (p >= A.F.n) break;
     uint32_t h = hs[r];
     if (!(fl[r] & 0x80)) {
       for (int b = 0; b < 3; ++b) A.lout[b][p] = A.lin[b][p];
       A.seg_out[p] = h;
       continue;
     }
-    uint32_t lo = A.node_lo[h], lc = A.node_lc[h], lb = A.node_lb[h];
+    uint32_t lo = A.F.node_lo[h], lc = A.F.node_lc[h], lb = A.F.node_lb[h];
     for (int b = 0; b < 3; ++b) {
       bool f = (fl[r] >> b) & 1u;
       uint32_t F = run[b] - lb;  // left flags in [lo, p)
       uint32_t np = f ? lo + F : lo + lc + ((uint32_t)p - lo - F);
       A.lout[b][np] = A.lin[b][p];
-      if (b == 0) A.seg_out[np] = f ? 2u * h + 1u : 2u * h + 2u;
+      if (b == 0) A.seg_out[np] = kd_child(A.F, h, f);
       if (f) run[b] += 1u;
     }
   }
 }
 
-__global__ void k_tree_points(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
-                              const uint32_t* __restrict__ l0, uint32_t n, float4* __restrict__ out) {
+// seg[p] = root of p's tree (or KD_LEAF for a tree without internal nodes).
+__global__ void k_seg_init(KdForest F, uint32_t* __restrict__ seg) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+  if (p >= F.n) return;
+  if (F.point_tree) seg[p] = F.tree_root[F.point_tree[p]];
+  else seg[p] = F.nnodes > 0 ? 0u : KD_LEAF;
+}
+
+// keys[i] = tree of the point vals[i] (the major key of the per-tree order).
+__global__ void k_tree_keys(const uint32_t* __restrict__ point_tree, const uint32_t* __restrict__ vals, uint32_t n,
+                            uint32_t* __restrict__ keys) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) keys[i] = point_tree[vals[i]];
+}
+
+// Leaf order: (x, y, z, bits(tree-local point index)).
+__global__ void k_tree_points(KdForest F, const uint32_t* __restrict__ l0) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= F.n) return;
   uint32_t i = l0[p];
-  out[p] = make_float4(x[i], y[i], z[i], u2f(i));
+  uint32_t pb = F.point_tree ? F.tree_pbase[F.point_tree[p]] : 0u;
+  F.tpts[p] = make_float4(F.x[i], F.y[i], F.z[i], u2f(i - pb));
+}
+
+size_t kdforest_temp_words(uint32_t n, uint32_t nnodes) {
+  size_t nb = ((size_t)n + KB_TILE - 1) / KB_TILE;
+  size_t w = 0;
+  w += 6 * (size_t)n;      // lists (double-buffered)
+  w += 2 * (size_t)n;      // seg, seg2
+  w += 2 * (size_t)n;      // keys, tree keys
+  w += 3 * nb + 3 * nb;    // blockcnt + blockoff
+  w += scan_temp_words(3 * nb);
+  w += radix_temp_words(n);
+  w += 2 * (size_t)nnodes + 8;  // node_med, node_axis
+  return w + 64;
+}
+
+hipError_t kd_build_forest(const KdForest& F, const uint32_t* level_off, uint32_t nlevels, uint32_t* temp,
+                           hipStream_t st) {
+  const uint32_t n = F.n;
+  if (n == 0) return hipSuccess;
+  const size_t nb = ((size_t)n + KB_TILE - 1) / KB_TILE;
+  uint32_t* p = temp;
+  uint32_t* lists[3];
+  uint32_t* lists2[3];
+  for (int a = 0; a < 3; ++a) { lists[a] = p; p += n; }
+  for (int a = 0; a < 3; ++a) { lists2[a] = p; p += n; }
+  uint32_t* seg = p; p += n;
+  uint32_t* seg2 = p; p += n;
+  uint32_t* keys = p; p += n;
+  uint32_t* tkeys = p; p += n;
+  uint32_t* blockcnt = p; p += 3 * nb;
+  uint32_t* blockoff = p; p += 3 * nb;
+  uint32_t* scantemp = p; p += scan_temp_words(3 * nb);
+  uint32_t* sorttemp = p; p += radix_temp_words(n);
+  uint32_t* node_med = p; p += F.nnodes + 1;
+  uint32_t* node_axis = p; p += F.nnodes + 1;
+
+  hipError_t e;
+  const float* C[3] = {F.x, F.y, F.z};
+  unsigned gb = (n + 255) / 256;
+  for (int a = 0; a < 3; ++a) {
+    hipLaunchKernelGGL(k_make_keys, dim3(gb), dim3(256), 0, st, C[a], n, keys, lists[a]);
+    if ((e = radix_sort_pairs(keys, lists[a], n, sorttemp, st)) != hipSuccess) return e;
+    if (F.ntrees > 1) {
+      // stable pass on the tree id: per-tree lists in (key, index) order
+      hipLaunchKernelGGL(k_tree_keys, dim3(gb), dim3(256), 0, st, F.point_tree, lists[a], n, tkeys);
+      if ((e = radix_pass(tkeys, lists[a], n, 0, sorttemp, keys, lists2[a], st)) != hipSuccess) return e;
+      uint32_t* t = lists[a]; lists[a] = lists2[a]; lists2[a] = t;
+    }
+  }
+  hipLaunchKernelGGL(k_seg_init, dim3(gb), dim3(256), 0, st, F, seg);
+  for (uint32_t level = 0; level < nlevels; ++level) {
+    const uint32_t first = level_off[level], count = level_off[level + 1] - level_off[level];
+    if (count == 0) continue;
+    hipLaunchKernelGGL(k_level_nodes, dim3((count + 255) / 256), dim3(256), 0, st, F, lists[0], lists[1], lists[2],
+                       first, count, node_med, node_axis);
+    LevelArgs A;
+    A.F = F;
+    for (int b = 0; b < 3; ++b) { A.lin[b] = lists[b]; A.lout[b] = lists2[b]; }
+    A.seg = seg; A.seg_out = seg2;
+    A.node_med = node_med; A.node_axis = node_axis;
+    A.nblocks = (uint32_t)nb; A.blockcnt = blockcnt;
+    hipLaunchKernelGGL(k_level_count, dim3((unsigned)nb), dim3(KB_THREADS), 0, st, A);
+    if ((e = exclusive_scan(blockcnt, blockoff, 3 * nb, scantemp, nullptr, st)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_level_scatter, dim3((unsigned)nb), dim3(KB_THREADS), 0, st, A, (const uint32_t*)blockoff);
+    for (int b = 0; b < 3; ++b) { uint32_t* t = lists[b]; lists[b] = lists2[b]; lists2[b] = t; }
+    uint32_t* t = seg; seg = seg2; seg2 = t;
+  }
+  hipLaunchKernelGGL(k_tree_points, dim3(gb), dim3(256), 0, st, F, (const uint32_t*)lists[0]);
+  return hipGetLastError();
 }
 
 size_t kdbuild_temp_words(uint32_t n, uint32_t L) {
   uint32_t nl = kd_num_leaves(n, L);
   size_t ni = nl > 0 ? nl - 1 : 0;
-  size_t nb = ((size_t)n + KB_TILE - 1) / KB_TILE;
-  size_t w = 0;
-  w += 6 * (size_t)n;      // lists (double-buffered)
-  w += 2 * (size_t)n;      // seg, seg2
-  w += (size_t)n;          // keys
-  w += 3 * nb + 3 * nb;    // blockcnt + blockoff
-  w += scan_temp_words(3 * nb);
-  w += radix_temp_words(n);
-  w += 6 * ni + 8;         // node_lo, hi, lc, lb, med, axis
-  return w + 64;
+  return kdforest_temp_words(n, (uint32_t)ni) + 4 * (ni + 1);
 }
 
-// Builds m.tpts and m.nodes.  temp must hold kdbuild_temp_words(n, L) words;
-// host_lo/host_hi/host_lc/host_lb are the shape tables of the nl-1 internal nodes.
+// Single tree: m.tpts, m.nodes, m.nrange.  temp must hold
+// kdbuild_temp_words(n, L) words; host_lo/host_hi/host_lc/host_lb are the
+// shape tables of the nl-1 internal nodes (heap order = level order).
 hipError_t kd_build_impl(DevMap& m, uint32_t* temp, const uint32_t* host_lo, const uint32_t* host_hi,
                          const uint32_t* host_lc, const uint32_t* host_lb, hipStream_t st) {
   const uint32_t n = m.n;
@@ -231,65 +317,33 @@ hipError_t kd_build_impl(DevMap& m, uint32_t* temp, const uint32_t* host_lo, con
     return hipSuccess;
   }
   const size_t ni = nl - 1;
-  const size_t nb = ((size_t)n + KB_TILE - 1) / KB_TILE;
-  uint32_t* p = temp;
-  uint32_t* lists[3];
-  uint32_t* lists2[3];
-  for (int a = 0; a < 3; ++a) { lists[a] = p; p += n; }
-  for (int a = 0; a < 3; ++a) { lists2[a] = p; p += n; }
-  uint32_t* seg = p; p += n;
-  uint32_t* seg2 = p; p += n;
-  uint32_t* keys = p; p += n;
-  uint32_t* blockcnt = p; p += 3 * nb;
-  uint32_t* blockoff = p; p += 3 * nb;
-  uint32_t* scantemp = p; p += scan_temp_words(3 * nb);
-  uint32_t* sorttemp = p; p += radix_temp_words(n);
-  uint32_t* node_lo = p; p += ni + 1;
-  uint32_t* node_hi = p; p += ni + 1;
-  uint32_t* node_lc = p; p += ni + 1;
-  uint32_t* node_lb = p; p += ni + 1;
-  uint32_t* node_med = p; p += ni + 1;
-  uint32_t* node_axis = p; p += ni + 1;
-
+  uint32_t* tab = temp + kdforest_temp_words(n, (uint32_t)ni);
+  uint32_t* node_lo = tab;
+  uint32_t* node_hi = node_lo + ni + 1;
+  uint32_t* node_lc = node_hi + ni + 1;
+  uint32_t* node_lb = node_lc + ni + 1;
   hipError_t e;
-  const float* C[3] = {m.x, m.y, m.z};
-  unsigned gb = (n + 255) / 256;
-  for (int a = 0; a < 3; ++a) {
-    hipLaunchKernelGGL(k_make_keys, dim3(gb), dim3(256), 0, st, C[a], n, keys, lists[a]);
-    if ((e = radix_sort_pairs(keys, lists[a], n, sorttemp, st)) != hipSuccess) return e;
-  }
   if (ni > 0) {
     if ((e = hipMemcpyAsync(node_lo, host_lo, 4 * ni, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(node_hi, host_hi, 4 * ni, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(node_lc, host_lc, 4 * ni, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(node_lb, host_lb, 4 * ni, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
   }
-  if ((e = hipMemsetAsync(seg, 0, 4 * (size_t)n, st)) != hipSuccess) return e;
-  for (uint32_t level = 0;; ++level) {
-    uint64_t f = (1ull << level) - 1ull;
-    if (f >= ni) break;
-    uint64_t eend = (2ull << level) - 1ull;
-    if (eend > ni) eend = ni;
-    unsigned nbn = (unsigned)((eend - f + 255) / 256);
-    hipLaunchKernelGGL(k_level_nodes, dim3(nbn), dim3(256), 0, st, m.x, m.y, m.z, lists[0], lists[1], lists[2],
-                       (uint32_t)f, (uint32_t)eend, node_lo, node_hi, node_lc, m.nodes, m.nrange, node_med,
-                       node_axis);
-    LevelArgs A;
-    A.x = m.x; A.y = m.y; A.z = m.z;
-    for (int b = 0; b < 3; ++b) { A.lin[b] = lists[b]; A.lout[b] = lists2[b]; }
-    A.seg = seg; A.seg_out = seg2;
-    A.f = (uint32_t)f; A.e = (uint32_t)eend;
-    A.node_lo = node_lo; A.node_lc = node_lc; A.node_lb = node_lb; A.node_med = node_med; A.node_axis = node_axis;
-    A.n = n; A.nblocks = (uint32_t)nb; A.blockcnt = blockcnt;
-    hipLaunchKernelGGL(k_level_count, dim3((unsigned)nb), dim3(KB_THREADS), 0, st, A);
-    if ((e = exclusive_scan(blockcnt, blockoff, 3 * nb, scantemp, nullptr, st)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_level_scatter, dim3((unsigned)nb), dim3(KB_THREADS), 0, st, A, (const uint32_t*)blockoff);
-    for (int b = 0; b < 3; ++b) { uint32_t* t = lists[b]; lists[b] = lists2[b]; lists2[b] = t; }
-    uint32_t* t = seg; seg = seg2; seg2 = t;
-  }
-  hipLaunchKernelGGL(k_tree_points, dim3(gb), dim3(256), 0, st, m.x, m.y, m.z, lists[0], n, m.tpts);
+  KdForest F;
+  memset(&F, 0, sizeof(F));
+  F.n = n;
+  F.x = m.x; F.y = m.y; F.z = m.z;
+  F.tpts = m.tpts;
+  F.nodes = m.nodes;
+  F.nrange = m.nrange;
+  F.ntrees = 1;
+  F.nnodes = (uint32_t)ni;
+  F.node_lo = node_lo; F.node_hi = node_hi; F.node_lc = node_lc; F.node_lb = node_lb;
+  std::vector<uint32_t> off(1, 0u);
+  for (uint64_t f = 0, w = 1; f < ni; f = 2 * f + 1, w *= 2) off.push_back((uint32_t)std::min<uint64_t>(f + w, ni));
+  if ((e = kd_build_forest(F, off.data(), (uint32_t)off.size() - 1u, temp, st)) != hipSuccess) return e;
   m.built = true;
-  return hipGetLastError();
+  return hipSuccess;
 }
 
 }  // namespace pm

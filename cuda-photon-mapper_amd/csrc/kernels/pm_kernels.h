@@ -53,6 +53,10 @@ hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t*
                           hipStream_t st);
 size_t radix_temp_words(size_t n);
 hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, size_t n, uint32_t* temp, hipStream_t st);
+// One stable pass on the 8-bit digit at `shift`: (kin, vin) -> (kout, vout).
+hipError_t radix_pass(const uint32_t* kin, const uint32_t* vin, size_t n, int shift, uint32_t* temp, uint32_t* kout,
+                      uint32_t* vout, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t st);
 hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t st);
 
 // trace (pm_trace.hip)
@@ -66,6 +70,35 @@ hipError_t launch_tally_final(const double* partial, uint32_t nblocks, double* o
 uint32_t trace_write_blocks(uint32_t npid);
 
 // kd-tree (pm_kdbuild.hip)
+static constexpr uint32_t KD_LEAF = 0xffffffffu;
+// A forest of canonical trees over one concatenated point array.  Tables are
+// device arrays indexed by global node id; null optional tables mean a single
+// tree in heap order (children 2h+1, 2h+2 below nnodes, point base 0).
+struct KdForest {
+  uint32_t n;                   // points of all trees
+  const float* x;
+  const float* y;
+  const float* z;
+  float4* tpts;                 // [n] (x, y, z, bits(tree-local point index)) in tree order
+  KdNode* nodes;                // [nnodes]
+  uint2* nrange;                // [nnodes] tree-local point range [lo, hi)
+  uint32_t ntrees;              // 1 .. 256
+  uint32_t nnodes;              // internal nodes of all trees
+  const uint32_t* point_tree;   // [n] tree of each point (null: one tree)
+  const uint32_t* tree_pbase;   // [ntrees] first point of each tree
+  const uint32_t* tree_root;    // [ntrees] root node id, KD_LEAF without internal nodes
+  const uint32_t* node_lo;      // [nnodes] global point range [lo, hi)
+  const uint32_t* node_hi;
+  const uint32_t* node_lc;      // left point count
+  const uint32_t* node_lb;      // left counts of the same level's earlier nodes (all trees)
+  const uint32_t* node_child;   // [2 nnodes] child ids or KD_LEAF (optional)
+  const uint32_t* node_pbase;   // [nnodes] first point of the node's tree (optional)
+  const uint32_t* level_nodes;  // node ids in level order (optional: identity)
+};
+size_t kdforest_temp_words(uint32_t n, uint32_t nnodes);
+// level_off: host array of nlevels + 1 offsets into level_nodes.
+hipError_t kd_build_forest(const KdForest& F, const uint32_t* level_off, uint32_t nlevels, uint32_t* temp,
+                           hipStream_t st);
 size_t kdbuild_temp_words(uint32_t n, uint32_t L);
 // Shape tables (host arrays of the nl-1 internal nodes): point range [lo, hi),
 // left count lc, and lb = left counts of the same level's earlier nodes.
@@ -90,6 +123,50 @@ struct RenderLaunch {
 };
 hipError_t launch_render(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane(const RenderLaunch& r, hipStream_t st);
+
+// batched scenes (pm_trace.hip, pm_gather_lane.hip)
+// Emission: scene s owns the trace blocks [block0, block0 + nblocks), whose
+// lanes trace its photon ids 0 .. n_emit-1 (the one-scene block partition).
+struct BatchTraceScene {
+  Scene sc;
+  TraceParams tp;
+  uint32_t n_emit;
+  uint32_t block0, nblocks;
+  uint32_t pad;
+};
+hipError_t launch_trace_count_batch(const BatchTraceScene* scenes, const uint32_t* block_scene, uint32_t nblocks,
+                                    uint32_t* cnt_s, uint32_t* cnt_v, hipStream_t st);
+hipError_t launch_trace_write_batch(const BatchTraceScene* scenes, const uint32_t* block_scene, uint32_t nblocks,
+                                    const uint32_t* off_s, const uint32_t* off_v, DevMap ms, DevMap mv,
+                                    double* tally_partial, hipStream_t st);
+// out9[9 s + c]: scene s's tally, summed over its blocks in the one-scene order.
+hipError_t launch_tally_final_batch(const double* partial, const BatchTraceScene* scenes, uint32_t nscenes,
+                                    double* out9, hipStream_t st);
+uint32_t trace_threads();
+struct BatchRenderScene {
+  Scene sc;
+  RenderParams rp;
+  GatherMap gs, gv;
+  float exposure;
+  int w, h;
+  int pad;
+  uint64_t img_off;  // first pixel of the scene's image in the batch image arena
+};
+struct BatchRenderLaunch {
+  const BatchRenderScene* scenes;
+  const uint2* items;              // work items (scene, 8x8 tile), most expensive first
+  uint32_t nitems;
+  uint32_t* head;                  // work-queue counter, zero at launch
+  float* rgb;                      // image arena, 3 floats per pixel
+  unsigned long long* counters;    // [surface queries, volume queries]
+  int heap_slots;                  // LDS heap keys per lane (>= every scene's k)
+};
+size_t render_batch_lds_bytes(int block_threads, int heap_slots);
+int render_batch_occupancy(int fp64, int cone, int block_threads, int heap_slots);  // workgroups per CU
+hipError_t launch_tonemap(const float* rgb, size_t npix, float exposure, uchar4* out, hipStream_t st);
+hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t n, uint32_t* dst, hipStream_t st);
+hipError_t launch_render_batch(const BatchRenderLaunch& r, int fp64, int cone, int block_threads, int grid,
+                               hipStream_t st);
 
 // learned-cache render (pm_cache_render.hip)
 enum : int { CACHE_GRID = 16 };  // error-tracker cells per axis (x 2 kinds)
@@ -121,6 +198,7 @@ hipError_t launch_cache_select(const float* in, const uint32_t* list_s, uint32_t
 hipError_t launch_kind_flags(const float* in, uint64_t n, int want_surface, uint32_t* flags, hipStream_t st);
 hipError_t launch_err_accum(const float* in, const float* row_err, uint32_t rows, unsigned long long* esum,
                             unsigned int* ecnt, hipStream_t st);
+hipError_t launch_sum_rows(const float* v, uint32_t n, double* acc, hipStream_t st);
 hipError_t launch_cache_flags(const float* in, uint64_t n, const unsigned long long* esum, const unsigned int* ecnt,
                               float threshold, int cache_surface, uint32_t* flags, hipStream_t st);
 hipError_t launch_cache_compact(const uint32_t* flags, const uint32_t* off, uint64_t n, uint32_t* list,

@@ -202,6 +202,50 @@ hipError_t radix_sort_pairs(uint32_t* keys, uint32_t* vals, size_t n, uint32_t* 
   return hipGetLastError();
 }
 
+hipError_t radix_pass(const uint32_t* kin, const uint32_t* vin, size_t n, int shift, uint32_t* temp, uint32_t* kout,
+                      uint32_t* vout, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  size_t nb = (n + RS_TILE - 1) / RS_TILE;
+  uint32_t* hist = temp;
+  uint32_t* offs = hist + 256 * nb;
+  uint32_t* stmp = offs + 256 * nb;
+  hipLaunchKernelGGL(k_radix_hist, dim3((unsigned)nb), dim3(RS_THREADS), 0, st, kin, n, shift, hist, (uint32_t)nb);
+  hipError_t e = exclusive_scan(hist, offs, 256 * nb, stmp, nullptr, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_radix_scatter, dim3((unsigned)nb), dim3(RS_THREADS), 0, st, kin, vin, n, shift, offs,
+                     (uint32_t)nb, kout, vout);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- fill
+// A kernel rather than hipMemsetAsync, so that the stream work stays
+// graph-capturable as kernel nodes only.
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint32_t value, size_t n) {
+  size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * 256;
+  for (; i < n; i += stride) p[i] = value;
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  size_t nb = (n + 255) / 256;
+  if (nb > 8192) nb = 8192;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)nb), dim3(256), 0, st, p, value, n);
+  return hipGetLastError();
+}
+
+__global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx, uint32_t n,
+                             uint32_t* __restrict__ dst) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t n, uint32_t* dst, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_u32, dim3((n + 255) / 256), dim3(256), 0, st, src, idx, n, dst);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- copy
 __global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
   size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;

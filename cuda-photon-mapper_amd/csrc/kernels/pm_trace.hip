@@ -95,6 +95,57 @@ __global__ void __launch_bounds__(TRACE_THREADS) k_trace_write(Scene sc, TracePa
   }
 }
 
+// Batched emission: block b traces photons of scene block_scene[b], with the
+// same per-photon work and the same per-block tally reduction as the
+// one-scene kernels (record offsets come from one scan over all scenes).
+__global__ void __launch_bounds__(TRACE_THREADS) k_trace_count_batch(const BatchTraceScene* __restrict__ scenes,
+                                                                      const uint32_t* __restrict__ block_scene,
+                                                                      uint32_t* __restrict__ cnt_s,
+                                                                      uint32_t* __restrict__ cnt_v) {
+  const BatchTraceScene& B = scenes[block_scene[blockIdx.x]];
+  const uint32_t local = (blockIdx.x - B.block0) * TRACE_THREADS + threadIdx.x;
+  const size_t g = (size_t)blockIdx.x * TRACE_THREADS + threadIdx.x;
+  CountSink sk{0u, 0u};
+  if (local < B.n_emit) {
+    const Scene sc = B.sc;
+    const TraceParams tp = B.tp;
+    Tally9 t;
+    tally_zero(t);
+    trace_photon(sc, tp, local, sk, t);
+  }
+  cnt_s[g] = sk.s;
+  cnt_v[g] = sk.v;
+}
+
+__global__ void __launch_bounds__(TRACE_THREADS) k_trace_write_batch(const BatchTraceScene* __restrict__ scenes,
+                                                                      const uint32_t* __restrict__ block_scene,
+                                                                      const uint32_t* __restrict__ off_s,
+                                                                      const uint32_t* __restrict__ off_v, DevMap ms,
+                                                                      DevMap mv, double* __restrict__ tally_partial) {
+  __shared__ double red[TRACE_THREADS];
+  const BatchTraceScene& B = scenes[block_scene[blockIdx.x]];
+  const uint32_t local = (blockIdx.x - B.block0) * TRACE_THREADS + threadIdx.x;
+  const size_t g = (size_t)blockIdx.x * TRACE_THREADS + threadIdx.x;
+  Tally9 t;
+  tally_zero(t);
+  if (local < B.n_emit) {
+    const Scene sc = B.sc;
+    const TraceParams tp = B.tp;
+    WriteSink sk{&ms, &mv, off_s[g], off_v[g]};
+    trace_photon(sc, tp, local, sk, t);
+  }
+  for (int c = 0; c < 9; ++c) {
+    red[threadIdx.x] = t.v[c];
+    __syncthreads();
+    for (int s = TRACE_THREADS / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) tally_partial[(size_t)blockIdx.x * 9 + c] = red[0];
+    __syncthreads();
+  }
+}
+
 // Sum of the per-block tallies, fixed order: thread t owns blocks t, t+256, ...
 __global__ void __launch_bounds__(256) k_tally_final(const double* __restrict__ partial, uint32_t nblocks,
                                                       double* __restrict__ out9) {
@@ -109,6 +160,26 @@ __global__ void __launch_bounds__(256) k_tally_final(const double* __restrict__ 
       __syncthreads();
     }
     if (threadIdx.x == 0) out9[c] = red[0];
+    __syncthreads();
+  }
+}
+
+// One workgroup per scene: k_tally_final over the scene's own blocks.
+__global__ void __launch_bounds__(256) k_tally_final_batch(const double* __restrict__ partial,
+                                                            const BatchTraceScene* __restrict__ scenes,
+                                                            double* __restrict__ out9) {
+  __shared__ double red[256];
+  const uint32_t b0 = scenes[blockIdx.x].block0, nb = scenes[blockIdx.x].nblocks;
+  for (int c = 0; c < 9; ++c) {
+    double s = 0.0;
+    for (uint32_t b = threadIdx.x; b < nb; b += 256) s += partial[(size_t)(b0 + b) * 9 + c];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out9[(size_t)blockIdx.x * 9 + c] = red[0];
     __syncthreads();
   }
 }
@@ -151,6 +222,32 @@ hipError_t launch_find_cap(const uint32_t* off, const uint32_t* cnt, uint32_t n,
                            hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_find_cap, dim3((n + 255) / 256), dim3(256), 0, st, off, cnt, n, cap, out);
+  return hipGetLastError();
+}
+
+uint32_t trace_threads() { return TRACE_THREADS; }
+
+hipError_t launch_trace_count_batch(const BatchTraceScene* scenes, const uint32_t* block_scene, uint32_t nblocks,
+                                    uint32_t* cnt_s, uint32_t* cnt_v, hipStream_t st) {
+  if (nblocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_trace_count_batch, dim3(nblocks), dim3(TRACE_THREADS), 0, st, scenes, block_scene, cnt_s,
+                     cnt_v);
+  return hipGetLastError();
+}
+
+hipError_t launch_trace_write_batch(const BatchTraceScene* scenes, const uint32_t* block_scene, uint32_t nblocks,
+                                    const uint32_t* off_s, const uint32_t* off_v, DevMap ms, DevMap mv,
+                                    double* tally_partial, hipStream_t st) {
+  if (nblocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_trace_write_batch, dim3(nblocks), dim3(TRACE_THREADS), 0, st, scenes, block_scene, off_s,
+                     off_v, ms, mv, tally_partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_tally_final_batch(const double* partial, const BatchTraceScene* scenes, uint32_t nscenes,
+                                    double* out9, hipStream_t st) {
+  if (nscenes == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tally_final_batch, dim3(nscenes), dim3(256), 0, st, partial, scenes, out9);
   return hipGetLastError();
 }
 

@@ -122,6 +122,32 @@ class PmCacheStats(ctypes.Structure):
     ]
 
 
+class PmBatchTuning(ctypes.Structure):
+    _fields_ = [
+        ("leaf_size", ctypes.c_int),
+        ("block_threads", ctypes.c_int),
+        ("heap_slots", ctypes.c_int),
+    ]
+
+
+class PmBatchStats(ctypes.Structure):
+    _fields_ = [
+        ("n_scenes", ctypes.c_uint32),
+        ("work_items", ctypes.c_uint32),
+        ("photons", ctypes.c_uint64),
+        ("surface_records", ctypes.c_uint64),
+        ("volume_records", ctypes.c_uint64),
+        ("pixels", ctypes.c_uint64),
+        ("surface_queries", ctypes.c_uint64),
+        ("volume_queries", ctypes.c_uint64),
+        ("ms_frame", ctypes.c_double),
+        ("ms_setup", ctypes.c_double),
+        ("tuning", PmBatchTuning),
+        ("graph_nodes", ctypes.c_uint32),
+        ("pad0", ctypes.c_uint32),
+    ]
+
+
 class PmError(RuntimeError):
     def __init__(self, status, what):
         super().__init__("%s failed with status %d" % (what, status))
@@ -175,6 +201,20 @@ _SIGS = {
     "pm_cache_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PmCacheStats)]),
     "pm_render_cache_crop": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_void_p, ctypes.c_void_p]),
+    "pm_batch_scene_config": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(PmConfig)]),
+    "pm_batch_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "pm_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "pm_batch_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PmBatchTuning)]),
+    "pm_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(PmBatchTuning),
+                                         ctypes.POINTER(ctypes.c_double)]),
+    "pm_batch_submit": (ctypes.c_int, [ctypes.c_void_p]),
+    "pm_batch_image": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+    "pm_batch_scene_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(PmStats)]),
+    "pm_batch_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PmBatchStats)]),
+    "pm_batch_get_records": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
+    "pm_batch_get_tree": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
 }
 
 
@@ -359,7 +399,7 @@ class PhotonMapper:
               "pm_cache_train")
         return loss.value
 
-    def cache_configure(self, threshold=0.02, train_samples=1 << 18, batch=1 << 14, lr=1e-3, march_stride=1,
+    def cache_configure(self, threshold=0.05, train_samples=1 << 18, batch=1 << 14, lr=1e-3, march_stride=1,
                         cache_surface=0):
         check(load().pm_cache_configure(self.h, threshold, train_samples, batch, lr, march_stride, cache_surface),
               "pm_cache_configure")
@@ -399,3 +439,84 @@ class PhotonMapper:
         check(load().pm_knn(self.h, which, qb.ptr, nq, k, rcap, ib.ptr, db.ptr, cb.ptr), "pm_knn")
         return (ib.download(np.uint32, nq * k).reshape(nq, k), db.download(np.float32, nq * k).reshape(nq, k),
                 cb.download(np.uint32, nq))
+
+
+def batch_scene(seed, index):
+    """Configuration of scene `index` of the batched workload drawn from `seed`."""
+    cfg = PmConfig()
+    check(load().pm_batch_scene_config(seed, index, ctypes.byref(cfg)), "pm_batch_scene_config")
+    return cfg
+
+
+class Batch:
+    """Many scenes rendered per frame by one hipGraph launch (pm_batch_*)."""
+
+    def __init__(self, cfgs):
+        self.cfgs = list(cfgs)
+        arr = (PmConfig * len(self.cfgs))(*self.cfgs)
+        h = ctypes.c_void_p()
+        check(load().pm_batch_create(arr, len(self.cfgs), ctypes.byref(h)), "pm_batch_create")
+        self.h = h.value
+
+    def close(self):
+        if self.h:
+            load().pm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tuning(self, leaf_size, block_threads, heap_slots):
+        t = PmBatchTuning(leaf_size, block_threads, heap_slots)
+        check(load().pm_batch_set_tuning(self.h, ctypes.byref(t)), "pm_batch_set_tuning")
+
+    def autotune(self, sample_stride=16):
+        t = PmBatchTuning()
+        ms = ctypes.c_double()
+        check(load().pm_batch_autotune(self.h, sample_stride, ctypes.byref(t), ctypes.byref(ms)), "pm_batch_autotune")
+        return (t.leaf_size, t.block_threads, t.heap_slots), ms.value
+
+    def submit(self):
+        check(load().pm_batch_submit(self.h), "pm_batch_submit")
+
+    def stats(self):
+        s = PmBatchStats()
+        check(load().pm_batch_stats_get(self.h, ctypes.byref(s)), "pm_batch_stats_get")
+        return s
+
+    def scene_stats(self, i):
+        s = PmStats()
+        check(load().pm_batch_scene_stats(self.h, i, ctypes.byref(s)), "pm_batch_scene_stats")
+        return s
+
+    def image(self, i):
+        c = self.cfgs[i]
+        n = c.width * c.height
+        rgb = DeviceBuffer(n * 12)
+        check(load().pm_batch_image(self.h, i, rgb.ptr, None), "pm_batch_image")
+        return rgb.download(np.float32, n * 3).reshape(c.height, c.width, 3)
+
+    def records(self, i, which):
+        s = self.scene_stats(i)
+        n = s.n_surface if which == PM_MAP_SURFACE else s.n_volume
+        pos = np.empty((n, 3), np.float32)
+        pw = np.empty((n, 3), np.float32)
+        d = np.empty((n, 3), np.float32)
+        check(load().pm_batch_get_records(self.h, i, which, pos.ctypes.data, pw.ctypes.data, d.ctypes.data, n),
+              "pm_batch_get_records")
+        return pos, pw, d
+
+    def tree(self, i, which, leaf_size):
+        s = self.scene_stats(i)
+        n = s.n_surface if which == PM_MAP_SURFACE else s.n_volume
+        nl = (n + leaf_size - 1) // leaf_size
+        ni = max(nl - 1, 0)
+        perm = np.empty(n, np.uint32)
+        split = np.empty(max(ni, 1), np.float32)
+        axis = np.empty(max(ni, 1), np.uint32)
+        check(load().pm_batch_get_tree(self.h, i, which, perm.ctypes.data, split.ctypes.data, axis.ctypes.data, n, ni),
+              "pm_batch_get_tree")
+        return perm, split[:ni], axis[:ni]

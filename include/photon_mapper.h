@@ -190,13 +190,14 @@ int pm_cache_train(pm_context* ctx, const float* inputs_dev, const float* target
 /* Radiance predictions (n x 3 FP32) for n device input rows. */
 int pm_cache_predict(pm_context* ctx, const float* inputs_dev, uint32_t n, float* rgb_dev);
 
-/* Cache render settings: fallback threshold on the mean squared log-radiance
- * training error of a 16^3 cell (per query kind), training samples per frame,
+/* Cache render settings: fallback threshold on the mean (over a 16^3 cell's
+ * training rows of this frame, per query kind, per colour channel) squared
+ * error of the predicted log(1 + radiance), training samples per frame,
  * Adam batch and learning rate, march points per cached volume query (the
  * first point's prediction stands for up to `march_stride` consecutive points,
  * weighted by their summed path weights; 1 = every point), and whether
  * surface queries use the cache (0: always gathered exactly).
- * Defaults 0.02, 2^18, 2^14, 1e-3, 1, 0. */
+ * Defaults 0.05, 2^18, 2^14, 1e-3, 1, 0. */
 int pm_cache_configure(pm_context* ctx, float threshold, uint32_t train_samples, uint32_t batch, float lr,
                        int march_stride, int cache_surface);
 
@@ -215,6 +216,66 @@ typedef struct pm_cache_stats {
  * the threshold, accumulate.  Same outputs as pm_render_crop. */
 int pm_render_cache_crop(pm_context* ctx, int x0, int y0, int w, int h, float* rgb_dev, pm_uchar4* out_dev);
 int pm_cache_stats_get(pm_context* ctx, pm_cache_stats* out);
+
+/* ------------------------------------------------------------------ batched scenes
+ * Many independent scenes of uneven photon counts and resolutions rendered
+ * per frame by one hipGraph launch: batched emission over all scenes (one
+ * count pass, one scan, one write pass), one kd-tree forest build over every
+ * photon map, and one persistent render kernel fed by a device work queue of
+ * (scene, 8x8 tile) items.  Every scene's records, trees and pixels are
+ * bit-identical to its own pm_create/pm_emit/pm_build/pm_render run with the
+ * same leaf size.  The workgroup size and heap slots never change a result;
+ * the leaf size changes only the order in which the (identical) neighbours'
+ * contributions are summed.  No reference counterpart (the reference renders
+ * one scene). */
+typedef struct pm_batch pm_batch;
+
+typedef struct pm_batch_tuning {
+  int leaf_size;      /* kd-tree leaf size L (1..64) of every map */
+  int block_threads;  /* render workgroup size: 64, 128, 192 or 256 */
+  int heap_slots;     /* LDS heap keys per lane: 32 or 64 (>= every scene's k) */
+} pm_batch_tuning;
+
+typedef struct pm_batch_stats {
+  uint32_t n_scenes;
+  uint32_t work_items;                       /* 8x8 tiles of all scenes */
+  uint64_t photons, surface_records, volume_records, pixels;
+  uint64_t surface_queries, volume_queries;  /* radiance queries of the last frame */
+  double ms_frame;                           /* last pm_batch_submit: graph launch to completion */
+  double ms_setup;                           /* sizing pass, tables and graph capture */
+  pm_batch_tuning tuning;
+  uint32_t graph_nodes;
+  uint32_t pad0;
+} pm_batch_stats;
+
+/* Scene `index` of the batched workload generated from `seed`: the unit room
+ * with 2-256 random spheres (60% Lambertian / 25% mirror / 15% dielectric,
+ * IOR 1.5), 10^4..2*10^6 photons (log-uniform), image side 128..1024,
+ * homogeneous medium sigma_s U[0, 0.3]; k = 32, r <= 0.05, 1 spp. */
+int pm_batch_scene_config(uint32_t seed, int index, pm_config* out);
+/* n_scenes <= 128 configurations with n_emit > 0 (photon-count mode) and the
+ * same storage, estimator, transmittance precision and media switch; k <= 64.
+ * Runs the sizing pass and captures the frame graph. */
+int pm_batch_create(const pm_config* cfgs, int n_scenes, pm_batch** out);
+int pm_batch_destroy(pm_batch* b);
+int pm_batch_set_tuning(pm_batch* b, const pm_batch_tuning* t);
+/* Times the frame (emission, build, a 1-in-`sample_stride` subset of the
+ * render work items) for every candidate of the grid leaf {4, 8, 16, 32} x
+ * block {64, 128, 256} x heap slots {32 (when every k <= 32), 64}, keeps the
+ * fastest and returns it (and its time). */
+int pm_batch_autotune(pm_batch* b, int sample_stride, pm_batch_tuning* best_out, double* best_ms_out);
+/* One frame of every scene (one graph launch); blocking. */
+int pm_batch_submit(pm_batch* b);
+/* Scene `scene`'s image of the last frame: rgb_dev w*h*3 FP32 and/or out_dev
+ * w*h uchar4 (either may be NULL). */
+int pm_batch_image(pm_batch* b, int scene, float* rgb_dev, pm_uchar4* out_dev);
+/* Per-scene record counts and flux tally of the last frame (timing fields 0). */
+int pm_batch_scene_stats(pm_batch* b, int scene, pm_stats* out);
+int pm_batch_stats_get(pm_batch* b, pm_batch_stats* out);
+int pm_batch_get_records(pm_batch* b, int scene, int map, float* pos_host, float* power_host, float* dir_host,
+                         uint32_t n);
+int pm_batch_get_tree(pm_batch* b, int scene, int map, uint32_t* perm_host, float* split_host, uint32_t* axis_host,
+                      uint32_t n, uint32_t n_internal);
 
 /* Inspection (copies to host memory; for tests and tools). */
 int pm_get_records(pm_context* ctx, int map, float* pos_host, float* power_host, float* dir_host, uint32_t n);
