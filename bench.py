@@ -16,7 +16,9 @@ stratified sample of the same frame.  `cache` reports the learned-cache form of
 the same frame (queries recorded, MLP trained online on exact kNN targets,
 MLP inference with exact kNN fallback): ms/frame, speed-up over the plain
 frame, its error against the plain frame on the same eye samples, and the
-MFMA rate of the MLP inference.
+MFMA rate of the MLP inference.  `batch` reports the batched workload (64
+random scenes per frame through one hipGraph launch) against the same scenes
+rendered one at a time.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -143,6 +145,56 @@ def run_cache(g, cfg, steps, train_frames, stride, threshold, plain_img):
     }
 
 
+def run_batch(steps, tune_stride, n_scenes=64, seed=1):
+    """The batched workload: n_scenes random scenes (generator seed 1), one
+    hipGraph launch per frame, tuning picked by the library's autotuner; then
+    the same scenes one at a time through pm_create/emit/build/render."""
+    lib = pmamd.load()
+    cfgs = [pmamd.batch_scene(seed, i) for i in range(n_scenes)]
+    t = time.perf_counter()
+    b = pmamd.Batch(cfgs)
+    t_setup = time.perf_counter() - t
+    t = time.perf_counter()
+    best, tune_ms = b.autotune(tune_stride)
+    t_tune = time.perf_counter() - t
+    log("batch: %d scenes, setup %.1f s, autotune %.1f s -> leaf %d, block %d, heap slots %d"
+        % (n_scenes, t_setup, t_tune, best[0], best[1], best[2]))
+    b.submit()
+    per = []
+    for i in range(steps):
+        b.submit()
+        per.append(b.stats().ms_frame)
+        log("batch step %d: %.1f ms" % (i, per[-1]))
+    st = b.stats()
+    q = st.surface_queries + st.volume_queries
+    ms = float(np.median(per))
+    b.close()
+    alone = 0.0
+    for c in cfgs:
+        c.leaf_size = best[0]
+        g = pmamd.PhotonMapper(c)
+        buf = pmamd.DeviceBuffer(c.width * c.height * 12)
+        g.emit()
+        g.build()
+        pmamd.check(lib.pm_render(g.h, buf.ptr, None), "pm_render")
+        s = g.stats()
+        alone += s.ms_emit + s.ms_build + s.ms_render
+        g.close()
+    log("batch: %.1f ms per frame; the scenes one at a time: %.1f ms" % (ms, alone))
+    return {
+        "workload": "%d random scenes (2-256 spheres: 60%% Lambertian / 25%% mirror / 15%% dielectric IOR 1.5; "
+                    "photons log-uniform 1e4-2e6; images 128^2-1024^2; homogeneous sigma_s U[0,0.3]; k=32, r<=0.05, "
+                    "1 spp)" % n_scenes,
+        "photons": int(st.photons), "pixels": int(st.pixels), "surface_records": int(st.surface_records),
+        "volume_records": int(st.volume_records), "radiance_queries": int(q),
+        "ms_per_frame": round(ms, 3), "radiance_queries_per_s": round(q / (ms * 1e-3), 1),
+        "tuning": {"leaf_size": best[0], "block_threads": best[1], "heap_slots": best[2],
+                   "autotune_sample_stride": tune_stride, "autotune_s": round(t_tune, 2)},
+        "graph_nodes": int(st.graph_nodes), "work_items": int(st.work_items),
+        "scenes_one_at_a_time_ms": round(alone, 3), "speedup_vs_one_at_a_time": round(alone / ms, 3),
+    }
+
+
 def run_cpu(cfg, budget_s):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
@@ -202,7 +254,9 @@ def main():
     ap.add_argument("--cache-steps", type=int, default=3, help="timed cache frames (0: skip the cache leg)")
     ap.add_argument("--cache-train-frames", type=int, default=12)
     ap.add_argument("--cache-stride", type=int, default=4)
-    ap.add_argument("--cache-threshold", type=float, default=0.05)
+    ap.add_argument("--cache-threshold", type=float, default=0.12)
+    ap.add_argument("--batch-steps", type=int, default=3, help="timed batch frames (0: skip the batch leg)")
+    ap.add_argument("--batch-tune-stride", type=int, default=4)
     ap.add_argument("--copy-calibrate", type=int, default=0,
                     help="run only the library's copy kernel over this many bytes (memory-counter calibration)")
     args = ap.parse_args()
@@ -264,6 +318,8 @@ def main():
                        img.reshape(-1, 3))
         ca["speedup_vs_plain_frame"] = round(out["ms_per_step"] / ca["ms_per_frame"], 3)
         out["cache"] = ca
+    if args.batch_steps > 0:
+        out["batch"] = run_batch(args.batch_steps, args.batch_tune_stride)
     if not args.no_cpu:
         cb = run_cpu(cfg, args.cpu_seconds)
         out["cpu_baseline"] = {

@@ -53,7 +53,6 @@ struct pm_batch {
   std::vector<uint64_t> img_off;
   uint64_t npix = 0;
   std::vector<uint2> items_h;
-  uint32_t nitems_tune = 0;
   DevBuf rsc_d, items_d, items_tune_d, head, rgb, counters;
   int fp64 = 0, cone = 0, kmax = 1, cus = 1, grid = 1;
   pm_batch_tuning tune;
@@ -338,9 +337,11 @@ int set_grid(pm_batch* b) {
   return PM_OK;
 }
 
-// Captures one frame over `nitems` work items into b->graph / b->exec.
-int capture(pm_batch* b, const uint2* items, uint32_t nitems) {
-  release_graph(b);
+// Enqueues one frame over `nitems` work items on b->stream.  ev (optional,
+// outside graph capture only) receives events after the emission and after
+// the build, so a caller can time the phases.
+hipError_t enqueue_frame(pm_batch* b, const uint2* items, uint32_t nitems, hipEvent_t ev_emitted,
+                         hipEvent_t ev_built) {
   hipStream_t st = b->stream;
   const size_t np = (size_t)b->tblocks * trace_threads();
   uint32_t* small = b->small.as<uint32_t>();
@@ -372,27 +373,40 @@ int capture(pm_batch* b, const uint2* items, uint32_t nitems) {
   R.rgb = b->rgb.as<float>();
   R.counters = b->counters.as<unsigned long long>();
   R.heap_slots = b->tune.heap_slots;
-  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  hipError_t e = hipSuccess;
-  auto step = [&](hipError_t r) {
-    if (e == hipSuccess) e = r;
-  };
-  step(launch_trace_count_batch(b->tsc_d.as<BatchTraceScene>(), b->block_scene_d.as<uint32_t>(), b->tblocks,
-                                b->cnt_s.as<uint32_t>(), b->cnt_v.as<uint32_t>(), st));
-  step(exclusive_scan(b->cnt_s.as<uint32_t>(), b->off_s.as<uint32_t>(), np, b->scan_tmp.as<uint32_t>(), small, st));
-  step(exclusive_scan(b->cnt_v.as<uint32_t>(), b->off_v.as<uint32_t>(), np, b->scan_tmp.as<uint32_t>(), small + 1,
-                      st));
-  step(launch_trace_write_batch(b->tsc_d.as<BatchTraceScene>(), b->block_scene_d.as<uint32_t>(), b->tblocks,
-                                b->off_s.as<uint32_t>(), b->off_v.as<uint32_t>(), b->ms, b->mv,
-                                b->tally_part.as<double>(), st));
-  step(launch_tally_final_batch(b->tally_part.as<double>(), b->tsc_d.as<BatchTraceScene>(), (uint32_t)b->ns,
-                                b->tally9.as<double>(), st));
-  step(kd_build_forest(F, b->level_off.data(), (uint32_t)b->level_off.size() - 1u, b->build_tmp.as<uint32_t>(), st));
-  step(launch_fill_u32(b->head.as<uint32_t>(), 0u, 4, st));
-  step(launch_fill_u32(b->counters.as<uint32_t>(), 0u, 4, st));
-  step(launch_render_batch(R, b->fp64, b->cone, b->tune.block_threads, b->grid, st));
+  hipError_t e;
+  if ((e = launch_trace_count_batch(b->tsc_d.as<BatchTraceScene>(), b->block_scene_d.as<uint32_t>(), b->tblocks,
+                                    b->cnt_s.as<uint32_t>(), b->cnt_v.as<uint32_t>(), st)) != hipSuccess)
+    return e;
+  if ((e = exclusive_scan(b->cnt_s.as<uint32_t>(), b->off_s.as<uint32_t>(), np, b->scan_tmp.as<uint32_t>(), small,
+                          st)) != hipSuccess)
+    return e;
+  if ((e = exclusive_scan(b->cnt_v.as<uint32_t>(), b->off_v.as<uint32_t>(), np, b->scan_tmp.as<uint32_t>(),
+                          small + 1, st)) != hipSuccess)
+    return e;
+  if ((e = launch_trace_write_batch(b->tsc_d.as<BatchTraceScene>(), b->block_scene_d.as<uint32_t>(), b->tblocks,
+                                    b->off_s.as<uint32_t>(), b->off_v.as<uint32_t>(), b->ms, b->mv,
+                                    b->tally_part.as<double>(), st)) != hipSuccess)
+    return e;
+  if ((e = launch_tally_final_batch(b->tally_part.as<double>(), b->tsc_d.as<BatchTraceScene>(), (uint32_t)b->ns,
+                                    b->tally9.as<double>(), st)) != hipSuccess)
+    return e;
+  if (ev_emitted && (e = hipEventRecord(ev_emitted, st)) != hipSuccess) return e;
+  if ((e = kd_build_forest(F, b->level_off.data(), (uint32_t)b->level_off.size() - 1u, b->build_tmp.as<uint32_t>(),
+                           st)) != hipSuccess)
+    return e;
+  if (ev_built && (e = hipEventRecord(ev_built, st)) != hipSuccess) return e;
+  if ((e = launch_fill_u32(b->head.as<uint32_t>(), 0u, 4, st)) != hipSuccess) return e;
+  if ((e = launch_fill_u32(b->counters.as<uint32_t>(), 0u, 4, st)) != hipSuccess) return e;
+  return launch_render_batch(R, b->fp64, b->cone, b->tune.block_threads, b->grid, st);
+}
+
+// Captures one frame over every work item into b->graph / b->exec.
+int capture(pm_batch* b) {
+  release_graph(b);
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = enqueue_frame(b, b->items_d.as<uint2>(), (uint32_t)b->items_h.size(), nullptr, nullptr);
   hipGraph_t g = nullptr;
-  hipError_t e2 = hipStreamEndCapture(st, &g);
+  hipError_t e2 = hipStreamEndCapture(b->stream, &g);
   if (e != hipSuccess || e2 != hipSuccess) {
     if (g) (void)hipGraphDestroy(g);
     set_last_hip(e != hipSuccess ? e : e2, "batch graph capture");
@@ -417,7 +431,9 @@ int run_frame(pm_batch* b, double* ms) {
   return PM_OK;
 }
 
-int apply_tuning(pm_batch* b, const pm_batch_tuning& t, const uint2* items, uint32_t nitems) {
+// capture_graph = false leaves the graph to the caller (the autotuner runs
+// the frame directly on the stream).
+int apply_tuning(pm_batch* b, const pm_batch_tuning& t, bool capture_graph) {
   if (t.leaf_size < 1 || t.leaf_size > 64) return PM_ERR_INVALID;
   if (t.block_threads != 64 && t.block_threads != 128 && t.block_threads != 192 && t.block_threads != 256)
     return PM_ERR_INVALID;
@@ -426,7 +442,11 @@ int apply_tuning(pm_batch* b, const pm_batch_tuning& t, const uint2* items, uint
   b->tune = t;
   if ((r = set_grid(b)) != PM_OK) return r;
   if (t.leaf_size != b->L && (r = setup_forest(b, t.leaf_size)) != PM_OK) return r;
-  return capture(b, items, nitems);
+  if (!capture_graph) {
+    release_graph(b);
+    return PM_OK;
+  }
+  return capture(b);
 }
 
 void destroy(pm_batch* b) {
@@ -510,12 +530,18 @@ int pm_batch_create(const pm_config* cfgs, int n, pm_batch** out) {
   int r = setup_scenes(b);
   if (r == PM_OK) r = size_records(b);
   if (r == PM_OK) r = setup_render(b);
-  if (r == PM_OK) r = apply_tuning(b, t, b->items_d.as<uint2>(), (uint32_t)b->items_h.size());
+  if (r == PM_OK) r = apply_tuning(b, t, true);
   if (r != PM_OK) {
     destroy(b);
     return r;
   }
   b->st.ms_setup = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  b->st.n_scenes = (uint32_t)n;
+  b->st.work_items = (uint32_t)b->items_h.size();
+  for (int i = 0; i < n; ++i) b->st.photons += cfgs[i].n_emit;
+  b->st.surface_records = b->S_tot;
+  b->st.volume_records = b->V_tot;
+  b->st.pixels = b->npix;
   b->tally.assign(9ull * n, 0.0);
   *out = b;
   return PM_OK;
@@ -529,7 +555,7 @@ int pm_batch_destroy(pm_batch* b) {
 
 int pm_batch_set_tuning(pm_batch* b, const pm_batch_tuning* t) {
   if (!b || !t) return PM_ERR_INVALID;
-  return apply_tuning(b, *t, b->items_d.as<uint2>(), (uint32_t)b->items_h.size());
+  return apply_tuning(b, *t, true);
 }
 
 int pm_batch_autotune(pm_batch* b, int sample_stride, pm_batch_tuning* best_out, double* best_ms_out) {
@@ -539,8 +565,27 @@ int pm_batch_autotune(pm_batch* b, int sample_stride, pm_batch_tuning* best_out,
   HIPCHK(b->items_tune_d.ensure(sizeof(uint2) * std::max<size_t>(sub.size(), 1)));
   if (!sub.empty())
     HIPCHK(hipMemcpy(b->items_tune_d.p, sub.data(), sizeof(uint2) * sub.size(), hipMemcpyHostToDevice));
-  const pm_batch_tuning start = b->tune;
-  pm_batch_tuning best = start;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() {
+      for (int i = 0; i < 4; ++i)
+        if (e[i]) (void)hipEventDestroy(e[i]);
+    }
+  } guard{ev};
+  for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&ev[i]));
+  // objective: kd-build + sampled render (the emission does not depend on the tuning)
+  auto trial = [&](double* ms) -> int {
+    HIPCHK(hipEventRecord(ev[0], b->stream));
+    HIPCHK(enqueue_frame(b, b->items_tune_d.as<uint2>(), (uint32_t)sub.size(), ev[1], ev[2]));
+    HIPCHK(hipEventRecord(ev[3], b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, ev[1], ev[3]));
+    *ms = t;
+    return PM_OK;
+  };
+  pm_batch_tuning best = b->tune;
   double best_ms = -1.0;
   const int leaves[4] = {4, 8, 16, 32}, blocks[3] = {64, 128, 256}, slots[2] = {32, 64};
   for (int li = 0; li < 4; ++li)
@@ -551,17 +596,18 @@ int pm_batch_autotune(pm_batch* b, int sample_stride, pm_batch_tuning* best_out,
         t.block_threads = blocks[bi];
         t.heap_slots = slots[si];
         if (t.heap_slots < b->kmax) continue;
-        int r = apply_tuning(b, t, b->items_tune_d.as<uint2>(), (uint32_t)sub.size());
+        int r = apply_tuning(b, t, false);
         if (r != PM_OK) return r;
-        double ms = 0.0;
-        if ((r = run_frame(b, &ms)) != PM_OK) return r;  // warm-up
-        if ((r = run_frame(b, &ms)) != PM_OK) return r;
+        double ms = 0.0, ms2 = 0.0;
+        if ((r = trial(&ms)) != PM_OK) return r;
+        if ((r = trial(&ms2)) != PM_OK) return r;
+        ms = std::min(ms, ms2);
         if (best_ms < 0.0 || ms < best_ms) {
           best_ms = ms;
           best = t;
         }
       }
-  int r = apply_tuning(b, best, b->items_d.as<uint2>(), (uint32_t)b->items_h.size());
+  int r = apply_tuning(b, best, true);
   if (r != PM_OK) return r;
   if (best_out) *best_out = best;
   if (best_ms_out) *best_ms_out = best_ms;
@@ -569,7 +615,11 @@ int pm_batch_autotune(pm_batch* b, int sample_stride, pm_batch_tuning* best_out,
 }
 
 int pm_batch_submit(pm_batch* b) {
-  if (!b || !b->exec) return PM_ERR_INVALID;
+  if (!b) return PM_ERR_INVALID;
+  if (!b->exec) {
+    int r = capture(b);
+    if (r != PM_OK) return r;
+  }
   double ms = 0.0;
   int r = run_frame(b, &ms);
   if (r != PM_OK) return r;
