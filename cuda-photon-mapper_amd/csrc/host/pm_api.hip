@@ -441,7 +441,7 @@ int pm_set_option(pm_context* c, int option, int value) {
       c->blocks_per_cu = value;
       return PM_OK;
     case PM_OPT_CACHE_FUSED_INFER:
-      if (value != 0 && value != 1) return PM_ERR_INVALID;
+      if (value < 0 || value > 2) return PM_ERR_INVALID;
       c->cache.fused_infer = value;
       return PM_OK;
     default:

@@ -124,7 +124,7 @@ int cache_predict_dev(pm_context* c, const float* in, uint32_t n, float* rgb) {
     int maxlds = 0;
     HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
     if ((size_t)maxlds >= mlp_infer_lds_bytes()) {
-      HIPCHK(mlp_infer(k.w, in, n, k.H[0].as<uint16_t>(), rgb, c->stream));
+      HIPCHK(mlp_infer(k.w, in, n, k.H[0].as<uint16_t>(), rgb, c->stream, k.fused_infer == 2 ? 1 : 0));
       return PM_OK;
     }
   }

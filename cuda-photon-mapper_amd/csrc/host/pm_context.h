@@ -64,7 +64,7 @@ struct CacheState {
   uint32_t batch = 1u << 14;
   float lr = 1e-3f;
   uint32_t frame = 0;
-  int fused_infer = 1;     // 1: pm_mlp_infer.hip kernels (when the LDS fits), 0: generic GEMMs
+  int fused_infer = 2;     // pm_mlp_infer.hip kernels (when the LDS fits): 2 = 16x16x32 MFMA, 1 = 16x16x16; 0: generic GEMMs
   int stride = 1;          // march points per volume record
   int cache_surface = 0;   // 0: surface queries are always gathered exactly
   DevBuf rec_cnt, rec_off, rec_in, rec_wt, rec_pix, rec_val, sel_in, sel_t, flags, flag_off, list;
@@ -105,7 +105,7 @@ struct pm_context {
   uint32_t n_emit_total = 0;
   double tally_h[9];
   bool emitted = false, built = false;
-  int gather_mode = 1;
+  int gather_mode = 2;
   int blocks_per_cu = 4;
   pm_stats stats;
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;

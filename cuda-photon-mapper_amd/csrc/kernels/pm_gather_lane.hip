@@ -166,13 +166,14 @@ hipError_t launch_render_lane(const RenderLaunch& r, hipStream_t st) {
 // cost) until the list is exhausted, so scenes of very different sizes keep
 // all waves busy.  Per pixel the arithmetic is lane_pixel's, as in the
 // one-scene render, so every scene's image is bit-identical to its own
-// render.  heap_slots (32 or 64 keys per lane) sizes each wave's LDS heap.
+// render (gather mode 2, the default).  heap_slots (32 or 64 4-byte keys per
+// lane) sizes each wave's LDS heap.
 template <bool FP64T, bool CONE>
 __global__ void __launch_bounds__(256) k_render_batch(BatchRenderLaunch r) {
   extern __shared__ uint32_t lds[];
   const int lane = lane_id();
   const int wave = (int)(threadIdx.x >> 6);
-  uint32_t* heap = lds + (size_t)wave * 2u * (uint32_t)r.heap_slots * 64u;
+  uint32_t* heap = lds + (size_t)wave * (uint32_t)r.heap_slots * 64u;
   unsigned long long ns_tot = 0, nv_tot = 0;
   for (;;) {
     uint32_t it = 0;
@@ -186,8 +187,7 @@ __global__ void __launch_bounds__(256) k_render_batch(BatchRenderLaunch r) {
     const int lx = tx * 8 + (lane & 7), ly = ty * 8 + (lane >> 3);
     if (lx < S.w && ly < S.h) {
       uint32_t ns, nv;
-      V3f v = lane_pixel<FP64T, CONE>(S.sc, S.rp, S.gs, S.gv, heap, r.heap_slots, lane, (uint32_t)lx, (uint32_t)ly,
-                                       ns, nv);
+      V3f v = lane_pixel2<FP64T, CONE>(S.sc, S.rp, S.gs, S.gv, heap, lane, (uint32_t)lx, (uint32_t)ly, ns, nv);
       store_pixel(r.rgb + 3 * S.img_off, nullptr, (size_t)ly * (size_t)S.w + (size_t)lx, v, S.exposure);
       ns_tot += ns;
       nv_tot += nv;
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(256) k_render_batch(BatchRenderLaunch r) {
 }
 
 size_t render_batch_lds_bytes(int block_threads, int heap_slots) {
-  return (size_t)(block_threads / 64) * 2u * (size_t)heap_slots * 64u * sizeof(uint32_t);
+  return (size_t)(block_threads / 64) * (size_t)heap_slots * 64u * sizeof(uint32_t);
 }
 
 hipError_t launch_render_batch(const BatchRenderLaunch& r, int fp64, int cone, int block_threads, int grid,

@@ -42,10 +42,11 @@ __device__ __forceinline__ bf16x4 lds_frag(const uint16_t* p) {
 __global__ void __launch_bounds__(IW) k_mlp_l1(const float* __restrict__ in, uint32_t rows,
                                                 const uint16_t* __restrict__ w1, const float* __restrict__ b1,
                                                 uint16_t* __restrict__ h1) {
-  __shared__ uint16_t W[MLP_HID * W1S];
-  for (int e = threadIdx.x; e < MLP_HID * MLP_KPAD; e += IW) {
-    int n = e / MLP_KPAD, k = e % MLP_KPAD;
-    W[n * W1S + k] = w1[e];
+  __shared__ __align__(16) uint16_t W[MLP_HID * W1S];
+  // 16-byte copies: rows of 224 bf16 = 28 x 8, LDS row stride 464 B
+  for (int e = threadIdx.x; e < MLP_HID * (MLP_KPAD / 8); e += IW) {
+    int n = e / (MLP_KPAD / 8), k8 = e % (MLP_KPAD / 8);
+    *(uint4*)&W[n * W1S + 8 * k8] = ((const uint4*)w1)[e];
   }
   __syncthreads();
   const int lane = lane_id(), wave = threadIdx.x >> 6;
@@ -115,9 +116,10 @@ __global__ void __launch_bounds__(IW) k_mlp_l25(L25Args A) {
   uint16_t* stage_all = lds + (3 * MLP_HID + MLP_OUTPAD) * WHS;
   for (int l = 0; l < 4; ++l) {
     const int nout = l == 3 ? MLP_OUTPAD : MLP_HID;
-    for (int e = threadIdx.x; e < nout * MLP_HID; e += IW) {
-      int n = e / MLP_HID, k = e % MLP_HID;
-      W[(l * MLP_HID + n) * WHS + k] = A.w[l][e];
+    // 16-byte copies: rows of 128 bf16 = 16 x 8, LDS row stride 272 B
+    for (int e = threadIdx.x; e < nout * (MLP_HID / 8); e += IW) {
+      int n = e / (MLP_HID / 8), k8 = e % (MLP_HID / 8);
+      *(uint4*)&W[(l * MLP_HID + n) * WHS + 8 * k8] = ((const uint4*)A.w[l])[e];
     }
   }
   __syncthreads();
@@ -206,13 +208,219 @@ __global__ void __launch_bounds__(IW) k_mlp_l25(L25Args A) {
   }
 }
 
-size_t mlp_infer_h1_rows() { return 1u << 18; }
+// ---------------------------------------------------------------- K = 32 variants
+// gfx950's v_mfma_f32_16x16x32_bf16 does twice the work of the 16x16x16
+// instruction per issue.  Lane layout: A[i][k] in lane i + 16*(k/8), element
+// k%8; B[k][j] in lane j + 16*(k/8); D as for 16x16x16.  A lane's 8 features of
+// one K step are 4 consecutive bands (sin, cos) of one input, so layer 1
+// computes one sin/cos and derives the next 3 bands by the double-angle
+// identities (sin 2x = 2 sin x cos x, cos 2x = (cos x - sin x)(cos x + sin x)),
+// whose FP32 error is far below the BF16 rounding of the features.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ bf16x8 lds_frag8(const uint16_t* p) {
+  uint4 v = *(const uint4*)p;
+  return *(const bf16x8*)&v;
+}
+
+__device__ __forceinline__ bf16x8 pack8(const float* f) {
+  uint4 v;
+  v.x = (uint32_t)f32_to_bf16(f[0]) | ((uint32_t)f32_to_bf16(f[1]) << 16);
+  v.y = (uint32_t)f32_to_bf16(f[2]) | ((uint32_t)f32_to_bf16(f[3]) << 16);
+  v.z = (uint32_t)f32_to_bf16(f[4]) | ((uint32_t)f32_to_bf16(f[5]) << 16);
+  v.w = (uint32_t)f32_to_bf16(f[6]) | ((uint32_t)f32_to_bf16(f[7]) << 16);
+  return *(const bf16x8*)&v;
+}
+
+static constexpr int STS = MLP_HID + 8;  // stage row stride (bf16), 272 B
+
+// Layer 1: features on the fly, W1 in LDS, H1 written through a per-wave
+// 16-row stage as 16-byte row segments.
+__global__ void __launch_bounds__(IW) k_mlp_l1x(const float* __restrict__ in, uint32_t rows,
+                                                 const uint16_t* __restrict__ w1, const float* __restrict__ b1,
+                                                 uint16_t* __restrict__ h1) {
+  __shared__ __align__(16) uint16_t W[MLP_HID * W1S];
+  __shared__ __align__(16) uint16_t stage_all[(IW / 64) * 16 * STS];
+  for (int e = threadIdx.x; e < MLP_HID * (MLP_KPAD / 8); e += IW) {
+    int n = e / (MLP_KPAD / 8), k8 = e % (MLP_KPAD / 8);
+    *(uint4*)&W[n * W1S + 8 * k8] = ((const uint4*)w1)[e];
+  }
+  __syncthreads();
+  const int lane = lane_id(), wave = threadIdx.x >> 6;
+  const int r16 = lane & 15, q = lane >> 4;
+  uint16_t* stage = stage_all + wave * 16 * STS;
+  const uint32_t tiles = (rows + ROWS_PER_WAVE - 1) / ROWS_PER_WAVE;
+  for (uint32_t t = blockIdx.x * (IW / 64) + wave; t < tiles; t += gridDim.x * (IW / 64)) {
+    const uint32_t row0 = t * ROWS_PER_WAVE;
+    f32x4 acc[2][8];
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc[g][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int ks = 0; ks < MLP_KPAD / 32; ++ks) {
+      const int f0 = ks * 32 + 8 * q;  // this lane's 8 features: 4 bands of one input
+      bf16x8 a[2];
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        const uint32_t row = row0 + g * 16 + r16;
+        float f[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (f0 < MLP_FEAT && row < rows) {
+          const int i = f0 / (2 * MLP_BANDS), b = (f0 % (2 * MLP_BANDS)) / 2;
+          const float v = in[(size_t)row * MLP_IN + i];
+          float sn, cs;
+          sincos_turns(v * u2f((uint32_t)(126 + b) << 23), &sn, &cs);  // 2^(b-1)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            f[2 * j] = sn;
+            f[2 * j + 1] = cs;
+            const float s2 = 2.0f * (sn * cs), c2 = (cs - sn) * (cs + sn);
+            sn = s2;
+            cs = c2;
+          }
+        }
+        a[g] = pack8(f);
+      }
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        bf16x8 bb = lds_frag8(&W[(c * 16 + r16) * W1S + ks * 32 + 8 * q]);
+        acc[0][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], bb, acc[0][c], 0, 0, 0);
+        acc[1][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], bb, acc[1][c], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int col = c * 16 + r16;
+        const float bias = b1[col];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          float v = acc[g][c][e] + bias;
+          stage[(4 * q + e) * STS + col] = f32_to_bf16(v > 0.0f ? v : 0.0f);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      // 16 rows x 256 B = 256 segments of 16 B: 4 per lane
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        const int seg = s4 * 64 + lane, r = seg >> 4, c8 = seg & 15;
+        const uint32_t row = row0 + g * 16 + r;
+        if (row < rows) *(uint4*)&h1[(size_t)row * MLP_HID + 8 * c8] = *(const uint4*)&stage[r * STS + 8 * c8];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+  }
+}
+
+// Layers 2..5 with K = 32 steps: 8 waves share the weights (2 per SIMD); each
+// wave converts its 32-row activations D -> A through a 16-row LDS stage.
+static constexpr int IW25 = 512;
+
+__global__ void __launch_bounds__(IW25) k_mlp_l25x(L25Args A) {
+  extern __shared__ uint16_t lds[];
+  uint16_t* W = lds;
+  uint16_t* stage_all = lds + (3 * MLP_HID + MLP_OUTPAD) * WHS;
+  for (int l = 0; l < 4; ++l) {
+    const int nout = l == 3 ? MLP_OUTPAD : MLP_HID;
+    for (int e = threadIdx.x; e < nout * (MLP_HID / 8); e += IW25) {
+      int n = e / (MLP_HID / 8), k8 = e % (MLP_HID / 8);
+      *(uint4*)&W[(l * MLP_HID + n) * WHS + 8 * k8] = ((const uint4*)A.w[l])[e];
+    }
+  }
+  __syncthreads();
+  const int lane = lane_id(), wave = threadIdx.x >> 6;
+  const int r16 = lane & 15, q = lane >> 4;
+  uint16_t* stage = stage_all + wave * 16 * WHS;
+  const uint32_t tiles = (A.rows + ROWS_PER_WAVE - 1) / ROWS_PER_WAVE;
+  for (uint32_t t = blockIdx.x * (IW25 / 64) + wave; t < tiles; t += gridDim.x * (IW25 / 64)) {
+    const uint32_t row0 = t * ROWS_PER_WAVE;
+    bf16x8 a[2][4];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const uint32_t row = row0 + g * 16 + r16;
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (row < A.rows) v = *(const uint4*)&A.h1[(size_t)row * MLP_HID + ks * 32 + 8 * q];
+        a[g][ks] = *(const bf16x8*)&v;
+      }
+    }
+    for (int l = 0; l < 3; ++l) {
+      const uint16_t* Wl = W + l * MLP_HID * WHS;
+      f32x4 acc[2][8];
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[g][c] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          bf16x8 bb = lds_frag8(&Wl[(c * 16 + r16) * WHS + ks * 32 + 8 * q]);
+          acc[0][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][ks], bb, acc[0][c], 0, 0, 0);
+          acc[1][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][ks], bb, acc[1][c], 0, 0, 0);
+        }
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        // bias + ReLU, D layout -> stage (row-major) -> A layout for the next layer
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const int col = c * 16 + r16;
+          const float bias = A.b[l][col];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            float v = acc[g][c][e] + bias;
+            stage[(4 * q + e) * WHS + col] = f32_to_bf16(v > 0.0f ? v : 0.0f);
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) a[g][ks] = lds_frag8(&stage[r16 * WHS + ks * 32 + 8 * q]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      }
+    }
+    const uint16_t* W5 = W + 3 * MLP_HID * WHS;
+    f32x4 y[2] = {f32x4{0.0f, 0.0f, 0.0f, 0.0f}, f32x4{0.0f, 0.0f, 0.0f, 0.0f}};
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      bf16x8 bb = lds_frag8(&W5[r16 * WHS + ks * 32 + 8 * q]);
+      y[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][ks], bb, y[0], 0, 0, 0);
+      y[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][ks], bb, y[1], 0, 0, 0);
+    }
+    if (r16 < MLP_OUT) {
+      const float bias = A.b[3][r16];
+#pragma unroll
+      for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const uint32_t row = row0 + g * 16 + 4 * q + e;
+          if (row < A.rows) {
+            float v = exp_(y[g][e] + bias) - 1.0f;
+            A.rgb[(size_t)row * MLP_OUT + r16] = v > 0.0f ? v : 0.0f;
+          }
+        }
+    }
+  }
+}
+
+size_t mlp_infer_lds_bytes_x() { return (size_t)((3 * MLP_HID + MLP_OUTPAD) * WHS + (IW25 / 64) * 16 * WHS) * 2; }
+
+size_t mlp_infer_h1_rows() { return 1u << 20; }
 size_t mlp_infer_lds_bytes() {
   return (size_t)((3 * MLP_HID + MLP_OUTPAD) * WHS + (IW / 64) * ROWS_PER_WAVE * WHS) * 2;
 }
 
 // Inference for n rows: h1 must hold mlp_infer_h1_rows() x 128 BF16.
-hipError_t mlp_infer(const MlpDev& w, const float* in, uint32_t n, uint16_t* h1, float* rgb, hipStream_t st) {
+hipError_t mlp_infer(const MlpDev& w, const float* in, uint32_t n, uint16_t* h1, float* rgb, hipStream_t st,
+                     int k32) {
   int dev = 0, cus = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -225,7 +433,10 @@ hipError_t mlp_infer(const MlpDev& w, const float* in, uint32_t n, uint16_t* h1,
     const uint32_t tiles = (rows + ROWS_PER_WAVE - 1) / ROWS_PER_WAVE;
     const uint32_t want = (tiles + IW / 64 - 1) / (IW / 64);
     uint32_t g1 = std::min<uint32_t>((uint32_t)cus * 2u, want), g2 = std::min<uint32_t>((uint32_t)cus, want);
-    hipLaunchKernelGGL(k_mlp_l1, dim3(g1), dim3(IW), 0, st, in + (size_t)o * MLP_IN, rows, w.wb[0], w.b[0], h1);
+    const uint32_t want25 = (tiles + IW25 / 64 - 1) / (IW25 / 64);
+    const uint32_t g25 = std::min<uint32_t>((uint32_t)cus, want25);
+    if (k32) hipLaunchKernelGGL(k_mlp_l1x, dim3(g1), dim3(IW), 0, st, in + (size_t)o * MLP_IN, rows, w.wb[0], w.b[0], h1);
+    else hipLaunchKernelGGL(k_mlp_l1, dim3(g1), dim3(IW), 0, st, in + (size_t)o * MLP_IN, rows, w.wb[0], w.b[0], h1);
     L25Args a;
     a.h1 = h1;
     a.rows = rows;
@@ -234,7 +445,8 @@ hipError_t mlp_infer(const MlpDev& w, const float* in, uint32_t n, uint16_t* h1,
       a.b[l] = w.b[l + 1];
     }
     a.rgb = rgb + (size_t)o * MLP_OUT;
-    hipLaunchKernelGGL(k_mlp_l25, dim3(g2), dim3(IW), lds25, st, a);
+    if (k32) hipLaunchKernelGGL(k_mlp_l25x, dim3(g25), dim3(IW25), mlp_infer_lds_bytes_x(), st, a);
+    else hipLaunchKernelGGL(k_mlp_l25, dim3(g2), dim3(IW), lds25, st, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   return hipSuccess;

@@ -160,12 +160,14 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
 
 /* Execution options (no effect on results, only on how the GPU runs them):
  * PM_OPT_GATHER_MODE   0 = one wave64 per radiance query (cross-lane sorted list),
- *                      1 = one lane per query, 8x8 pixel tile per wave (default),
+ *                      1 = one lane per query, 8x8 pixel tile per wave,
  *                      2 = as 1 with 4-byte heap keys (d^2 only) and a second
- *                          fixed-radius pass for the exact neighbour set;
+ *                          fixed-radius pass for the exact neighbour set
+ *                          (default: twice the waves in flight);
  * PM_OPT_BLOCKS_PER_CU workgroups per compute unit of the persistent mode-0 grid;
- * PM_OPT_CACHE_FUSED_INFER 1 = cache inference through the fused LDS-resident
- *                      kernels (default), 0 = through the generic GEMMs. */
+ * PM_OPT_CACHE_FUSED_INFER 2 = cache inference through the fused LDS-resident
+ *                      kernels with 16x16x32 BF16 MFMA (default), 1 = the same
+ *                      with 16x16x16 BF16 MFMA, 0 = through the generic GEMMs. */
 enum { PM_OPT_GATHER_MODE = 1, PM_OPT_BLOCKS_PER_CU = 2, PM_OPT_CACHE_FUSED_INFER = 3 };
 int pm_set_option(pm_context* ctx, int option, int value);
 
