@@ -249,11 +249,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--gather-mode", type=int, default=2,
-                    help="0: wave per query, 1: lane per query, 2: lane per query with 4-byte heap keys")
+    ap.add_argument("--gather-mode", type=int, default=4,
+                    help="0: wave per query; lane per query with 1: 8-byte, 2: 4-byte, 3/4: 2-byte heap keys")
     ap.add_argument("--frame", type=str, default="", help="WxHxSPP override of the frame (profiling runs only)")
     ap.add_argument("--cache-steps", type=int, default=3, help="timed cache frames (0: skip the cache leg)")
-    ap.add_argument("--cache-train-frames", type=int, default=12)
+    ap.add_argument("--cache-train-frames", type=int, default=24)
     ap.add_argument("--cache-stride", type=int, default=4)
     ap.add_argument("--cache-threshold", type=float, default=0.12)
     ap.add_argument("--batch-steps", type=int, default=3, help="timed batch frames (0: skip the batch leg)")
@@ -286,7 +286,7 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": (tr.get("bytes_per_launch") if tr else None),
-            "kernel": "k_render_lane2 (eye ray + march + kNN gather)", "bytes_per_query": bpq,
+            "kernel": "k_render_lane4 (eye ray + march + kNN gather)", "bytes_per_query": bpq,
             "queries_per_launch": int(q_frame), "kernel_ms": round(kern_ms, 3)}
     photons_traced = stats.n_emit_total
     emit_ms = float(np.mean([p[0] for p in per]))

@@ -411,8 +411,13 @@ int pm_render_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev, 
   r.counters = c->counters.as<unsigned long long>();
   HIPCHK(hipMemsetAsync(c->counters.p, 0, 16, c->stream));
   HIPCHK(hipEventRecord(c->e0, c->stream));
-  HIPCHK(r.gather_mode == 2 ? launch_render_lane2(r, c->stream)
-                            : (r.gather_mode == 1 ? launch_render_lane(r, c->stream) : launch_render(r, c->stream)));
+  switch (r.gather_mode) {
+    case 0: HIPCHK(launch_render(r, c->stream)); break;
+    case 1: HIPCHK(launch_render_lane(r, c->stream)); break;
+    case 3: HIPCHK(launch_render_lane3(r, c->stream)); break;
+    case 4: HIPCHK(launch_render_lane4(r, c->stream)); break;
+    default: HIPCHK(launch_render_lane2(r, c->stream)); break;
+  }
   HIPCHK(hipEventRecord(c->e1, c->stream));
   unsigned long long q[2] = {0, 0};
   HIPCHK(hipMemcpyAsync(q, c->counters.p, 16, hipMemcpyDeviceToHost, c->stream));
@@ -433,7 +438,7 @@ int pm_set_option(pm_context* c, int option, int value) {
   if (!c) return PM_ERR_INVALID;
   switch (option) {
     case PM_OPT_GATHER_MODE:
-      if (value < 0 || value > 2) return PM_ERR_INVALID;
+      if (value < 0 || value > 4) return PM_ERR_INVALID;
       c->gather_mode = value;
       return PM_OK;
     case PM_OPT_BLOCKS_PER_CU:

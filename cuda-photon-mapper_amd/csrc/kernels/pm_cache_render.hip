@@ -18,7 +18,7 @@
 //   7. k_accumulate   : per pixel, compensated sum of weight x value / spp.
 // No reference counterpart (the reference has no cache); the plain path it
 // replaces is photon_mapping_kernel, photonMappingKernel.cu:1409-1462.
-#include "pm_lane_knn2.h"
+#include "pm_lane_knn3.h"
 #include "pm_mlp.h"
 
 namespace pm {
@@ -109,17 +109,17 @@ __global__ void __launch_bounds__(64) k_rec(CacheRecordLaunch r) {
 }
 
 // Exact kNN value (albedo-free for surfaces) of records list[i] (or i).
-__global__ void __launch_bounds__(64) k_values(CacheValueLaunch v) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_values(CacheValueLaunch v) {
   extern __shared__ uint32_t heap_lds[];
   const int lane = lane_id();
   const uint32_t i = blockIdx.x * 64u + (uint32_t)lane;
   if (i >= v.n) return;
   const uint32_t rec = v.list ? v.list[i] : i;
   const float* p = v.in + (size_t)rec * MLP_IN;
-  LaneGather2<false> g;  // 4-byte heap: 16 KB of LDS per wave
+  LaneGather3<false, 4> g;  // 2-byte heap: 8 KB of LDS per wave
   g.gs = v.gs;
   g.gv = v.gv;
-  g.hp.H = heap_lds;
+  g.hp.H = (uint16_t*)heap_lds;
   g.hp.lane = lane;
   g.hp.k = 0;
   g.hp.size = 0;
@@ -259,7 +259,7 @@ hipError_t launch_cache_record(const CacheRecordLaunch& r, hipStream_t st) {
 
 hipError_t launch_cache_values(const CacheValueLaunch& v, hipStream_t st) {
   if (v.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_values, dim3((v.n + 63) / 64), dim3(64), 64 * 64 * sizeof(uint32_t), st, v);
+  hipLaunchKernelGGL(k_values, dim3((v.n + 63) / 64), dim3(64), 64 * 64 * sizeof(uint16_t), st, v);
   return hipGetLastError();
 }
 

@@ -18,6 +18,7 @@
 //    so the result is always the exact kNN set.
 #include "pm_lane_knn.h"
 #include "pm_lane_knn2.h"
+#include "pm_lane_knn3.h"
 
 namespace pm {
 
@@ -58,6 +59,34 @@ __device__ __forceinline__ V3f lane_pixel2(const Scene& sc, const RenderParams& 
                                            const GatherMap& gv, uint32_t* heap, int lane, uint32_t px, uint32_t py,
                                            uint32_t& nsurf, uint32_t& nvol) {
   LaneGather2<CONE> g;
+  g.gs = gs;
+  g.gv = gv;
+  g.hp.H = heap;
+  g.hp.lane = lane;
+  g.hp.k = 0;
+  g.hp.size = 0;
+  g.hg_g = sc.med.g;
+  g.nsurf = 0u;
+  g.nvol = 0u;
+  g.prev_r = -1.0f;
+  NSum3<float> acc;
+  nsum3_init(acc);
+  for (int s = 0; s < rp.spp; ++s) {
+    V3f v = FP64T ? render_sample_t<float, double>(sc, rp, px, py, (uint32_t)s, g)
+                  : render_sample_t<float, float>(sc, rp, px, py, (uint32_t)s, g);
+    nsum3_add(acc, v);
+  }
+  nsurf = g.nsurf;
+  nvol = g.nvol;
+  return nsum3_get(acc) * (1.0f / (float)rp.spp);
+}
+
+// The same with the 2-byte-heap gather (pm_lane_knn3.h).
+template <bool FP64T, bool CONE, int NB = 8>
+__device__ __forceinline__ V3f lane_pixel3(const Scene& sc, const RenderParams& rp, const GatherMap& gs,
+                                           const GatherMap& gv, uint16_t* heap, int lane, uint32_t px, uint32_t py,
+                                           uint32_t& nsurf, uint32_t& nvol) {
+  LaneGather3<CONE, NB> g;
   g.gs = gs;
   g.gv = gv;
   g.hp.H = heap;
@@ -136,6 +165,73 @@ __global__ void __launch_bounds__(64) k_render_lane2(RenderLaunch r) {
   }
 }
 
+// Gather modes 3 and 4: the 2-byte heap (8 KB of LDS per wave), leaf points
+// loaded in batches of NB = 8 (mode 3) or 4 (mode 4).
+template <bool FP64T, bool CONE, int NB>
+__global__ void __launch_bounds__(64) k_render_lane3(RenderLaunch r) {
+  extern __shared__ uint16_t heap16_lds[];  // [64 slots][64 lanes]
+  const int lane = lane_id();
+  const int tiles_x = (r.w + 7) / 8;
+  const int tile = (int)blockIdx.x;
+  const int tx = tile % tiles_x, ty = tile / tiles_x;
+  const int lx = tx * 8 + (lane & 7), ly = ty * 8 + (lane >> 3);
+  if (lx >= r.w || ly >= r.h) return;
+  const uint32_t px = (uint32_t)(r.x0 + lx), py = (uint32_t)(r.y0 + ly);
+  uint32_t ns, nv;
+  V3f v = lane_pixel3<FP64T, CONE, NB>(r.sc, r.rp, r.gs, r.gv, heap16_lds, lane, px, py, ns, nv);
+  store_pixel(r.rgb, r.u8, (size_t)ly * (size_t)r.w + (size_t)lx, v, r.exposure);
+  if (r.counters) {
+    atomicAdd(&r.counters[0], (unsigned long long)ns);
+    atomicAdd(&r.counters[1], (unsigned long long)nv);
+  }
+}
+
+// Mode 4: batches of 4 with the register budget of 5 waves per SIMD.
+template <bool FP64T, bool CONE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_render_lane4(RenderLaunch r) {
+  extern __shared__ uint16_t heap16_lds[];  // [64 slots][64 lanes]
+  const int lane = lane_id();
+  const int tiles_x = (r.w + 7) / 8;
+  const int tile = (int)blockIdx.x;
+  const int tx = tile % tiles_x, ty = tile / tiles_x;
+  const int lx = tx * 8 + (lane & 7), ly = ty * 8 + (lane >> 3);
+  if (lx >= r.w || ly >= r.h) return;
+  const uint32_t px = (uint32_t)(r.x0 + lx), py = (uint32_t)(r.y0 + ly);
+  uint32_t ns, nv;
+  V3f v = lane_pixel3<FP64T, CONE, 4>(r.sc, r.rp, r.gs, r.gv, heap16_lds, lane, px, py, ns, nv);
+  store_pixel(r.rgb, r.u8, (size_t)ly * (size_t)r.w + (size_t)lx, v, r.exposure);
+  if (r.counters) {
+    atomicAdd(&r.counters[0], (unsigned long long)ns);
+    atomicAdd(&r.counters[1], (unsigned long long)nv);
+  }
+}
+
+template <int NB>
+hipError_t launch_render_lane3_nb(const RenderLaunch& r, hipStream_t st) {
+  if (r.w <= 0 || r.h <= 0) return hipSuccess;
+  unsigned tiles = (unsigned)(((r.w + 7) / 8) * ((r.h + 7) / 8));
+  size_t lds = 64 * 64 * sizeof(uint16_t);
+  const bool fp64 = r.rp.fp64_transmittance != 0, cone = r.estimator == 1;
+  if (fp64 && cone) hipLaunchKernelGGL((k_render_lane3<true, true, NB>), dim3(tiles), dim3(64), lds, st, r);
+  else if (fp64) hipLaunchKernelGGL((k_render_lane3<true, false, NB>), dim3(tiles), dim3(64), lds, st, r);
+  else if (cone) hipLaunchKernelGGL((k_render_lane3<false, true, NB>), dim3(tiles), dim3(64), lds, st, r);
+  else hipLaunchKernelGGL((k_render_lane3<false, false, NB>), dim3(tiles), dim3(64), lds, st, r);
+  return hipGetLastError();
+}
+
+hipError_t launch_render_lane3(const RenderLaunch& r, hipStream_t st) { return launch_render_lane3_nb<8>(r, st); }
+hipError_t launch_render_lane4(const RenderLaunch& r, hipStream_t st) {
+  if (r.w <= 0 || r.h <= 0) return hipSuccess;
+  unsigned tiles = (unsigned)(((r.w + 7) / 8) * ((r.h + 7) / 8));
+  size_t lds = 64 * 64 * sizeof(uint16_t);
+  const bool fp64 = r.rp.fp64_transmittance != 0, cone = r.estimator == 1;
+  if (fp64 && cone) hipLaunchKernelGGL((k_render_lane4<true, true>), dim3(tiles), dim3(64), lds, st, r);
+  else if (fp64) hipLaunchKernelGGL((k_render_lane4<true, false>), dim3(tiles), dim3(64), lds, st, r);
+  else if (cone) hipLaunchKernelGGL((k_render_lane4<false, true>), dim3(tiles), dim3(64), lds, st, r);
+  else hipLaunchKernelGGL((k_render_lane4<false, false>), dim3(tiles), dim3(64), lds, st, r);
+  return hipGetLastError();
+}
+
 hipError_t launch_render_lane2(const RenderLaunch& r, hipStream_t st) {
   if (r.w <= 0 || r.h <= 0) return hipSuccess;
   unsigned tiles = (unsigned)(((r.w + 7) / 8) * ((r.h + 7) / 8));
@@ -166,14 +262,14 @@ hipError_t launch_render_lane(const RenderLaunch& r, hipStream_t st) {
 // cost) until the list is exhausted, so scenes of very different sizes keep
 // all waves busy.  Per pixel the arithmetic is lane_pixel's, as in the
 // one-scene render, so every scene's image is bit-identical to its own
-// render (gather mode 2, the default).  heap_slots (32 or 64 4-byte keys per
+// render (gather mode 4, the default).  heap_slots (32 or 64 2-byte keys per
 // lane) sizes each wave's LDS heap.
 template <bool FP64T, bool CONE>
 __global__ void __launch_bounds__(256) k_render_batch(BatchRenderLaunch r) {
   extern __shared__ uint32_t lds[];
   const int lane = lane_id();
   const int wave = (int)(threadIdx.x >> 6);
-  uint32_t* heap = lds + (size_t)wave * (uint32_t)r.heap_slots * 64u;
+  uint16_t* heap = (uint16_t*)lds + (size_t)wave * (uint32_t)r.heap_slots * 64u;
   unsigned long long ns_tot = 0, nv_tot = 0;
   for (;;) {
     uint32_t it = 0;
@@ -181,13 +277,15 @@ __global__ void __launch_bounds__(256) k_render_batch(BatchRenderLaunch r) {
     it = read_lane(it, 0);
     if (it >= r.nitems) break;
     const uint2 item = r.items[it];
-    const BatchRenderScene& S = r.scenes[item.x];
+    // wave-uniform scene and tile: scalar loads of the scene descriptor
+    const BatchRenderScene& S = r.scenes[uniform(item.x)];
+    const uint32_t tile = uniform(item.y);
     const int tiles_x = (S.w + 7) / 8;
-    const int tx = (int)item.y % tiles_x, ty = (int)item.y / tiles_x;
+    const int tx = (int)tile % tiles_x, ty = (int)tile / tiles_x;
     const int lx = tx * 8 + (lane & 7), ly = ty * 8 + (lane >> 3);
     if (lx < S.w && ly < S.h) {
       uint32_t ns, nv;
-      V3f v = lane_pixel2<FP64T, CONE>(S.sc, S.rp, S.gs, S.gv, heap, lane, (uint32_t)lx, (uint32_t)ly, ns, nv);
+      V3f v = lane_pixel3<FP64T, CONE, 4>(S.sc, S.rp, S.gs, S.gv, heap, lane, (uint32_t)lx, (uint32_t)ly, ns, nv);
       store_pixel(r.rgb + 3 * S.img_off, nullptr, (size_t)ly * (size_t)S.w + (size_t)lx, v, S.exposure);
       ns_tot += ns;
       nv_tot += nv;
@@ -200,7 +298,7 @@ __global__ void __launch_bounds__(256) k_render_batch(BatchRenderLaunch r) {
 }
 
 size_t render_batch_lds_bytes(int block_threads, int heap_slots) {
-  return (size_t)(block_threads / 64) * (size_t)heap_slots * 64u * sizeof(uint32_t);
+  return (size_t)(block_threads / 64) * (size_t)heap_slots * 64u * sizeof(uint16_t);
 }
 
 hipError_t launch_render_batch(const BatchRenderLaunch& r, int fp64, int cone, int block_threads, int grid,

@@ -119,11 +119,13 @@ struct RenderLaunch {
   uchar4* u8;      // w*h or null
   unsigned long long* counters;  // [surface queries, volume queries]
   int blocks_per_cu;             // persistent grid: workgroups per compute unit
-  int gather_mode;               // 0: one wave per query, 1: one lane per query, 2: lane, 4-byte heap
+  int gather_mode;               // 0: wave per query; lane per query: 1 8-byte, 2 4-byte, 3 2-byte heap
 };
 hipError_t launch_render(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane2(const RenderLaunch& r, hipStream_t st);
+hipError_t launch_render_lane3(const RenderLaunch& r, hipStream_t st);
+hipError_t launch_render_lane4(const RenderLaunch& r, hipStream_t st);
 
 // batched scenes (pm_trace.hip, pm_gather_lane.hip)
 // Emission: scene s owns the trace blocks [block0, block0 + nblocks), whose

@@ -59,10 +59,10 @@ struct LaneHeap32 {
 };
 
 // Stackless near-first traversal (the loop structure of lane_knn) that hands
-// the leaf points, in batches of up to 8, to a visitor:
+// the leaf points, in batches of up to NB (loads issued together), to a visitor:
 //   uint32_t V::bound()  -- a far child is visited iff diff^2 <= u2f(bound())
-//   void V::points(const float4* p, const uint32_t* dh, int cnt)
-template <class V>
+//   template <int NB> void V::points(const float4* p, const uint32_t* dh, int cnt)
+template <int NB = 8, class V>
 __device__ __forceinline__ void lane_visit(const GatherMap& g, float qx, float qy, float qz, V& v) {
   const uint32_t n = g.n, nl = g.nl;
   if (n == 0) return;
@@ -117,18 +117,18 @@ __device__ __forceinline__ void lane_visit(const GatherMap& g, float qx, float q
     const bool at_leaf = !done && down && h + 1u >= nl;
     if (ballot(at_leaf) == 0ull) break;
     if (!at_leaf) continue;
-    for (uint32_t b0 = lo; b0 < hi; b0 += 8u) {
-      float4 p[8];
+    for (uint32_t b0 = lo; b0 < hi; b0 += (uint32_t)NB) {
+      float4 p[NB];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int j = 0; j < NB; ++j) {
         uint32_t i = b0 + (uint32_t)j;
         p[j] = g.tpts[i < hi ? i : hi - 1u];
       }
-      uint32_t dh[8];
+      uint32_t dh[NB];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) dh[j] = f2u(dist2(qx, qy, qz, p[j].x, p[j].y, p[j].z));
+      for (int j = 0; j < NB; ++j) dh[j] = f2u(dist2(qx, qy, qz, p[j].x, p[j].y, p[j].z));
       const uint32_t left = hi - b0;
-      v.points(p, dh, left < 8u ? (int)left : 8);
+      v.template points<NB>(p, dh, left < (uint32_t)NB ? (int)left : NB);
     }
     from = h;
     down = false;
@@ -140,9 +140,10 @@ struct Knn32Visitor {
   LaneHeap32* hp;
   uint32_t wh;  // prune bound: the search bound until the heap is full, then its root
   __device__ uint32_t bound() const { return wh; }
+  template <int NB>
   __device__ void points(const float4*, const uint32_t* dh, int cnt) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < NB; ++j) {
       if (j < cnt) {
         const uint32_t d = dh[j];
         if (hp->size < hp->k) {
@@ -169,9 +170,10 @@ struct SumVisitor {
   uint32_t n_less, n_tie, tie_min;
   Add* add;
   __device__ uint32_t bound() const { return D; }
+  template <int NB>
   __device__ void points(const float4* p, const uint32_t* dh, int cnt) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < NB; ++j) {
       if (j < cnt) {
         const uint32_t d = dh[j];
         const uint32_t idx = f2u(p[j].w);
@@ -191,9 +193,10 @@ struct SumVisitor {
 struct TieVisitor {
   uint32_t D, last, best;
   __device__ uint32_t bound() const { return D; }
+  template <int NB>
   __device__ void points(const float4* p, const uint32_t* dh, int cnt) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < NB; ++j) {
       if (j < cnt && dh[j] == D) {
         const uint32_t idx = f2u(p[j].w);
         if (idx > last && idx < best) best = idx;

@@ -161,9 +161,14 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
 /* Execution options (no effect on results, only on how the GPU runs them):
  * PM_OPT_GATHER_MODE   0 = one wave64 per radiance query (cross-lane sorted list),
  *                      1 = one lane per query, 8x8 pixel tile per wave,
+ *                          8-byte (d^2, index) heap keys in LDS,
  *                      2 = as 1 with 4-byte heap keys (d^2 only) and a second
- *                          fixed-radius pass for the exact neighbour set
- *                          (default: twice the waves in flight);
+ *                          fixed-radius pass for the exact neighbour set,
+ *                      3 = as 2 with 2-byte keys (d^2 buckets 2^-7 wide) and
+ *                          the exact order restored inside the k-th bucket,
+ *                      4 = as 3, leaf points loaded 4 at a time under a
+ *                          5-waves-per-SIMD register budget (default);
+ *                      every mode gives the same neighbour sets and images;
  * PM_OPT_BLOCKS_PER_CU workgroups per compute unit of the persistent mode-0 grid;
  * PM_OPT_CACHE_FUSED_INFER 2 = cache inference through the fused LDS-resident
  *                      kernels with 16x16x32 BF16 MFMA (default), 1 = the same
