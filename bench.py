@@ -157,8 +157,8 @@ def run_batch(steps, tune_stride, n_scenes=64, seed=1):
     t = time.perf_counter()
     best, tune_ms = b.autotune(tune_stride)
     t_tune = time.perf_counter() - t
-    log("batch: %d scenes, setup %.1f s, autotune %.1f s -> leaf %d, block %d, heap slots %d"
-        % (n_scenes, t_setup, t_tune, best[0], best[1], best[2]))
+    log("batch: %d scenes, setup %.1f s, autotune %.1f s -> leaf %d, block %d, heap slots %d, key bytes %d"
+        % (n_scenes, t_setup, t_tune, best[0], best[1], best[2], best[3]))
     b.submit()
     per = []
     for i in range(steps):
@@ -170,9 +170,11 @@ def run_batch(steps, tune_stride, n_scenes=64, seed=1):
     ms = float(np.median(per))
     b.close()
     alone = 0.0
+    mode = {8: 1, 4: 2, 2: 4}[best[3]]  # the one-scene gather with the same heap keys
     for c in cfgs:
         c.leaf_size = best[0]
         g = pmamd.PhotonMapper(c)
+        g.set_option(pmamd.PM_OPT_GATHER_MODE, mode)
         buf = pmamd.DeviceBuffer(c.width * c.height * 12)
         g.emit()
         g.build()
@@ -188,7 +190,7 @@ def run_batch(steps, tune_stride, n_scenes=64, seed=1):
         "photons": int(st.photons), "pixels": int(st.pixels), "surface_records": int(st.surface_records),
         "volume_records": int(st.volume_records), "radiance_queries": int(q),
         "ms_per_frame": round(ms, 3), "radiance_queries_per_s": round(q / (ms * 1e-3), 1),
-        "tuning": {"leaf_size": best[0], "block_threads": best[1], "heap_slots": best[2],
+        "tuning": {"leaf_size": best[0], "block_threads": best[1], "heap_slots": best[2], "heap_key_bytes": best[3],
                    "autotune_sample_stride": tune_stride, "autotune_s": round(t_tune, 2)},
         "graph_nodes": int(st.graph_nodes), "work_items": int(st.work_items),
         "scenes_one_at_a_time_ms": round(alone, 3), "speedup_vs_one_at_a_time": round(alone / ms, 3),
@@ -249,8 +251,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--gather-mode", type=int, default=4,
-                    help="0: wave per query; lane per query with 1: 8-byte, 2: 4-byte, 3/4: 2-byte heap keys")
+    ap.add_argument("--gather-mode", type=int, default=5,
+                    help="0: wave per query; lane per query with 1: 8-byte, 2: 4-byte, 3/4: 2-byte heap keys; 5: auto")
     ap.add_argument("--frame", type=str, default="", help="WxHxSPP override of the frame (profiling runs only)")
     ap.add_argument("--cache-steps", type=int, default=3, help="timed cache frames (0: skip the cache leg)")
     ap.add_argument("--cache-train-frames", type=int, default=24)

@@ -331,7 +331,8 @@ int setup_render(pm_batch* b) {
 }
 
 int set_grid(pm_batch* b) {
-  const int occ = render_batch_occupancy(b->fp64, b->cone, b->tune.block_threads, b->tune.heap_slots);
+  const int occ = render_batch_occupancy(b->fp64, b->cone, b->tune.block_threads, b->tune.heap_slots,
+                                         b->tune.heap_key_bytes);
   if (occ < 1) return PM_ERR_INVALID;
   b->grid = b->cus * occ;
   return PM_OK;
@@ -373,6 +374,7 @@ hipError_t enqueue_frame(pm_batch* b, const uint2* items, uint32_t nitems, hipEv
   R.rgb = b->rgb.as<float>();
   R.counters = b->counters.as<unsigned long long>();
   R.heap_slots = b->tune.heap_slots;
+  R.key_bytes = b->tune.heap_key_bytes;
   hipError_t e;
   if ((e = launch_trace_count_batch(b->tsc_d.as<BatchTraceScene>(), b->block_scene_d.as<uint32_t>(), b->tblocks,
                                     b->cnt_s.as<uint32_t>(), b->cnt_v.as<uint32_t>(), st)) != hipSuccess)
@@ -438,6 +440,7 @@ int apply_tuning(pm_batch* b, const pm_batch_tuning& t, bool capture_graph) {
   if (t.block_threads != 64 && t.block_threads != 128 && t.block_threads != 192 && t.block_threads != 256)
     return PM_ERR_INVALID;
   if ((t.heap_slots != 32 && t.heap_slots != 64) || t.heap_slots < b->kmax) return PM_ERR_INVALID;
+  if (t.heap_key_bytes != 8 && t.heap_key_bytes != 4 && t.heap_key_bytes != 2) return PM_ERR_INVALID;
   int r;
   b->tune = t;
   if ((r = set_grid(b)) != PM_OK) return r;
@@ -527,6 +530,7 @@ int pm_batch_create(const pm_config* cfgs, int n, pm_batch** out) {
   t.leaf_size = 32;
   t.block_threads = 64;
   t.heap_slots = b->kmax <= 32 ? 32 : 64;
+  t.heap_key_bytes = 2;
   int r = setup_scenes(b);
   if (r == PM_OK) r = size_records(b);
   if (r == PM_OK) r = setup_render(b);
@@ -585,28 +589,49 @@ int pm_batch_autotune(pm_batch* b, int sample_stride, pm_batch_tuning* best_out,
     *ms = t;
     return PM_OK;
   };
+  // staged sweep: heap keys (width x slots), then leaf size, then workgroup size
   pm_batch_tuning best = b->tune;
   double best_ms = -1.0;
-  const int leaves[4] = {4, 8, 16, 32}, blocks[3] = {64, 128, 256}, slots[2] = {32, 64};
-  for (int li = 0; li < 4; ++li)
-    for (int bi = 0; bi < 3; ++bi)
-      for (int si = 0; si < 2; ++si) {
-        pm_batch_tuning t;
-        t.leaf_size = leaves[li];
-        t.block_threads = blocks[bi];
-        t.heap_slots = slots[si];
-        if (t.heap_slots < b->kmax) continue;
-        int r = apply_tuning(b, t, false);
-        if (r != PM_OK) return r;
-        double ms = 0.0, ms2 = 0.0;
-        if ((r = trial(&ms)) != PM_OK) return r;
-        if ((r = trial(&ms2)) != PM_OK) return r;
-        ms = std::min(ms, ms2);
-        if (best_ms < 0.0 || ms < best_ms) {
-          best_ms = ms;
-          best = t;
-        }
-      }
+  auto consider = [&](const pm_batch_tuning& t) -> int {
+    if (t.heap_slots < b->kmax) return PM_OK;
+    int r = apply_tuning(b, t, false);
+    if (r != PM_OK) return r;
+    double ms = 0.0, ms2 = 0.0;
+    if ((r = trial(&ms)) != PM_OK) return r;
+    if ((r = trial(&ms2)) != PM_OK) return r;
+    ms = std::min(ms, ms2);
+    if (best_ms < 0.0 || ms < best_ms) {
+      best_ms = ms;
+      best = t;
+    }
+    return PM_OK;
+  };
+  const int keyb[3] = {8, 4, 2}, slots[2] = {32, 64}, leaves[4] = {4, 8, 16, 32}, blocks[3] = {64, 128, 256};
+  const pm_batch_tuning start = b->tune;
+  for (int kb : keyb)
+    for (int sl : slots) {
+      pm_batch_tuning t = start;
+      t.heap_key_bytes = kb;
+      t.heap_slots = sl;
+      int r = consider(t);
+      if (r != PM_OK) return r;
+    }
+  const pm_batch_tuning s1 = best;
+  for (int L : leaves) {
+    if (L == s1.leaf_size) continue;
+    pm_batch_tuning t = s1;
+    t.leaf_size = L;
+    int r = consider(t);
+    if (r != PM_OK) return r;
+  }
+  const pm_batch_tuning s2 = best;
+  for (int bt : blocks) {
+    if (bt == s2.block_threads) continue;
+    pm_batch_tuning t = s2;
+    t.block_threads = bt;
+    int r = consider(t);
+    if (r != PM_OK) return r;
+  }
   int r = apply_tuning(b, best, true);
   if (r != PM_OK) return r;
   if (best_out) *best_out = best;

@@ -105,7 +105,7 @@ struct pm_context {
   uint32_t n_emit_total = 0;
   double tally_h[9];
   bool emitted = false, built = false;
-  int gather_mode = 4;
+  int gather_mode = 5;  // auto
   int blocks_per_cu = 4;
   pm_stats stats;
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;

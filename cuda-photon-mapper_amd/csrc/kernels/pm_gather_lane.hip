@@ -260,16 +260,17 @@ hipError_t launch_render_lane(const RenderLaunch& r, hipStream_t st) {
 // batch.  Each wave takes the next (scene, tile) item from a device work
 // queue (an atomic counter over an item list ordered by decreasing expected
 // cost) until the list is exhausted, so scenes of very different sizes keep
-// all waves busy.  Per pixel the arithmetic is lane_pixel's, as in the
-// one-scene render, so every scene's image is bit-identical to its own
-// render (gather mode 4, the default).  heap_slots (32 or 64 2-byte keys per
-// lane) sizes each wave's LDS heap.
-template <bool FP64T, bool CONE>
+// all waves busy.  Per pixel the arithmetic is that of the one-scene lane
+// render with the same heap keys (KB = 8: mode 1, 4: mode 2, 2: mode 4), so
+// every scene's image is bit-identical to its own render.  heap_slots (32 or
+// 64 keys per lane) and KB size each wave's LDS heap.
+template <bool FP64T, bool CONE, int KB>
 __global__ void __launch_bounds__(256) k_render_batch(BatchRenderLaunch r) {
   extern __shared__ uint32_t lds[];
   const int lane = lane_id();
   const int wave = (int)(threadIdx.x >> 6);
-  uint16_t* heap = (uint16_t*)lds + (size_t)wave * (uint32_t)r.heap_slots * 64u;
+  const size_t heap_words = (size_t)r.heap_slots * 64u * KB / 4u;
+  uint32_t* heap = lds + (size_t)wave * heap_words;
   unsigned long long ns_tot = 0, nv_tot = 0;
   for (;;) {
     uint32_t it = 0;
@@ -285,7 +286,15 @@ __global__ void __launch_bounds__(256) k_render_batch(BatchRenderLaunch r) {
     const int lx = tx * 8 + (lane & 7), ly = ty * 8 + (lane >> 3);
     if (lx < S.w && ly < S.h) {
       uint32_t ns, nv;
-      V3f v = lane_pixel3<FP64T, CONE, 4>(S.sc, S.rp, S.gs, S.gv, heap, lane, (uint32_t)lx, (uint32_t)ly, ns, nv);
+      V3f v;
+      if (KB == 8)
+        v = lane_pixel<FP64T, CONE>(S.sc, S.rp, S.gs, S.gv, heap, r.heap_slots, lane, (uint32_t)lx, (uint32_t)ly, ns,
+                                    nv);
+      else if (KB == 4)
+        v = lane_pixel2<FP64T, CONE>(S.sc, S.rp, S.gs, S.gv, heap, lane, (uint32_t)lx, (uint32_t)ly, ns, nv);
+      else
+        v = lane_pixel3<FP64T, CONE, 4>(S.sc, S.rp, S.gs, S.gv, (uint16_t*)heap, lane, (uint32_t)lx, (uint32_t)ly, ns,
+                                        nv);
       store_pixel(r.rgb + 3 * S.img_off, nullptr, (size_t)ly * (size_t)S.w + (size_t)lx, v, S.exposure);
       ns_tot += ns;
       nv_tot += nv;
@@ -297,34 +306,49 @@ __global__ void __launch_bounds__(256) k_render_batch(BatchRenderLaunch r) {
   }
 }
 
-size_t render_batch_lds_bytes(int block_threads, int heap_slots) {
-  return (size_t)(block_threads / 64) * (size_t)heap_slots * 64u * sizeof(uint16_t);
+size_t render_batch_lds_bytes(int block_threads, int heap_slots, int key_bytes) {
+  return (size_t)(block_threads / 64) * (size_t)heap_slots * 64u * (size_t)key_bytes;
+}
+
+template <int KB>
+static hipError_t launch_render_batch_kb(const BatchRenderLaunch& r, int fp64, int cone, int block_threads, int grid,
+                                         hipStream_t st) {
+  const size_t lds = render_batch_lds_bytes(block_threads, r.heap_slots, KB);
+  if (fp64 && cone) hipLaunchKernelGGL((k_render_batch<true, true, KB>), dim3(grid), dim3(block_threads), lds, st, r);
+  else if (fp64) hipLaunchKernelGGL((k_render_batch<true, false, KB>), dim3(grid), dim3(block_threads), lds, st, r);
+  else if (cone) hipLaunchKernelGGL((k_render_batch<false, true, KB>), dim3(grid), dim3(block_threads), lds, st, r);
+  else hipLaunchKernelGGL((k_render_batch<false, false, KB>), dim3(grid), dim3(block_threads), lds, st, r);
+  return hipGetLastError();
 }
 
 hipError_t launch_render_batch(const BatchRenderLaunch& r, int fp64, int cone, int block_threads, int grid,
                                hipStream_t st) {
   if (r.nitems == 0) return hipSuccess;
-  const size_t lds = render_batch_lds_bytes(block_threads, r.heap_slots);
-  if (fp64 && cone) hipLaunchKernelGGL((k_render_batch<true, true>), dim3(grid), dim3(block_threads), lds, st, r);
-  else if (fp64) hipLaunchKernelGGL((k_render_batch<true, false>), dim3(grid), dim3(block_threads), lds, st, r);
-  else if (cone) hipLaunchKernelGGL((k_render_batch<false, true>), dim3(grid), dim3(block_threads), lds, st, r);
-  else hipLaunchKernelGGL((k_render_batch<false, false>), dim3(grid), dim3(block_threads), lds, st, r);
-  return hipGetLastError();
+  if (r.key_bytes == 8) return launch_render_batch_kb<8>(r, fp64, cone, block_threads, grid, st);
+  if (r.key_bytes == 4) return launch_render_batch_kb<4>(r, fp64, cone, block_threads, grid, st);
+  return launch_render_batch_kb<2>(r, fp64, cone, block_threads, grid, st);
 }
 
-int render_batch_occupancy(int fp64, int cone, int block_threads, int heap_slots) {
-  const size_t lds = render_batch_lds_bytes(block_threads, heap_slots);
+template <int KB>
+static int occupancy_kb(int fp64, int cone, int block_threads, size_t lds) {
   int nb = 0;
   hipError_t e;
   if (fp64 && cone)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<true, true>, block_threads, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<true, true, KB>, block_threads, lds);
   else if (fp64)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<true, false>, block_threads, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<true, false, KB>, block_threads, lds);
   else if (cone)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<false, true>, block_threads, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<false, true, KB>, block_threads, lds);
   else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<false, false>, block_threads, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<false, false, KB>, block_threads, lds);
   return e == hipSuccess ? nb : 0;
+}
+
+int render_batch_occupancy(int fp64, int cone, int block_threads, int heap_slots, int key_bytes) {
+  const size_t lds = render_batch_lds_bytes(block_threads, heap_slots, key_bytes);
+  if (key_bytes == 8) return occupancy_kb<8>(fp64, cone, block_threads, lds);
+  if (key_bytes == 4) return occupancy_kb<4>(fp64, cone, block_threads, lds);
+  return occupancy_kb<2>(fp64, cone, block_threads, lds);
 }
 
 __global__ void k_tonemap(const float* __restrict__ rgb, size_t npix, float exposure, uchar4* __restrict__ out) {

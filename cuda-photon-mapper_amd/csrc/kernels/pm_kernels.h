@@ -163,9 +163,10 @@ struct BatchRenderLaunch {
   float* rgb;                      // image arena, 3 floats per pixel
   unsigned long long* counters;    // [surface queries, volume queries]
   int heap_slots;                  // LDS heap keys per lane (>= every scene's k)
+  int key_bytes;                   // heap key width: 8, 4 or 2 (gather modes 1, 2, 4)
 };
-size_t render_batch_lds_bytes(int block_threads, int heap_slots);
-int render_batch_occupancy(int fp64, int cone, int block_threads, int heap_slots);  // workgroups per CU
+size_t render_batch_lds_bytes(int block_threads, int heap_slots, int key_bytes);
+int render_batch_occupancy(int fp64, int cone, int block_threads, int heap_slots, int key_bytes);  // per CU
 hipError_t launch_tonemap(const float* rgb, size_t npix, float exposure, uchar4* out, hipStream_t st);
 hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t n, uint32_t* dst, hipStream_t st);
 hipError_t launch_render_batch(const BatchRenderLaunch& r, int fp64, int cone, int block_threads, int grid,

@@ -167,7 +167,10 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
  *                      3 = as 2 with 2-byte keys (d^2 buckets 2^-7 wide) and
  *                          the exact order restored inside the k-th bucket,
  *                      4 = as 3, leaf points loaded 4 at a time under a
- *                          5-waves-per-SIMD register budget (default);
+ *                          5-waves-per-SIMD register budget,
+ *                      5 = auto (default): 4 when the frame has at least 16
+ *                          8x8 tiles per compute unit and the maps hold at
+ *                          least 2^20 records, else 1;
  *                      every mode gives the same neighbour sets and images;
  * PM_OPT_BLOCKS_PER_CU workgroups per compute unit of the persistent mode-0 grid;
  * PM_OPT_CACHE_FUSED_INFER 2 = cache inference through the fused LDS-resident
@@ -243,6 +246,7 @@ typedef struct pm_batch_tuning {
   int leaf_size;      /* kd-tree leaf size L (1..64) of every map */
   int block_threads;  /* render workgroup size: 64, 128, 192 or 256 */
   int heap_slots;     /* LDS heap keys per lane: 32 or 64 (>= every scene's k) */
+  int heap_key_bytes; /* heap key width: 8 (d^2, index), 4 (d^2) or 2 (d^2 bucket); see PM_OPT_GATHER_MODE */
 } pm_batch_tuning;
 
 typedef struct pm_batch_stats {
@@ -268,10 +272,11 @@ int pm_batch_scene_config(uint32_t seed, int index, pm_config* out);
 int pm_batch_create(const pm_config* cfgs, int n_scenes, pm_batch** out);
 int pm_batch_destroy(pm_batch* b);
 int pm_batch_set_tuning(pm_batch* b, const pm_batch_tuning* t);
-/* Times the frame (emission, build, a 1-in-`sample_stride` subset of the
- * render work items) for every candidate of the grid leaf {4, 8, 16, 32} x
- * block {64, 128, 256} x heap slots {32 (when every k <= 32), 64}, keeps the
- * fastest and returns it (and its time). */
+/* Times the kd-build and a 1-in-`sample_stride` subset of the render work
+ * items for a staged sweep: heap key width {8, 4, 2} x heap slots {32 (when
+ * every k <= 32), 64}, then leaf size {4, 8, 16, 32}, then workgroup size
+ * {64, 128, 256}, each stage from the best so far; keeps the fastest and
+ * returns it (and its time). */
 int pm_batch_autotune(pm_batch* b, int sample_stride, pm_batch_tuning* best_out, double* best_ms_out);
 /* One frame of every scene (one graph launch); blocking. */
 int pm_batch_submit(pm_batch* b);
