@@ -99,7 +99,7 @@ def rel_rmse(a, b, exposure=None):
     return float(np.sqrt(((a.astype(np.float64) - b) ** 2).mean()) / np.abs(b).mean())
 
 
-def run_cache(g, cfg, steps, train_frames, stride, threshold, plain_img):
+def run_cache(g, cfg, steps, train_frames, stride, threshold, batch, plain_img):
     """Cache frames of the same workload: emit + build + cache render (record
     queries, train on 2^18 exact kNN targets, MLP inference, exact fallback),
     after `train_frames` frames of online training from a fresh network."""
@@ -107,7 +107,7 @@ def run_cache(g, cfg, steps, train_frames, stride, threshold, plain_img):
     npx = cfg.width * cfg.height
     rgb = pmamd.DeviceBuffer(npx * 12)
     g.cache_reset(1)
-    g.cache_configure(threshold=threshold, march_stride=stride)
+    g.cache_configure(threshold=threshold, batch=batch, march_stride=stride)
 
     def frame():
         g.emit()
@@ -134,7 +134,8 @@ def run_cache(g, cfg, steps, train_frames, stride, threshold, plain_img):
         "ms_per_frame": round((t1 - t0) / steps * 1e3, 3),
         "breakdown_ms": {"record": round(m[0], 3), "train": round(m[1], 3), "infer": round(m[2], 3),
                          "fallback": round(m[3], 3), "accumulate": round(m[4], 3), "render_total": round(m[5], 3)},
-        "march_stride": stride, "threshold": threshold, "train_frames_before_timing": train_frames,
+        "march_stride": stride, "threshold": threshold, "adam_batch": batch, "train_samples_per_frame": 1 << 18,
+        "train_frames_before_timing": train_frames,
         "cache_queries": int(cs.records), "fallback_queries": int(cs.fallbacks), "train_loss": round(cs.train_loss, 5),
         "rmse_vs_plain_same_samples": {"display_rel": round(rel_rmse(img, plain_img, cfg.exposure), 5),
                                        "linear_rel": round(rel_rmse(img, plain_img), 5)},
@@ -255,7 +256,8 @@ def main():
                     help="0: wave per query; lane per query with 1: 8-byte, 2: 4-byte, 3/4: 2-byte heap keys; 5: auto")
     ap.add_argument("--frame", type=str, default="", help="WxHxSPP override of the frame (profiling runs only)")
     ap.add_argument("--cache-steps", type=int, default=3, help="timed cache frames (0: skip the cache leg)")
-    ap.add_argument("--cache-train-frames", type=int, default=24)
+    ap.add_argument("--cache-train-frames", type=int, default=48)
+    ap.add_argument("--cache-batch", type=int, default=8192, help="Adam batch (2^18 training samples per frame)")
     ap.add_argument("--cache-stride", type=int, default=4)
     ap.add_argument("--cache-threshold", type=float, default=0.12)
     ap.add_argument("--batch-steps", type=int, default=3, help="timed batch frames (0: skip the batch leg)")
@@ -318,7 +320,7 @@ def main():
     }
     if args.cache_steps > 0:
         ca = run_cache(g, cfg, args.cache_steps, args.cache_train_frames, args.cache_stride, args.cache_threshold,
-                       img.reshape(-1, 3))
+                       args.cache_batch, img.reshape(-1, 3))
         ca["speedup_vs_plain_frame"] = round(out["ms_per_step"] / ca["ms_per_frame"], 3)
         out["cache"] = ca
     if args.batch_steps > 0:

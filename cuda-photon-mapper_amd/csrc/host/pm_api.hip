@@ -455,6 +455,10 @@ int pm_set_option(pm_context* c, int option, int value) {
       if (value < 1 || value > 64) return PM_ERR_INVALID;
       c->blocks_per_cu = value;
       return PM_OK;
+    case PM_OPT_CACHE_INFER_CHUNK:
+      if (value < 14 || value > 20) return PM_ERR_INVALID;
+      c->cache.infer_chunk_log2 = value;
+      return PM_OK;
     case PM_OPT_CACHE_FUSED_INFER:
       if (value < 0 || value > 2) return PM_ERR_INVALID;
       c->cache.fused_infer = value;

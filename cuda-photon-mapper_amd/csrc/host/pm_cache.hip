@@ -124,7 +124,8 @@ int cache_predict_dev(pm_context* c, const float* in, uint32_t n, float* rgb) {
     int maxlds = 0;
     HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
     if ((size_t)maxlds >= mlp_infer_lds_bytes()) {
-      HIPCHK(mlp_infer(k.w, in, n, k.H[0].as<uint16_t>(), rgb, c->stream, k.fused_infer == 2 ? 1 : 0));
+      HIPCHK(mlp_infer(k.w, in, n, k.H[0].as<uint16_t>(), rgb, c->stream, k.fused_infer == 2 ? 1 : 0,
+                       1u << k.infer_chunk_log2));
       return PM_OK;
     }
   }
@@ -301,6 +302,9 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   rl.counts = k.rec_cnt.as<uint32_t>();
   rl.write = 0;
   rl.stride = k.stride;
+  // near-light radius: 4 merged groups (none without merging)
+  const float near = k.stride > 1 ? 4.0f * (float)k.stride * c->rp.march_step : 0.0f;
+  rl.near2 = near * near;
   HIPCHK(launch_cache_record(rl, st));
   HIPCHK(exclusive_scan(k.rec_cnt.as<uint32_t>(), k.rec_off.as<uint32_t>(), npix, c->scratch.as<uint32_t>(), nullptr,
                         st));
@@ -377,8 +381,13 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   HIPCHK(k.flag_off.ensure(4ull * std::max<uint64_t>(N, 1)));
   HIPCHK(k.list.ensure(8ull * std::max<uint64_t>(N, 1)));
   HIPCHK(c->scratch.ensure(4ull * scan_temp_words(std::max<uint64_t>(N, npix))));
+  CacheFlagParams fp;
+  fp.threshold = k.threshold;
+  fp.cache_surface = k.cache_surface;
+  for (int j = 0; j < 3; ++j) fp.light[j] = c->scene.light[j];
+  fp.near2 = rl.near2;
   HIPCHK(launch_cache_flags(k.rec_in.as<float>(), N, k.err_sum.as<unsigned long long>(), k.err_cnt.as<unsigned int>(),
-                            k.threshold, k.cache_surface, k.flags.as<uint32_t>(), st));
+                            fp, k.flags.as<uint32_t>(), st));
   HIPCHK(c->small.ensure(64));
   HIPCHK(exclusive_scan(k.flags.as<uint32_t>(), k.flag_off.as<uint32_t>(), N, c->scratch.as<uint32_t>(),
                         c->small.as<uint32_t>() + 8, st));

@@ -25,9 +25,15 @@ namespace pm {
 
 // Recording gather: stores each query and its weight, returns zero radiance.
 // With march stride s > 1, one volume record stands for up to s consecutive
-// march points of the same segment (the first point's position, the sum of
-// their weights); a point farther than 1.5 s steps from the open record, or a
-// surface query, or a point on a ray of another direction, starts a new one.
+// march points of the same segment: its weight is the sum of theirs and its
+// position their weight-weighted centroid (a midpoint rule along the ray,
+// second order in the step, where the first point alone would be first
+// order).  A point farther than 1.5 s steps from the open record's first
+// point, a surface query, or a point on a ray of another direction starts a
+// new record.  Within the near-light radius (4 s steps) of the point light,
+// where the in-scattered radiance grows as 1 / distance^2 and no low-order
+// rule over s steps holds, every march point keeps its own record (and is
+// gathered exactly, see k_flags).
 struct RecordGather {
   bool write;
   uint32_t n;        // records of this lane so far
@@ -40,6 +46,10 @@ struct RecordGather {
   float reach2;      // (1.5 * stride * step)^2
   int open;          // points merged into the open volume record (0: none)
   V3f open_x, open_d;
+  float open_sw;     // sum of the merged points' scalar weights (w.x + w.y + w.z)
+  V3f open_swx;      // sum of scalar weight x position
+  V3f light;
+  float near2;       // (4 s steps)^2: no merging closer than this to the light
   __device__ void put(V3f x, V3f d, V3f nrm, V3f w) {
     if (write) {
       size_t i = (size_t)base + n;
@@ -58,23 +68,37 @@ struct RecordGather {
     return v3(0.0f, 0.0f, 0.0f);
   }
   __device__ V3f volume(V3f x, V3f wo, V3f w) {
-    if (open > 0 && open < stride && wo.x == open_d.x && wo.y == open_d.y && wo.z == open_d.z) {
+    const float lx = x.x - light.x, ly = x.y - light.y, lz = x.z - light.z;
+    const bool near = (lx * lx + ly * ly) + lz * lz < near2;
+    if (!near && open > 0 && open < stride && wo.x == open_d.x && wo.y == open_d.y && wo.z == open_d.z) {
       float dx = x.x - open_x.x, dy = x.y - open_x.y, dz = x.z - open_x.z;
       if ((dx * dx + dy * dy) + dz * dz <= reach2) {
+        const float sw = (w.x + w.y) + w.z;
+        open_sw = open_sw + sw;
+        open_swx = open_swx + x * sw;
         if (write) {
           size_t i = (size_t)base + n - 1u;
           wt[3 * i] += w.x;
           wt[3 * i + 1] += w.y;
           wt[3 * i + 2] += w.z;
+          if (open_sw > 0.0f) {
+            const float inv = 1.0f / open_sw;
+            float* p = in + i * MLP_IN;
+            p[0] = open_swx.x * inv;
+            p[1] = open_swx.y * inv;
+            p[2] = open_swx.z * inv;
+          }
         }
         ++open;
         return v3(0.0f, 0.0f, 0.0f);
       }
     }
     put(x, wo, v3(0.0f, 0.0f, 0.0f), w);
-    open = 1;
+    open = near ? 0 : 1;
     open_x = x;
     open_d = wo;
+    open_sw = (w.x + w.y) + w.z;
+    open_swx = x * open_sw;
     return v3(0.0f, 0.0f, 0.0f);
   }
 };
@@ -100,6 +124,8 @@ __global__ void __launch_bounds__(64) k_rec(CacheRecordLaunch r) {
   const float reach = 1.5f * (float)g.stride * r.rp.march_step;
   g.reach2 = reach * reach;
   g.open = 0;
+  g.light = v3(r.sc.light[0], r.sc.light[1], r.sc.light[2]);
+  g.near2 = r.near2;
   const uint32_t px = (uint32_t)(r.x0 + lx), py = (uint32_t)(r.y0 + ly);
   for (int s = 0; s < r.rp.spp; ++s) {
     if (FP64T) (void)render_sample_t<float, double>(r.sc, r.rp, px, py, (uint32_t)s, g);
@@ -186,18 +212,23 @@ __global__ void k_err_accum(const float* __restrict__ in, const float* __restric
   atomicAdd(&ecnt[cell], 1u);
 }
 
+// Fallback decision per record: surface queries (unless cached), volume
+// queries within the near-light radius (recorded one per march point, where
+// the radiance is steepest), and records in cells whose training error is
+// above the threshold or that saw no training row.
 __global__ void k_flags(const float* __restrict__ in, uint64_t n, const unsigned long long* __restrict__ esum,
-                        const unsigned int* __restrict__ ecnt, float threshold, int cache_surface,
-                        uint32_t* __restrict__ flags) {
+                        const unsigned int* __restrict__ ecnt, CacheFlagParams fp, uint32_t* __restrict__ flags) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float* p = in + i * MLP_IN;
   const bool surf = !(p[6] == 0.0f && p[7] == 0.0f && p[8] == 0.0f);
+  const float lx = p[0] - fp.light[0], ly = p[1] - fp.light[1], lz = p[2] - fp.light[2];
+  const bool near = !surf && (lx * lx + ly * ly) + lz * lz < fp.near2;
   uint32_t cell = err_cell(p);
   uint32_t c = ecnt[cell];
   // mean over the cell's training rows of the per-channel squared log error
   const double mse = (double)esum[cell] / 16777216.0 / ((double)c * (double)MLP_OUT);
-  bool fb = (surf && !cache_surface) || c == 0u || mse > (double)threshold;
+  bool fb = (surf && !fp.cache_surface) || near || c == 0u || mse > (double)fp.threshold;
   flags[i] = fb ? 1u : 0u;
 }
 
@@ -290,10 +321,9 @@ hipError_t launch_sum_rows(const float* v, uint32_t n, double* acc, hipStream_t 
 }
 
 hipError_t launch_cache_flags(const float* in, uint64_t n, const unsigned long long* esum, const unsigned int* ecnt,
-                              float threshold, int cache_surface, uint32_t* flags, hipStream_t st) {
+                              const CacheFlagParams& fp, uint32_t* flags, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, esum, ecnt, threshold,
-                     cache_surface, flags);
+  hipLaunchKernelGGL(k_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, esum, ecnt, fp, flags);
   return hipGetLastError();
 }
 

@@ -185,6 +185,13 @@ struct CacheRecordLaunch {
   float* wt;                 // [N x 3]
   uint32_t* pix;             // [N]
   int stride;                // march points per volume record
+  float near2;               // squared near-light radius: no merging inside
+};
+struct CacheFlagParams {
+  float threshold;           // per-cell mean squared log error
+  int cache_surface;         // 0: surface records always fall back
+  float light[3];
+  float near2;               // volume records this close to the light fall back
 };
 struct CacheValueLaunch {
   GatherMap gs, gv;
@@ -204,7 +211,7 @@ hipError_t launch_err_accum(const float* in, const float* row_err, uint32_t rows
                             unsigned int* ecnt, hipStream_t st);
 hipError_t launch_sum_rows(const float* v, uint32_t n, double* acc, hipStream_t st);
 hipError_t launch_cache_flags(const float* in, uint64_t n, const unsigned long long* esum, const unsigned int* ecnt,
-                              float threshold, int cache_surface, uint32_t* flags, hipStream_t st);
+                              const CacheFlagParams& fp, uint32_t* flags, hipStream_t st);
 hipError_t launch_cache_compact(const uint32_t* flags, const uint32_t* off, uint64_t n, uint32_t* list,
                                 hipStream_t st);
 hipError_t launch_cache_accumulate(const uint32_t* off, const uint32_t* cnt, uint32_t npix, const float* wt,

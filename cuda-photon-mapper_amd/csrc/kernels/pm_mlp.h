@@ -67,8 +67,9 @@ hipError_t mlp_refresh_bf16(MlpDev& w, hipStream_t st);
 // Fused inference (pm_mlp_infer.hip): h1 holds mlp_infer_h1_rows() x 128 BF16.
 size_t mlp_infer_h1_rows();
 size_t mlp_infer_lds_bytes();
-// k32 = 1: the v_mfma_f32_16x16x32_bf16 kernels, 0: the 16x16x16 ones.
+// k32 = 1: the v_mfma_f32_16x16x32_bf16 kernels, 0: the 16x16x16 ones;
+// rows per launch pair (layer 1, layers 2..5) = min(chunk_rows, mlp_infer_h1_rows()).
 hipError_t mlp_infer(const MlpDev& w, const float* in, uint32_t n, uint16_t* h1, float* rgb, hipStream_t st,
-                     int k32);
+                     int k32, uint32_t chunk_rows);
 
 }  // namespace pm

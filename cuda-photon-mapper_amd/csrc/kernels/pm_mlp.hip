@@ -83,22 +83,63 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   for (int k0 = k_begin; k0 < k_end; k0 += GK) {
-    // stage A (64 x 32) and B (32 x 64): 2048 elements each, 8 per thread
-    for (int e = tid; e < GT * GK; e += 256) {
-      int mm, kk;
-      if (g.transA) { mm = e % GT; kk = e / GT; } else { kk = e % GK; mm = e / GK; }
-      int gm = m0 + mm, gk = k0 + kk;
-      uint16_t v = 0;
-      if (gm < g.M && gk < k_end) v = g.transA ? g.A[(size_t)gk * g.lda + gm] : g.A[(size_t)gm * g.lda + gk];
-      As[mm][kk] = v;
+    // stage A (64 x 32) and B (32 x 64): 2048 elements each = one run of 8
+    // elements along the operand's contiguous dimension per thread, loaded
+    // with one 16-byte load when the run is in range and aligned (the
+    // training shapes), element by element otherwise
+    {
+      const int run = tid;  // 256 runs of 8
+      int mm, kk;           // first element of the run
+      if (g.transA) { mm = (run % (GT / 8)) * 8; kk = run / (GT / 8); } else { kk = (run % (GK / 8)) * 8; mm = run / (GK / 8); }
+      const int gm = m0 + mm, gk = k0 + kk;
+      uint16_t v[8];
+      const size_t base = g.transA ? (size_t)gk * g.lda + gm : (size_t)gm * g.lda + gk;
+      const bool full = g.transA ? (gk < k_end && gm + 7 < g.M) : (gm < g.M && gk + 7 < k_end);
+      if (full && ((uintptr_t)(g.A + base) & 15u) == 0) {
+        uint4 q = *(const uint4*)&g.A[base];
+        v[0] = (uint16_t)(q.x & 0xffffu); v[1] = (uint16_t)(q.x >> 16);
+        v[2] = (uint16_t)(q.y & 0xffffu); v[3] = (uint16_t)(q.y >> 16);
+        v[4] = (uint16_t)(q.z & 0xffffu); v[5] = (uint16_t)(q.z >> 16);
+        v[6] = (uint16_t)(q.w & 0xffffu); v[7] = (uint16_t)(q.w >> 16);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int em = g.transA ? gm + j : gm, ek = g.transA ? gk : gk + j;
+          v[j] = (em < g.M && ek < k_end) ? g.A[g.transA ? (size_t)ek * g.lda + em : (size_t)em * g.lda + ek] : 0;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (g.transA) As[mm + j][kk] = v[j];
+        else As[mm][kk + j] = v[j];
+      }
     }
-    for (int e = tid; e < GK * GT; e += 256) {
+    {
+      const int run = tid;
       int kk, nn;
-      if (g.transB) { kk = e % GK; nn = e / GK; } else { nn = e % GT; kk = e / GT; }
-      int gk = k0 + kk, gn = n0 + nn;
-      uint16_t v = 0;
-      if (gn < g.N && gk < k_end) v = g.transB ? g.B[(size_t)gn * g.ldb + gk] : g.B[(size_t)gk * g.ldb + gn];
-      Bs[kk][nn] = v;
+      if (g.transB) { kk = (run % (GK / 8)) * 8; nn = run / (GK / 8); } else { nn = (run % (GT / 8)) * 8; kk = run / (GT / 8); }
+      const int gk = k0 + kk, gn = n0 + nn;
+      uint16_t v[8];
+      const size_t base = g.transB ? (size_t)gn * g.ldb + gk : (size_t)gk * g.ldb + gn;
+      const bool full = g.transB ? (gn < g.N && gk + 7 < k_end) : (gk < k_end && gn + 7 < g.N);
+      if (full && ((uintptr_t)(g.B + base) & 15u) == 0) {
+        uint4 q = *(const uint4*)&g.B[base];
+        v[0] = (uint16_t)(q.x & 0xffffu); v[1] = (uint16_t)(q.x >> 16);
+        v[2] = (uint16_t)(q.y & 0xffffu); v[3] = (uint16_t)(q.y >> 16);
+        v[4] = (uint16_t)(q.z & 0xffffu); v[5] = (uint16_t)(q.z >> 16);
+        v[6] = (uint16_t)(q.w & 0xffffu); v[7] = (uint16_t)(q.w >> 16);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int ek = g.transB ? gk + j : gk, en = g.transB ? gn : gn + j;
+          v[j] = (en < g.N && ek < k_end) ? g.B[g.transB ? (size_t)en * g.ldb + ek : (size_t)ek * g.ldb + en] : 0;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (g.transB) Bs[kk + j][nn] = v[j];
+        else Bs[kk][nn + j] = v[j];
+      }
     }
     __syncthreads();
 #pragma unroll

@@ -420,14 +420,14 @@ size_t mlp_infer_lds_bytes() {
 
 // Inference for n rows: h1 must hold mlp_infer_h1_rows() x 128 BF16.
 hipError_t mlp_infer(const MlpDev& w, const float* in, uint32_t n, uint16_t* h1, float* rgb, hipStream_t st,
-                     int k32) {
+                     int k32, uint32_t chunk_rows) {
   int dev = 0, cus = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e != hipSuccess) return e;
   if (cus <= 0) cus = 1;
   const size_t lds25 = mlp_infer_lds_bytes();
-  const uint32_t chunk = (uint32_t)mlp_infer_h1_rows();
+  const uint32_t chunk = std::max<uint32_t>(1024u, std::min<uint32_t>(chunk_rows, (uint32_t)mlp_infer_h1_rows()));
   for (uint32_t o = 0; o < n; o += chunk) {
     const uint32_t rows = std::min(chunk, n - o);
     const uint32_t tiles = (rows + ROWS_PER_WAVE - 1) / ROWS_PER_WAVE;

@@ -175,8 +175,11 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
  * PM_OPT_BLOCKS_PER_CU workgroups per compute unit of the persistent mode-0 grid;
  * PM_OPT_CACHE_FUSED_INFER 2 = cache inference through the fused LDS-resident
  *                      kernels with 16x16x32 BF16 MFMA (default), 1 = the same
- *                      with 16x16x16 BF16 MFMA, 0 = through the generic GEMMs. */
-enum { PM_OPT_GATHER_MODE = 1, PM_OPT_BLOCKS_PER_CU = 2, PM_OPT_CACHE_FUSED_INFER = 3 };
+ *                      with 16x16x16 BF16 MFMA, 0 = through the generic GEMMs;
+ * PM_OPT_CACHE_INFER_CHUNK log2 of the rows per fused-inference launch pair,
+ *                      14..20 (default 19): smaller chunks keep the layer-1
+ *                      activations in the on-die caches between the two launches. */
+enum { PM_OPT_GATHER_MODE = 1, PM_OPT_BLOCKS_PER_CU = 2, PM_OPT_CACHE_FUSED_INFER = 3, PM_OPT_CACHE_INFER_CHUNK = 4 };
 int pm_set_option(pm_context* ctx, int option, int value);
 
 /* ------------------------------------------------------------------ learned radiance cache
@@ -205,9 +208,11 @@ int pm_cache_predict(pm_context* ctx, const float* inputs_dev, uint32_t n, float
 /* Cache render settings: fallback threshold on the mean (over a 16^3 cell's
  * training rows of this frame, per query kind, per colour channel) squared
  * error of the predicted log(1 + radiance), training samples per frame,
- * Adam batch and learning rate, march points per cached volume query (the
- * first point's prediction stands for up to `march_stride` consecutive points,
- * weighted by their summed path weights; 1 = every point), and whether
+ * Adam batch and learning rate, march points per cached volume query (one
+ * query, placed at the weight-weighted centroid of up to `march_stride`
+ * consecutive points of a ray segment, stands for all of them with their
+ * summed path weights; 1 = every point; within 4 x march_stride steps of the
+ * light every point is its own query and is gathered exactly), and whether
  * surface queries use the cache (0: always gathered exactly).
  * Defaults 0.05, 2^18, 2^14, 1e-3, 1, 0. */
 int pm_cache_configure(pm_context* ctx, float threshold, uint32_t train_samples, uint32_t batch, float lr,
