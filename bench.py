@@ -253,7 +253,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--gather-mode", type=int, default=5,
-                    help="0: wave per query; lane per query with 1: 8-byte, 2: 4-byte, 3/4: 2-byte heap keys; 5: auto")
+                    help="0: wave per query; lane per query with 1: 8-byte, 2: 4-byte, 3/4/6: 2-byte heap keys; 5: auto")
     ap.add_argument("--frame", type=str, default="", help="WxHxSPP override of the frame (profiling runs only)")
     ap.add_argument("--cache-steps", type=int, default=3, help="timed cache frames (0: skip the cache leg)")
     ap.add_argument("--cache-train-frames", type=int, default=48)
@@ -290,7 +290,7 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": (tr.get("bytes_per_launch") if tr else None),
-            "kernel": "k_render_lane4 (eye ray + march + kNN gather)", "bytes_per_query": bpq,
+            "kernel": "k_render_lane6 (eye ray + march + kNN gather)", "bytes_per_query": bpq,
             "queries_per_launch": int(q_frame), "kernel_ms": round(kern_ms, 3)}
     photons_traced = stats.n_emit_total
     emit_ms = float(np.mean([p[0] for p in per]))

@@ -186,6 +186,23 @@ PM_HD void camera_ray(const RenderParams& rp, uint32_t px, uint32_t py, uint32_t
   *o = v3<T>(T(rp.eye[0]), T(rp.eye[1]), T(rp.eye[2]));
 }
 
+// A gather type may answer volume queries one march step late (to merge the
+// exact second pass of step j with the first pass of step j + 1): it then
+// declares `static constexpr bool kDeferredVolume = true` and provides
+//   bool volume_defer(x, wo, w, c) -- queue this point; c = w * value of an
+//                                     earlier point when one is ready
+//   bool volume_flush(c)           -- the last queued point's w * value
+// The march adds the weighted values in step order either way, so the sum
+// is the same as with the immediate g.volume.
+template <class G, class = void>
+struct DeferredVolume {
+  static constexpr bool value = false;
+};
+template <class G>
+struct DeferredVolume<G, decltype((void)G::kDeferredVolume, void())> {
+  static constexpr bool value = G::kDeferredVolume;
+};
+
 // Ray-march of the in-scattered radiance over [t0, t1) along o + t d:
 // samples at t_j = t0 + (j + xi) * step, each weighted by transmittance and
 // the step length.  Homogeneous: T = exp(-sigma_t * (t - t0)).  Density field:
@@ -211,8 +228,17 @@ PM_HD void march_segment(const Scene& sc, const RenderParams& rp, V3<T> o, V3<T>
       tau = tau + (TT)sigma_t_at(m, x) * (TT)ds;
     }
     V3<T> w = thr * (Tj * ds);
-    V3<T> S = g.volume(x, wo, w);
-    nsum3_add(L, mul(w, S));
+    if constexpr (DeferredVolume<G>::value) {
+      V3<T> c;
+      if (g.volume_defer(x, wo, w, c)) nsum3_add(L, c);
+    } else {
+      V3<T> S = g.volume(x, wo, w);
+      nsum3_add(L, mul(w, S));
+    }
+  }
+  if constexpr (DeferredVolume<G>::value) {
+    V3<T> c;
+    if (g.volume_flush(c)) nsum3_add(L, c);
   }
   T Tseg;
   if (m.type == MED_HOMOGENEOUS) Tseg = (T)exp_(-st * (TT)(t1 - t0));

@@ -19,6 +19,7 @@
 #include "pm_lane_knn.h"
 #include "pm_lane_knn2.h"
 #include "pm_lane_knn3.h"
+#include "pm_lane_knn4.h"
 
 namespace pm {
 
@@ -97,6 +98,35 @@ __device__ __forceinline__ V3f lane_pixel3(const Scene& sc, const RenderParams& 
   g.nsurf = 0u;
   g.nvol = 0u;
   g.prev_r = -1.0f;
+  NSum3<float> acc;
+  nsum3_init(acc);
+  for (int s = 0; s < rp.spp; ++s) {
+    V3f v = FP64T ? render_sample_t<float, double>(sc, rp, px, py, (uint32_t)s, g)
+                  : render_sample_t<float, float>(sc, rp, px, py, (uint32_t)s, g);
+    nsum3_add(acc, v);
+  }
+  nsurf = g.nsurf;
+  nvol = g.nvol;
+  return nsum3_get(acc) * (1.0f / (float)rp.spp);
+}
+
+// The same with the deferred-volume gather (pm_lane_knn4.h).
+template <bool FP64T, bool CONE>
+__device__ __forceinline__ V3f lane_pixel4(const Scene& sc, const RenderParams& rp, const GatherMap& gs,
+                                           const GatherMap& gv, uint16_t* heap, int lane, uint32_t px, uint32_t py,
+                                           uint32_t& nsurf, uint32_t& nvol) {
+  LaneGather4<CONE, 4> g;
+  g.gs = gs;
+  g.gv = gv;
+  g.hp.H = heap;
+  g.hp.lane = lane;
+  g.hp.k = 0;
+  g.hp.size = 0;
+  g.hg_g = sc.med.g;
+  g.nsurf = 0u;
+  g.nvol = 0u;
+  g.prev_r = -1.0f;
+  g.pend = false;
   NSum3<float> acc;
   nsum3_init(acc);
   for (int s = 0; s < rp.spp; ++s) {
@@ -229,6 +259,39 @@ hipError_t launch_render_lane4(const RenderLaunch& r, hipStream_t st) {
   else if (fp64) hipLaunchKernelGGL((k_render_lane4<true, false>), dim3(tiles), dim3(64), lds, st, r);
   else if (cone) hipLaunchKernelGGL((k_render_lane4<false, true>), dim3(tiles), dim3(64), lds, st, r);
   else hipLaunchKernelGGL((k_render_lane4<false, false>), dim3(tiles), dim3(64), lds, st, r);
+  return hipGetLastError();
+}
+
+// Gather mode 6: the deferred-volume gather (2-byte heap, one shared
+// traversal per march step), 5 waves per SIMD.
+template <bool FP64T, bool CONE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_render_lane6(RenderLaunch r) {
+  extern __shared__ uint16_t heap16_lds[];  // [64 slots][64 lanes]
+  const int lane = lane_id();
+  const int tiles_x = (r.w + 7) / 8;
+  const int tile = (int)blockIdx.x;
+  const int tx = tile % tiles_x, ty = tile / tiles_x;
+  const int lx = tx * 8 + (lane & 7), ly = ty * 8 + (lane >> 3);
+  if (lx >= r.w || ly >= r.h) return;
+  const uint32_t px = (uint32_t)(r.x0 + lx), py = (uint32_t)(r.y0 + ly);
+  uint32_t ns, nv;
+  V3f v = lane_pixel4<FP64T, CONE>(r.sc, r.rp, r.gs, r.gv, heap16_lds, lane, px, py, ns, nv);
+  store_pixel(r.rgb, r.u8, (size_t)ly * (size_t)r.w + (size_t)lx, v, r.exposure);
+  if (r.counters) {
+    atomicAdd(&r.counters[0], (unsigned long long)ns);
+    atomicAdd(&r.counters[1], (unsigned long long)nv);
+  }
+}
+
+hipError_t launch_render_lane6(const RenderLaunch& r, hipStream_t st) {
+  if (r.w <= 0 || r.h <= 0) return hipSuccess;
+  unsigned tiles = (unsigned)(((r.w + 7) / 8) * ((r.h + 7) / 8));
+  size_t lds = 64 * 64 * sizeof(uint16_t);
+  const bool fp64 = r.rp.fp64_transmittance != 0, cone = r.estimator == 1;
+  if (fp64 && cone) hipLaunchKernelGGL((k_render_lane6<true, true>), dim3(tiles), dim3(64), lds, st, r);
+  else if (fp64) hipLaunchKernelGGL((k_render_lane6<true, false>), dim3(tiles), dim3(64), lds, st, r);
+  else if (cone) hipLaunchKernelGGL((k_render_lane6<false, true>), dim3(tiles), dim3(64), lds, st, r);
+  else hipLaunchKernelGGL((k_render_lane6<false, false>), dim3(tiles), dim3(64), lds, st, r);
   return hipGetLastError();
 }
 

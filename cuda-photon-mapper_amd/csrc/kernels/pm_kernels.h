@@ -126,6 +126,7 @@ hipError_t launch_render_lane(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane2(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane3(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane4(const RenderLaunch& r, hipStream_t st);
+hipError_t launch_render_lane6(const RenderLaunch& r, hipStream_t st);
 
 // batched scenes (pm_trace.hip, pm_gather_lane.hip)
 // Emission: scene s owns the trace blocks [block0, block0 + nblocks), whose

@@ -168,9 +168,11 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
  *                          the exact order restored inside the k-th bucket,
  *                      4 = as 3, leaf points loaded 4 at a time under a
  *                          5-waves-per-SIMD register budget,
- *                      5 = auto (default): 4 when the frame has at least 16
+ *                      5 = auto (default): 6 when the frame has at least 16
  *                          8x8 tiles per compute unit and the maps hold at
  *                          least 2^20 records, else 1;
+ *                      6 = as 4, with each march step's second pass merged
+ *                          into the next step's first-pass traversal;
  *                      every mode gives the same neighbour sets and images;
  * PM_OPT_BLOCKS_PER_CU workgroups per compute unit of the persistent mode-0 grid;
  * PM_OPT_CACHE_FUSED_INFER 2 = cache inference through the fused LDS-resident
