@@ -65,10 +65,11 @@ def gather_bytes_per_query(k, bf16=False):
     return k * per + 36
 
 
-def run_gpu(cfg, steps, warmup, gather_mode):
+def run_gpu(cfg, steps, warmup, gather_mode, tile_regions=0):
     lib = pmamd.load()
     g = pmamd.PhotonMapper(cfg)
     g.set_option(pmamd.PM_OPT_GATHER_MODE, gather_mode)
+    g.set_option(pmamd.PM_OPT_TILE_REGIONS, tile_regions)
     npx = cfg.width * cfg.height
     rgb = pmamd.DeviceBuffer(npx * 12)
     u8 = pmamd.DeviceBuffer(npx * 4)
@@ -253,7 +254,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--gather-mode", type=int, default=5,
-                    help="0: wave per query; lane per query with 1: 8-byte, 2: 4-byte, 3/4/6: 2-byte heap keys; 5: auto")
+                    help="0: wave per query; lane per query with 1: 8-byte, 2: 4-byte, 3/4/6/7: 2-byte heap keys; 5: auto")
+    ap.add_argument("--tile-regions", type=int, default=0,
+                    help="0: raster tile order; 1, 2, 4, 8: per-XCD regions, that many across")
     ap.add_argument("--frame", type=str, default="", help="WxHxSPP override of the frame (profiling runs only)")
     ap.add_argument("--cache-steps", type=int, default=3, help="timed cache frames (0: skip the cache leg)")
     ap.add_argument("--cache-train-frames", type=int, default=48)
@@ -280,7 +283,7 @@ def main():
     if args.frame:
         w, h, spp = (int(v) for v in args.frame.lower().split("x"))
         cfg.width, cfg.height, cfg.spp = w, h, spp
-    sec_per_frame, per, stats, img, g = run_gpu(cfg, args.steps, args.warmup, args.gather_mode)
+    sec_per_frame, per, stats, img, g = run_gpu(cfg, args.steps, args.warmup, args.gather_mode, args.tile_regions)
     q_frame = stats.surface_queries + stats.volume_queries
     value = q_frame / sec_per_frame
     kern_ms = float(np.mean([p[3] for p in per]))
@@ -290,7 +293,7 @@ def main():
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": (tr.get("bytes_per_launch") if tr else None),
-            "kernel": "k_render_lane6 (eye ray + march + kNN gather)", "bytes_per_query": bpq,
+            "kernel": "k_render_lane7 (eye ray + march + kNN gather)", "bytes_per_query": bpq,
             "queries_per_launch": int(q_frame), "kernel_ms": round(kern_ms, 3)}
     photons_traced = stats.n_emit_total
     emit_ms = float(np.mean([p[0] for p in per]))

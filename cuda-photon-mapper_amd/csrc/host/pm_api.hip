@@ -407,12 +407,13 @@ int pm_render_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev, 
   r.u8 = (uchar4*)out_dev;
   r.blocks_per_cu = c->blocks_per_cu;
   r.gather_mode = c->gather_mode;
+  r.tile_regions = c->tile_regions;
   HIPCHK(c->counters.ensure(16));
   r.counters = c->counters.as<unsigned long long>();
   HIPCHK(hipMemsetAsync(c->counters.p, 0, 16, c->stream));
   HIPCHK(hipEventRecord(c->e0, c->stream));
   if (r.gather_mode == 5) {
-    // auto: the two-pass 2-byte-heap gather (with the march's passes merged)
+    // auto: the two-pass gather (march passes merged, first passes counted)
     // pays off when the frame keeps many more waves in flight than the 8-byte
     // heaps allow and the maps are large; small frames or maps run the
     // one-pass gather
@@ -420,7 +421,7 @@ int pm_render_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev, 
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
     const uint64_t tiles = (uint64_t)((w + 7) / 8) * (uint64_t)((h + 7) / 8);
     const uint64_t recs = (uint64_t)c->ms.m.n + (uint64_t)c->mv.m.n;
-    r.gather_mode = tiles >= 16ull * (uint64_t)cus && recs >= (1ull << 20) ? 6 : 1;
+    r.gather_mode = tiles >= 16ull * (uint64_t)cus && recs >= (1ull << 20) ? 7 : 1;
   }
   switch (r.gather_mode) {
     case 0: HIPCHK(launch_render(r, c->stream)); break;
@@ -428,6 +429,7 @@ int pm_render_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev, 
     case 3: HIPCHK(launch_render_lane3(r, c->stream)); break;
     case 4: HIPCHK(launch_render_lane4(r, c->stream)); break;
     case 6: HIPCHK(launch_render_lane6(r, c->stream)); break;
+    case 7: HIPCHK(launch_render_lane7(r, c->stream)); break;
     default: HIPCHK(launch_render_lane2(r, c->stream)); break;
   }
   HIPCHK(hipEventRecord(c->e1, c->stream));
@@ -450,7 +452,7 @@ int pm_set_option(pm_context* c, int option, int value) {
   if (!c) return PM_ERR_INVALID;
   switch (option) {
     case PM_OPT_GATHER_MODE:
-      if (value < 0 || value > 6) return PM_ERR_INVALID;
+      if (value < 0 || value > 7) return PM_ERR_INVALID;
       c->gather_mode = value;
       return PM_OK;
     case PM_OPT_BLOCKS_PER_CU:
@@ -464,6 +466,10 @@ int pm_set_option(pm_context* c, int option, int value) {
     case PM_OPT_CACHE_FUSED_INFER:
       if (value < 0 || value > 2) return PM_ERR_INVALID;
       c->cache.fused_infer = value;
+      return PM_OK;
+    case PM_OPT_TILE_REGIONS:
+      if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8) return PM_ERR_INVALID;
+      c->tile_regions = value;
       return PM_OK;
     default:
       return PM_ERR_INVALID;

@@ -20,6 +20,7 @@
 #include "pm_lane_knn2.h"
 #include "pm_lane_knn3.h"
 #include "pm_lane_knn4.h"
+#include "pm_lane_knn5.h"
 
 namespace pm {
 
@@ -110,12 +111,12 @@ __device__ __forceinline__ V3f lane_pixel3(const Scene& sc, const RenderParams& 
   return nsum3_get(acc) * (1.0f / (float)rp.spp);
 }
 
-// The same with the deferred-volume gather (pm_lane_knn4.h).
-template <bool FP64T, bool CONE>
+// The same with the deferred-volume gathers (pm_lane_knn4.h, pm_lane_knn5.h).
+template <bool FP64T, bool CONE, class G = LaneGather4<CONE, 4>>
 __device__ __forceinline__ V3f lane_pixel4(const Scene& sc, const RenderParams& rp, const GatherMap& gs,
                                            const GatherMap& gv, uint16_t* heap, int lane, uint32_t px, uint32_t py,
                                            uint32_t& nsurf, uint32_t& nvol) {
-  LaneGather4<CONE, 4> g;
+  G g;
   g.gs = gs;
   g.gv = gv;
   g.hp.H = heap;
@@ -155,16 +156,60 @@ __device__ __forceinline__ void store_pixel(float* rgb, uchar4* u8, size_t o, V3
   }
 }
 
+// Workgroups are dispatched to the 8 XCDs of the chip round-robin, and each
+// XCD has its own L2.  With tile_regions = rx (1, 2, 4 or 8) the crop's tiles
+// are cut into an rx x (8 / rx) grid of regions and workgroup b renders the
+// (b / 8)-th tile, in raster order, of region b % 8: each XCD then works on
+// one compact part of the frame, and its L2 holds the photons of that part's
+// view frustum.  tile_regions = 0 renders tile b in raster order.
+__host__ __device__ __forceinline__ void lane_regions(const RenderLaunch& r, int& tiles_x, int& tiles_y, int& rx,
+                                                      int& ry, int& rw, int& rh) {
+  tiles_x = (r.w + 7) / 8;
+  tiles_y = (r.h + 7) / 8;
+  rx = r.tile_regions;
+  ry = 8 / (rx > 0 ? rx : 1);
+  rw = (tiles_x + rx - 1) / (rx > 0 ? rx : 1);
+  rh = (tiles_y + ry - 1) / ry;
+}
+
+unsigned lane_grid(const RenderLaunch& r) {
+  int tiles_x, tiles_y, rx, ry, rw, rh;
+  lane_regions(r, tiles_x, tiles_y, rx, ry, rw, rh);
+  if (rx <= 0) return (unsigned)(tiles_x * tiles_y);
+  return 8u * (unsigned)(rw * rh);
+}
+
+// Pixel (lx, ly) of the crop for this lane; false when it lies outside.
+__device__ __forceinline__ bool lane_tile(const RenderLaunch& r, int lane, int& lx, int& ly) {
+  int tiles_x, tiles_y, rx, ry, rw, rh;
+  lane_regions(r, tiles_x, tiles_y, rx, ry, rw, rh);
+  const int b = (int)blockIdx.x;
+  int tx, ty;
+  if (rx <= 0) {
+    tx = b % tiles_x;
+    ty = b / tiles_x;
+  } else {
+    const int reg = b & 7, slot = b >> 3;
+    const int gx = reg % rx, gy = reg / rx;
+    // region bounds: the first (tiles mod rx) regions of a row are one wider
+    const int x0 = gx * (tiles_x / rx) + min(gx, tiles_x % rx), w = tiles_x / rx + (gx < tiles_x % rx ? 1 : 0);
+    const int y0 = gy * (tiles_y / ry) + min(gy, tiles_y % ry), h = tiles_y / ry + (gy < tiles_y % ry ? 1 : 0);
+    if (w <= 0 || slot >= w * h) return false;
+    tx = x0 + slot % w;
+    ty = y0 + slot / w;
+  }
+  lx = tx * 8 + (lane & 7);
+  ly = ty * 8 + (lane >> 3);
+  return lx < r.w && ly < r.h;
+}
+
 // One wave (one workgroup) per 8x8 tile of the crop; lane = pixel.
 template <bool FP64T, bool CONE>
 __global__ void __launch_bounds__(64) k_render_lane(RenderLaunch r) {
   extern __shared__ uint32_t heap_lds[];  // [2][64 slots][64 lanes]
   const int lane = lane_id();
-  const int tiles_x = (r.w + 7) / 8;
-  const int tile = (int)blockIdx.x;
-  const int tx = tile % tiles_x, ty = tile / tiles_x;
-  const int lx = tx * 8 + (lane & 7), ly = ty * 8 + (lane >> 3);
-  if (lx >= r.w || ly >= r.h) return;
+  int lx, ly;
+  if (!lane_tile(r, lane, lx, ly)) return;
   const uint32_t px = (uint32_t)(r.x0 + lx), py = (uint32_t)(r.y0 + ly);
   uint32_t ns, nv;
   V3f v = lane_pixel<FP64T, CONE>(r.sc, r.rp, r.gs, r.gv, heap_lds, 64, lane, px, py, ns, nv);
@@ -180,11 +225,8 @@ template <bool FP64T, bool CONE>
 __global__ void __launch_bounds__(64) k_render_lane2(RenderLaunch r) {
   extern __shared__ uint32_t heap_lds[];  // [64 slots][64 lanes]
   const int lane = lane_id();
-  const int tiles_x = (r.w + 7) / 8;
-  const int tile = (int)blockIdx.x;
-  const int tx = tile % tiles_x, ty = tile / tiles_x;
-  const int lx = tx * 8 + (lane & 7), ly = ty * 8 + (lane >> 3);
-  if (lx >= r.w || ly >= r.h) return;
+  int lx, ly;
+  if (!lane_tile(r, lane, lx, ly)) return;
   const uint32_t px = (uint32_t)(r.x0 + lx), py = (uint32_t)(r.y0 + ly);
   uint32_t ns, nv;
   V3f v = lane_pixel2<FP64T, CONE>(r.sc, r.rp, r.gs, r.gv, heap_lds, lane, px, py, ns, nv);
@@ -201,11 +243,8 @@ template <bool FP64T, bool CONE, int NB>
 __global__ void __launch_bounds__(64) k_render_lane3(RenderLaunch r) {
   extern __shared__ uint16_t heap16_lds[];  // [64 slots][64 lanes]
   const int lane = lane_id();
-  const int tiles_x = (r.w + 7) / 8;
-  const int tile = (int)blockIdx.x;
-  const int tx = tile % tiles_x, ty = tile / tiles_x;
-  const int lx = tx * 8 + (lane & 7), ly = ty * 8 + (lane >> 3);
-  if (lx >= r.w || ly >= r.h) return;
+  int lx, ly;
+  if (!lane_tile(r, lane, lx, ly)) return;
   const uint32_t px = (uint32_t)(r.x0 + lx), py = (uint32_t)(r.y0 + ly);
   uint32_t ns, nv;
   V3f v = lane_pixel3<FP64T, CONE, NB>(r.sc, r.rp, r.gs, r.gv, heap16_lds, lane, px, py, ns, nv);
@@ -221,11 +260,8 @@ template <bool FP64T, bool CONE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_render_lane4(RenderLaunch r) {
   extern __shared__ uint16_t heap16_lds[];  // [64 slots][64 lanes]
   const int lane = lane_id();
-  const int tiles_x = (r.w + 7) / 8;
-  const int tile = (int)blockIdx.x;
-  const int tx = tile % tiles_x, ty = tile / tiles_x;
-  const int lx = tx * 8 + (lane & 7), ly = ty * 8 + (lane >> 3);
-  if (lx >= r.w || ly >= r.h) return;
+  int lx, ly;
+  if (!lane_tile(r, lane, lx, ly)) return;
   const uint32_t px = (uint32_t)(r.x0 + lx), py = (uint32_t)(r.y0 + ly);
   uint32_t ns, nv;
   V3f v = lane_pixel3<FP64T, CONE, 4>(r.sc, r.rp, r.gs, r.gv, heap16_lds, lane, px, py, ns, nv);
@@ -239,7 +275,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
 template <int NB>
 hipError_t launch_render_lane3_nb(const RenderLaunch& r, hipStream_t st) {
   if (r.w <= 0 || r.h <= 0) return hipSuccess;
-  unsigned tiles = (unsigned)(((r.w + 7) / 8) * ((r.h + 7) / 8));
+  const unsigned tiles = lane_grid(r);
   size_t lds = 64 * 64 * sizeof(uint16_t);
   const bool fp64 = r.rp.fp64_transmittance != 0, cone = r.estimator == 1;
   if (fp64 && cone) hipLaunchKernelGGL((k_render_lane3<true, true, NB>), dim3(tiles), dim3(64), lds, st, r);
@@ -252,7 +288,7 @@ hipError_t launch_render_lane3_nb(const RenderLaunch& r, hipStream_t st) {
 hipError_t launch_render_lane3(const RenderLaunch& r, hipStream_t st) { return launch_render_lane3_nb<8>(r, st); }
 hipError_t launch_render_lane4(const RenderLaunch& r, hipStream_t st) {
   if (r.w <= 0 || r.h <= 0) return hipSuccess;
-  unsigned tiles = (unsigned)(((r.w + 7) / 8) * ((r.h + 7) / 8));
+  const unsigned tiles = lane_grid(r);
   size_t lds = 64 * 64 * sizeof(uint16_t);
   const bool fp64 = r.rp.fp64_transmittance != 0, cone = r.estimator == 1;
   if (fp64 && cone) hipLaunchKernelGGL((k_render_lane4<true, true>), dim3(tiles), dim3(64), lds, st, r);
@@ -268,11 +304,8 @@ template <bool FP64T, bool CONE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_render_lane6(RenderLaunch r) {
   extern __shared__ uint16_t heap16_lds[];  // [64 slots][64 lanes]
   const int lane = lane_id();
-  const int tiles_x = (r.w + 7) / 8;
-  const int tile = (int)blockIdx.x;
-  const int tx = tile % tiles_x, ty = tile / tiles_x;
-  const int lx = tx * 8 + (lane & 7), ly = ty * 8 + (lane >> 3);
-  if (lx >= r.w || ly >= r.h) return;
+  int lx, ly;
+  if (!lane_tile(r, lane, lx, ly)) return;
   const uint32_t px = (uint32_t)(r.x0 + lx), py = (uint32_t)(r.y0 + ly);
   uint32_t ns, nv;
   V3f v = lane_pixel4<FP64T, CONE>(r.sc, r.rp, r.gs, r.gv, heap16_lds, lane, px, py, ns, nv);
@@ -285,7 +318,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
 
 hipError_t launch_render_lane6(const RenderLaunch& r, hipStream_t st) {
   if (r.w <= 0 || r.h <= 0) return hipSuccess;
-  unsigned tiles = (unsigned)(((r.w + 7) / 8) * ((r.h + 7) / 8));
+  const unsigned tiles = lane_grid(r);
   size_t lds = 64 * 64 * sizeof(uint16_t);
   const bool fp64 = r.rp.fp64_transmittance != 0, cone = r.estimator == 1;
   if (fp64 && cone) hipLaunchKernelGGL((k_render_lane6<true, true>), dim3(tiles), dim3(64), lds, st, r);
@@ -295,9 +328,39 @@ hipError_t launch_render_lane6(const RenderLaunch& r, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Gather mode 7: mode 6 with the march's first passes done by counting in a
+// bucket window (pm_lane_knn5.h), 5 waves per SIMD.
+template <bool FP64T, bool CONE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_render_lane7(RenderLaunch r) {
+  extern __shared__ uint16_t heap16_lds[];  // [64 slots][64 lanes]
+  const int lane = lane_id();
+  int lx, ly;
+  if (!lane_tile(r, lane, lx, ly)) return;
+  const uint32_t px = (uint32_t)(r.x0 + lx), py = (uint32_t)(r.y0 + ly);
+  uint32_t ns, nv;
+  V3f v = lane_pixel4<FP64T, CONE, LaneGather5<CONE>>(r.sc, r.rp, r.gs, r.gv, heap16_lds, lane, px, py, ns, nv);
+  store_pixel(r.rgb, r.u8, (size_t)ly * (size_t)r.w + (size_t)lx, v, r.exposure);
+  if (r.counters) {
+    atomicAdd(&r.counters[0], (unsigned long long)ns);
+    atomicAdd(&r.counters[1], (unsigned long long)nv);
+  }
+}
+
+hipError_t launch_render_lane7(const RenderLaunch& r, hipStream_t st) {
+  if (r.w <= 0 || r.h <= 0) return hipSuccess;
+  const unsigned tiles = lane_grid(r);
+  size_t lds = 64 * 64 * sizeof(uint16_t);
+  const bool fp64 = r.rp.fp64_transmittance != 0, cone = r.estimator == 1;
+  if (fp64 && cone) hipLaunchKernelGGL((k_render_lane7<true, true>), dim3(tiles), dim3(64), lds, st, r);
+  else if (fp64) hipLaunchKernelGGL((k_render_lane7<true, false>), dim3(tiles), dim3(64), lds, st, r);
+  else if (cone) hipLaunchKernelGGL((k_render_lane7<false, true>), dim3(tiles), dim3(64), lds, st, r);
+  else hipLaunchKernelGGL((k_render_lane7<false, false>), dim3(tiles), dim3(64), lds, st, r);
+  return hipGetLastError();
+}
+
 hipError_t launch_render_lane2(const RenderLaunch& r, hipStream_t st) {
   if (r.w <= 0 || r.h <= 0) return hipSuccess;
-  unsigned tiles = (unsigned)(((r.w + 7) / 8) * ((r.h + 7) / 8));
+  const unsigned tiles = lane_grid(r);
   size_t lds = 64 * 64 * sizeof(uint32_t);
   const bool fp64 = r.rp.fp64_transmittance != 0, cone = r.estimator == 1;
   if (fp64 && cone) hipLaunchKernelGGL((k_render_lane2<true, true>), dim3(tiles), dim3(64), lds, st, r);
@@ -309,7 +372,7 @@ hipError_t launch_render_lane2(const RenderLaunch& r, hipStream_t st) {
 
 hipError_t launch_render_lane(const RenderLaunch& r, hipStream_t st) {
   if (r.w <= 0 || r.h <= 0) return hipSuccess;
-  unsigned tiles = (unsigned)(((r.w + 7) / 8) * ((r.h + 7) / 8));
+  const unsigned tiles = lane_grid(r);
   size_t lds = 2 * 64 * 64 * sizeof(uint32_t);
   const bool fp64 = r.rp.fp64_transmittance != 0, cone = r.estimator == 1;
   if (fp64 && cone) hipLaunchKernelGGL((k_render_lane<true, true>), dim3(tiles), dim3(64), lds, st, r);

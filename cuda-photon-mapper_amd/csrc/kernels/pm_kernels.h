@@ -120,6 +120,7 @@ struct RenderLaunch {
   unsigned long long* counters;  // [surface queries, volume queries]
   int blocks_per_cu;             // persistent grid: workgroups per compute unit
   int gather_mode;               // 0: wave per query; lane per query: 1 8-byte, 2 4-byte, 3 2-byte heap
+  int tile_regions;              // lane modes: 0 raster tile order; rx = 1, 2, 4, 8: rx x (8 / rx) per-XCD regions
 };
 hipError_t launch_render(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane(const RenderLaunch& r, hipStream_t st);
@@ -127,6 +128,7 @@ hipError_t launch_render_lane2(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane3(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane4(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane6(const RenderLaunch& r, hipStream_t st);
+hipError_t launch_render_lane7(const RenderLaunch& r, hipStream_t st);
 
 // batched scenes (pm_trace.hip, pm_gather_lane.hip)
 // Emission: scene s owns the trace blocks [block0, block0 + nblocks), whose

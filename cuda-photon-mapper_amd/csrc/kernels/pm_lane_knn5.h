@@ -1,0 +1,199 @@
+// pm_lane_knn5.h -- the deferred-volume gather (pm_lane_knn4.h) with the
+// first pass of a march point done by counting instead of a heap.
+//
+// The first pass only has to find Q, the 16-bit bucket (FP32 bits of d^2
+// >> 15) of the k-th neighbour; the second pass then takes the exact
+// neighbour set from Q.  Along a march the k-th-neighbour radius changes little
+// from one step to the next, so the first pass of a point counts its
+// neighbours in a window of 128 consecutive buckets, [Q' - 64, Q' + 64) around
+// the previous point's bucket Q' (a factor 2^(1/4) in radius either way):
+//   * points in buckets below the window increment a register counter;
+//   * points in the window increment their bucket's 8-bit counter in LDS (128
+//     counters per lane, 8 KB per wave, in the LDS of the heap);
+//   * points above the window are outside the search bound, which is the top
+//     of the window (or the radius cap), so the traversal never visits them.
+// Q is the first bucket at which the running count reaches k.  When the window
+// reaches the cap and holds fewer than k points, the point has fewer than k
+// neighbours within the cap, and its count is known.  A point whose k-th
+// neighbour falls outside the window, or whose window holds more than 255
+// points (so that no 8-bit counter can wrap), runs the heap pass of
+// pm_lane_knn4.h instead.  Either way Q is the bucket that the heap pass finds,
+// so neighbour sets, r_k and estimates are those of the other gather modes.
+// Counting costs one LDS add per point in the window and no data-dependent
+// loop, where the heap's insertions run sift loops that the whole wave waits
+// through for the few lanes that insert.
+#pragma once
+
+#include "pm_lane_knn4.h"
+
+namespace pm {
+
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+// First pass by counting in the bucket window [K0, K0 + 128).
+struct HistVisitor {
+  static constexpr uint32_t kBuckets = 128;  // 8-bit counters: 32 dwords per lane, 8 KB per wave
+  lds_u32* H;        // this wave's counters: dword w of lane l at H[w * 64 + l], 4 buckets per dword
+  int lane;
+  uint32_t K0;       // first bucket of the window
+  uint32_t top_h;    // search bound (FP32 bits of d^2): at most the top of the window
+  uint32_t below;    // points under the bound in buckets below K0
+  uint32_t inwin;    // points in the window
+  __device__ uint32_t bound() const { return top_h; }
+  __device__ void clear() {
+#pragma unroll
+    for (uint32_t w = 0; w < kBuckets / 4; ++w) H[w * 64 + lane] = 0u;
+  }
+  template <int NB>
+  __device__ void points(const uint32_t* dh, int cnt) {
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const uint32_t d = dh[j];
+      if (j < cnt && d <= top_h) {
+        const uint32_t q = dq16(d);
+        if (q < K0) {
+          ++below;
+        } else {
+          const uint32_t b = q - K0;  // < kBuckets: d <= top_h <= top of bucket K0 + kBuckets - 1
+          __hip_atomic_fetch_add(&H[(b >> 2) * 64u + (uint32_t)lane], 1u << (8u * (b & 3u)), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_WAVEFRONT);
+          ++inwin;
+        }
+      }
+    }
+  }
+  // The bucket Q of the k-th neighbour under the bound; false when fewer than
+  // k points lie under it, when k points lie below the window, or when a
+  // counter may have wrapped (more than 255 points in the window).
+  __device__ bool select(uint32_t k, uint32_t& Q) const {
+    if (below >= k || below + inwin < k || inwin > 255u) return false;
+    const uint32_t need = k - below;
+    // the dword whose running count reaches `need`: with inwin <= 255 no sum
+    // of a dword's bytes carries, so v * 0x01010101 >> 24 is their sum
+    uint32_t cum = 0u, wq = 0u, vq = 0u, before = 0u;
+    bool found = false;
+#pragma unroll
+    for (uint32_t w = 0; w < kBuckets / 4; ++w) {
+      const uint32_t v = H[w * 64 + lane];
+      const uint32_t c = cum + ((v * 0x01010101u) >> 24);
+      if (!found && c >= need) {
+        found = true;
+        wq = w;
+        vq = v;
+        before = cum;
+      }
+      cum = c;
+    }
+    uint32_t s = 0u;
+    uint32_t run = before + (vq & 255u);
+    if (run < need) { s = 1u; run += (vq >> 8) & 255u; }
+    if (run < need) { s = 2u; run += (vq >> 16) & 255u; }
+    if (run < need) s = 3u;
+    Q = K0 + 4u * wq + s;
+    return found;
+  }
+};
+
+// First pass of the new point (counting) and second pass of the pending point
+// (sums) over one traversal.
+template <class Add>
+struct DualHistVisitor {
+  V3f qn, qp;  // new point, pending point
+  HistVisitor* hv;
+  Sum16Visitor<Add>* sv;
+  __device__ bool far(int a, float split) const {
+    const float dn = axis_of(qn, a) - split, dp = axis_of(qp, a) - split;
+    const bool same_side = (dn < 0.0f) == (dp < 0.0f);
+    return dn * dn <= u2f(hv->bound()) || !same_side || dp * dp <= u2f(sv->bound());
+  }
+  template <int NB>
+  __device__ void points(const float4* p, int cnt) {
+    uint32_t dn[NB], dp[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      dn[j] = f2u(dist2(qn.x, qn.y, qn.z, p[j].x, p[j].y, p[j].z));
+      dp[j] = f2u(dist2(qp.x, qp.y, qp.z, p[j].x, p[j].y, p[j].z));
+    }
+    hv->template points<NB>(dn, cnt);
+    sv->template points<NB>(p, dp, cnt);
+  }
+};
+
+template <bool CONE>
+struct LaneGather5 : LaneGather4<CONE, 4> {
+  using Base = LaneGather4<CONE, 4>;
+  using typename Base::VolAdd;
+  // The window reaches kWinUp buckets above the pending point's bucket Q
+  // (2^(kWinUp/256) in radius) and 128 - kWinUp below it.
+  static constexpr uint32_t kWinUp = 64;
+
+  // The heap pass of x: seeded with bh, then with the cap when fewer than k
+  // points fall under the seed (one traversal call site for both).
+  __device__ void heap_pass(V3f x, uint32_t bh, uint32_t caph, Knn16Visitor& kv) {
+#pragma nounroll
+    for (int t = 0; t < 2; ++t) {
+      this->hp.size = 0;
+      kv.bound_h = t == 0 ? bh : caph;
+      lane_visit<4>(this->gv, x.x, x.y, x.z, kv);
+      if (kv.full || bh == caph) break;
+    }
+  }
+
+  __device__ bool volume_defer(V3f x, V3f wo, V3f w, V3f& c) {
+    ++this->nvol;
+    const GatherMap& gv = this->gv;
+    const uint32_t caph = f2u(gv.rcap2);
+    uint32_t bh = caph;
+    if (this->prev_r >= 0.0f) {
+      float dx = x.x - this->prev_x.x, dy = x.y - this->prev_x.y, dz = x.z - this->prev_x.z;
+      float b = (this->prev_r + sqrt_((dx * dx + dy * dy) + dz * dz)) * 1.0001f;
+      float b2 = b * b;
+      if (b2 < gv.rcap2) bh = f2u(b2);
+    }
+    this->hp.k = gv.k;
+    this->hp.size = 0;
+    Knn16Visitor kv{&this->hp, bh, 0u, false};
+    const bool had = this->pend;
+    bool heap = true;
+    if (had) {
+      if (this->pend_m == 0) {
+        c = mul(this->pend_w, v3(0.0f, 0.0f, 0.0f));
+      } else {
+        VolAdd add{&gv, this->pend_wo, this->hg_g, {}};
+        nsum3_init(add.s);
+        Sum16Visitor<VolAdd> sv{this->pend_Q, caph, !this->pend_full, 0u, LKEY_INF, LKEY_INF, LKEY_INF, LKEY_INF,
+                                &add};
+        if (this->pend_full) {
+          // counting window below and around the pending point's bucket; the
+          // search bound is the top of the window (or the cap)
+          const uint32_t hiq = this->pend_Q + kWinUp;
+          const uint32_t K0 = hiq > HistVisitor::kBuckets ? hiq - HistVisitor::kBuckets : 0u;
+          const uint32_t top = dq16_top(K0 + HistVisitor::kBuckets - 1u);
+          HistVisitor hv{(lds_u32*)this->hp.H, this->hp.lane, K0, top < caph ? top : caph, 0u, 0u};
+          hv.clear();
+          DualHistVisitor<VolAdd> dv{x, this->pend_x, &hv, &sv};
+          lane_visit_dual<4>(gv, x.x, x.y, x.z, dv);
+          uint32_t Q;
+          if (hv.select((uint32_t)gv.k, Q)) {
+            heap = false;
+            kv.full = true;
+            kv.root = Q;
+            this->hp.size = gv.k;
+          } else if (hv.top_h == caph && hv.below + hv.inwin < (uint32_t)gv.k) {
+            // every point within the cap was counted: fewer than k neighbours
+            heap = false;
+            this->hp.size = (int)(hv.below + hv.inwin);
+          }
+        } else {
+          lane_visit<4>(gv, this->pend_x.x, this->pend_x.y, this->pend_x.z, sv);  // fewer than k within the cap
+        }
+        c = this->finish_pending(sv, add);
+      }
+    }
+    if (heap) this->heap_pass(x, bh, caph, kv);
+    this->queue(x, wo, w, kv);
+    return had;
+  }
+};
+
+}  // namespace pm

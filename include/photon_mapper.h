@@ -168,11 +168,14 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
  *                          the exact order restored inside the k-th bucket,
  *                      4 = as 3, leaf points loaded 4 at a time under a
  *                          5-waves-per-SIMD register budget,
- *                      5 = auto (default): 6 when the frame has at least 16
+ *                      5 = auto (default): 7 when the frame has at least 16
  *                          8x8 tiles per compute unit and the maps hold at
  *                          least 2^20 records, else 1;
  *                      6 = as 4, with each march step's second pass merged
  *                          into the next step's first-pass traversal;
+ *                      7 = as 6, with the march's first passes done by
+ *                          counting in a 64-bucket window around the previous
+ *                          step's k-th-neighbour bucket (heap pass on a miss);
  *                      every mode gives the same neighbour sets and images;
  * PM_OPT_BLOCKS_PER_CU workgroups per compute unit of the persistent mode-0 grid;
  * PM_OPT_CACHE_FUSED_INFER 2 = cache inference through the fused LDS-resident
@@ -180,8 +183,19 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
  *                      with 16x16x16 BF16 MFMA, 0 = through the generic GEMMs;
  * PM_OPT_CACHE_INFER_CHUNK log2 of the rows per fused-inference launch pair,
  *                      14..20 (default 19): smaller chunks keep the layer-1
- *                      activations in the on-die caches between the two launches. */
-enum { PM_OPT_GATHER_MODE = 1, PM_OPT_BLOCKS_PER_CU = 2, PM_OPT_CACHE_FUSED_INFER = 3, PM_OPT_CACHE_INFER_CHUNK = 4 };
+ *                      activations in the on-die caches between the two launches;
+ * PM_OPT_TILE_REGIONS  order in which the lane modes' workgroups take 8x8 tiles:
+ *                      0 = raster order (default), rx = 1, 2, 4 or 8 = the
+ *                      frame cut into rx x (8 / rx) regions, one per XCD
+ *                      (workgroups go to the 8 XCDs round-robin), so that each
+ *                      XCD's L2 serves one compact part of the frame. */
+enum {
+  PM_OPT_GATHER_MODE = 1,
+  PM_OPT_BLOCKS_PER_CU = 2,
+  PM_OPT_CACHE_FUSED_INFER = 3,
+  PM_OPT_CACHE_INFER_CHUNK = 4,
+  PM_OPT_TILE_REGIONS = 5
+};
 int pm_set_option(pm_context* ctx, int option, int value);
 
 /* ------------------------------------------------------------------ learned radiance cache
