@@ -94,13 +94,47 @@ struct HistVisitor {
   }
 };
 
+// Second pass with the contributions of a batch compacted per lane: a lane's
+// neighbours among the batch's points are gathered first, then the
+// contribution code runs max-over-lanes times instead of once per point
+// position that any lane takes (same points, same order).  For contributions
+// that do not use d^2 (the volume estimate).
+template <class Add>
+struct Sum16CVisitor : Sum16Visitor<Add> {
+  template <int NB>
+  __device__ void points(const float4* p, const uint32_t* dh, int cnt) {
+    uint32_t sel[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) sel[j] = 0u;
+    uint32_t nt = 0u;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      const uint32_t d = dh[j];
+      const uint32_t idx = f2u(p[j].w);
+      const bool in = j < cnt;
+      const bool take = in && (this->inclusive ? d <= this->cap_h : dq16(d) < this->Q);
+#pragma unroll
+      for (int t = 0; t <= j; ++t) sel[t] = take && nt == (uint32_t)t ? idx : sel[t];
+      nt += take ? 1u : 0u;
+      if (in && !take && !this->inclusive && dq16(d) == this->Q) this->keep(d, idx);
+    }
+    this->n_less += nt;
+    for (uint32_t t = 0; t < nt; ++t) {
+      uint32_t idx = sel[0];
+#pragma unroll
+      for (int s = 1; s < NB; ++s) idx = t == (uint32_t)s ? sel[s] : idx;
+      (*this->add)(idx, 0u);
+    }
+  }
+};
+
 // First pass of the new point (counting) and second pass of the pending point
 // (sums) over one traversal.
 template <class Add>
 struct DualHistVisitor {
   V3f qn, qp;  // new point, pending point
   HistVisitor* hv;
-  Sum16Visitor<Add>* sv;
+  Sum16CVisitor<Add>* sv;
   __device__ bool far(int a, float split) const {
     const float dn = axis_of(qn, a) - split, dp = axis_of(qp, a) - split;
     const bool same_side = (dn < 0.0f) == (dp < 0.0f);
@@ -161,8 +195,13 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
       } else {
         VolAdd add{&gv, this->pend_wo, this->hg_g, {}};
         nsum3_init(add.s);
-        Sum16Visitor<VolAdd> sv{this->pend_Q, caph, !this->pend_full, 0u, LKEY_INF, LKEY_INF, LKEY_INF, LKEY_INF,
-                                &add};
+        Sum16CVisitor<VolAdd> sv;
+        sv.Q = this->pend_Q;
+        sv.cap_h = caph;
+        sv.inclusive = !this->pend_full;
+        sv.n_less = 0u;
+        sv.d0 = sv.i0 = sv.d1 = sv.i1 = LKEY_INF;
+        sv.add = &add;
         if (this->pend_full) {
           // counting window below and around the pending point's bucket; the
           // search bound is the top of the window (or the cap)
