@@ -5,20 +5,21 @@
 // >> 15) of the k-th neighbour; the second pass then takes the exact
 // neighbour set from Q.  Along a march the k-th-neighbour radius changes little
 // from one step to the next, so the first pass of a point counts its
-// neighbours in a window of 128 consecutive buckets, [Q' - 64, Q' + 64) around
-// the previous point's bucket Q' (a factor 2^(1/4) in radius either way):
+// neighbours in a window of 256 consecutive buckets, [Q' - 160, Q' + 96) around
+// the previous point's bucket Q' (a factor 2^(5/8) down and 2^(3/8) up in
+// radius):
 //   * points in buckets below the window increment a register counter;
-//   * points in the window increment their bucket's 8-bit counter in LDS (128
+//   * points in the window increment their bucket's 4-bit counter in LDS (256
 //     counters per lane, 8 KB per wave, in the LDS of the heap);
 //   * points above the window are outside the search bound, which is the top
 //     of the window (or the radius cap), so the traversal never visits them.
 // Q is the first bucket at which the running count reaches k.  When the window
 // reaches the cap and holds fewer than k points, the point has fewer than k
 // neighbours within the cap, and its count is known.  A point whose k-th
-// neighbour falls outside the window, or whose window holds more than 255
-// points (so that no 8-bit counter can wrap), runs the heap pass of
-// pm_lane_knn4.h instead.  Either way Q is the bucket that the heap pass finds,
-// so neighbour sets, r_k and estimates are those of the other gather modes.
+// neighbour falls outside the window, or one of whose counters wrapped (more
+// than 15 points in a bucket), runs the heap pass of pm_lane_knn4.h instead.
+// Either way Q is the bucket that the heap pass finds, so neighbour sets, r_k
+// and estimates are those of the other gather modes.
 // Counting costs one LDS add per point in the window and no data-dependent
 // loop, where the heap's insertions run sift loops that the whole wave waits
 // through for the few lanes that insert.
@@ -30,10 +31,10 @@ namespace pm {
 
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 
-// First pass by counting in the bucket window [K0, K0 + 128).
+// First pass by counting in the bucket window [K0, K0 + 256).
 struct HistVisitor {
-  static constexpr uint32_t kBuckets = 128;  // 8-bit counters: 32 dwords per lane, 8 KB per wave
-  lds_u32* H;        // this wave's counters: dword w of lane l at H[w * 64 + l], 4 buckets per dword
+  static constexpr uint32_t kBuckets = 256;  // 4-bit counters: 32 dwords per lane, 8 KB per wave
+  lds_u32* H;        // this wave's counters: dword w of lane l at H[w * 64 + l], 8 buckets per dword
   int lane;
   uint32_t K0;       // first bucket of the window
   uint32_t top_h;    // search bound (FP32 bits of d^2): at most the top of the window
@@ -42,7 +43,7 @@ struct HistVisitor {
   __device__ uint32_t bound() const { return top_h; }
   __device__ void clear() {
 #pragma unroll
-    for (uint32_t w = 0; w < kBuckets / 4; ++w) H[w * 64 + lane] = 0u;
+    for (uint32_t w = 0; w < kBuckets / 8; ++w) H[w * 64 + lane] = 0u;
   }
   template <int NB>
   __device__ void points(const uint32_t* dh, int cnt) {
@@ -55,7 +56,7 @@ struct HistVisitor {
           ++below;
         } else {
           const uint32_t b = q - K0;  // < kBuckets: d <= top_h <= top of bucket K0 + kBuckets - 1
-          __hip_atomic_fetch_add(&H[(b >> 2) * 64u + (uint32_t)lane], 1u << (8u * (b & 3u)), __ATOMIC_RELAXED,
+          __hip_atomic_fetch_add(&H[(b >> 3) * 64u + (uint32_t)lane], 1u << (4u * (b & 7u)), __ATOMIC_RELAXED,
                                  __HIP_MEMORY_SCOPE_WAVEFRONT);
           ++inwin;
         }
@@ -64,18 +65,22 @@ struct HistVisitor {
   }
   // The bucket Q of the k-th neighbour under the bound; false when fewer than
   // k points lie under it, when k points lie below the window, or when a
-  // counter may have wrapped (more than 255 points in the window).
+  // counter wrapped.  A counter that passes 15 carries into the next one (or
+  // out of the dword), which lowers the sum of all counters by 15 (or 16) and
+  // can never raise it, so the counters are exact exactly when their sum is
+  // the register count `inwin`.
   __device__ bool select(uint32_t k, uint32_t& Q) const {
-    if (below >= k || below + inwin < k || inwin > 255u) return false;
+    if (below >= k || below + inwin < k) return false;
     const uint32_t need = k - below;
-    // the dword whose running count reaches `need`: with inwin <= 255 no sum
-    // of a dword's bytes carries, so v * 0x01010101 >> 24 is their sum
     uint32_t cum = 0u, wq = 0u, vq = 0u, before = 0u;
     bool found = false;
 #pragma unroll
-    for (uint32_t w = 0; w < kBuckets / 4; ++w) {
+    for (uint32_t w = 0; w < kBuckets / 8; ++w) {
       const uint32_t v = H[w * 64 + lane];
-      const uint32_t c = cum + ((v * 0x01010101u) >> 24);
+      // the 8 counters summed pairwise into 4 bytes (each <= 30), then the
+      // bytes (sum <= 120, so no byte carries)
+      const uint32_t b = (v & 0x0F0F0F0Fu) + ((v >> 4) & 0x0F0F0F0Fu);
+      const uint32_t c = cum + ((b * 0x01010101u) >> 24);
       if (!found && c >= need) {
         found = true;
         wq = w;
@@ -84,12 +89,15 @@ struct HistVisitor {
       }
       cum = c;
     }
-    uint32_t s = 0u;
-    uint32_t run = before + (vq & 255u);
-    if (run < need) { s = 1u; run += (vq >> 8) & 255u; }
-    if (run < need) { s = 2u; run += (vq >> 16) & 255u; }
-    if (run < need) s = 3u;
-    Q = K0 + 4u * wq + s;
+    if (cum != inwin) return false;
+    // first counter of dword wq at which the running count reaches `need`
+    uint32_t run = before, s = 0u;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      run += (vq >> (4 * i)) & 15u;
+      s += run < need ? 1u : 0u;
+    }
+    Q = K0 + 8u * wq + s;
     return found;
   }
 };
@@ -157,9 +165,9 @@ template <bool CONE>
 struct LaneGather5 : LaneGather4<CONE, 4> {
   using Base = LaneGather4<CONE, 4>;
   using typename Base::VolAdd;
-  // The window reaches kWinUp buckets above the pending point's bucket Q
-  // (2^(kWinUp/256) in radius) and 128 - kWinUp below it.
-  static constexpr uint32_t kWinUp = 64;
+  // The window reaches kWinUp buckets above the pending point's bucket Q'
+  // (a factor 2^(kWinUp/256) in radius) and 256 - kWinUp below it.
+  static constexpr uint32_t kWinUp = 96;
 
   // The heap pass of x: seeded with bh, then with the cap when fewer than k
   // points fall under the seed (one traversal call site for both).
