@@ -63,6 +63,8 @@ int alloc_map(MapBufs& mb, uint32_t n, int bf16, int L) {
   uint32_t nl = kd_num_leaves(n, (uint32_t)L);
   HIPCHK(mb.nodes.ensure(sizeof(KdNode) * std::max<uint32_t>(nl, 1)));
   HIPCHK(mb.nrange.ensure(sizeof(uint2) * std::max<uint32_t>(nl, 1)));
+  HIPCHK(mb.bmin.ensure(sizeof(float4) * std::max<uint32_t>(2 * nl, 1)));
+  HIPCHK(mb.bmax.ensure(sizeof(float4) * std::max<uint32_t>(2 * nl, 1)));
   DevMap& m = mb.m;
   memset(&m, 0, sizeof(m));
   m.n = n;
@@ -77,6 +79,8 @@ int alloc_map(MapBufs& mb, uint32_t n, int bf16, int L) {
   m.tpts = mb.tpts.as<float4>();
   m.nodes = mb.nodes.as<KdNode>();
   m.nrange = mb.nrange.as<uint2>();
+  m.bmin = mb.bmin.as<float4>();
+  m.bmax = mb.bmax.as<float4>();
   m.built = false;
   return PM_OK;
 }
@@ -225,6 +229,8 @@ GatherMap gather_map(const pm_context* c, const DevMap& m, int k, float rcap) {
   g.tpts = m.tpts;
   g.nodes = m.nodes;
   g.nrange = m.nrange;
+  g.bmin = m.bmin;
+  g.bmax = m.bmax;
   g.payload = m.payload;
   g.n = m.n;
   g.nl = m.nl;

@@ -342,7 +342,69 @@ hipError_t kd_build_impl(DevMap& m, uint32_t* temp, const uint32_t* host_lo, con
   std::vector<uint32_t> off(1, 0u);
   for (uint64_t f = 0, w = 1; f < ni; f = 2 * f + 1, w *= 2) off.push_back((uint32_t)std::min<uint64_t>(f + w, ni));
   if ((e = kd_build_forest(F, off.data(), (uint32_t)off.size() - 1u, temp, st)) != hipSuccess) return e;
+  if ((e = kd_node_boxes(m, st)) != hipSuccess) return e;
   m.built = true;
+  return hipSuccess;
+}
+
+// Leaf h = nl - 1 + i: the bounds of its points, found from its parent's
+// range and left count.
+__global__ void k_leaf_boxes(const float4* tpts, const KdNode* nodes, const uint2* nrange, uint32_t n, uint32_t nl,
+                             float4* bmin, float4* bmax) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nl) return;
+  const uint32_t h = nl - 1u + i;
+  uint32_t lo = 0u, hi = n;
+  if (h > 0u) {
+    const uint32_t p = (h - 1u) >> 1;
+    const uint2 r = nrange[p];
+    const uint32_t mid = r.x + kd_left_count(nodes[p].meta);
+    if (h == 2u * p + 1u) {
+      lo = r.x;
+      hi = mid;
+    } else {
+      lo = mid;
+      hi = r.y;
+    }
+  }
+  float4 mn = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), mx = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+  for (uint32_t j = lo; j < hi; ++j) {
+    const float4 q = tpts[j];
+    mn.x = fminf(mn.x, q.x); mn.y = fminf(mn.y, q.y); mn.z = fminf(mn.z, q.z);
+    mx.x = fmaxf(mx.x, q.x); mx.y = fmaxf(mx.y, q.y); mx.z = fmaxf(mx.z, q.z);
+  }
+  bmin[h] = mn;
+  bmax[h] = mx;
+}
+
+// Internal nodes [first, first + count) of one level: the union of their
+// children's boxes (the level below is done).
+__global__ void k_union_boxes(uint32_t first, uint32_t count, float4* bmin, float4* bmax) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t h = first + i, l = 2u * h + 1u, r = 2u * h + 2u;
+  const float4 a = bmin[l], b = bmin[r], c = bmax[l], d = bmax[r];
+  bmin[h] = make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), 0.0f);
+  bmax[h] = make_float4(fmaxf(c.x, d.x), fmaxf(c.y, d.y), fmaxf(c.z, d.z), 0.0f);
+}
+
+hipError_t kd_node_boxes(const DevMap& m, hipStream_t st) {
+  if (!m.bmin || !m.bmax || m.n == 0 || m.nl == 0) return hipSuccess;
+  const uint32_t nl = m.nl, ni = nl - 1u;
+  hipLaunchKernelGGL(k_leaf_boxes, dim3((nl + 255u) / 256u), dim3(256), 0, st, (const float4*)m.tpts,
+                     (const KdNode*)m.nodes, (const uint2*)m.nrange, m.n, nl, m.bmin, m.bmax);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || ni == 0) return e;
+  // internal levels from the deepest up: level d holds heap nodes [2^d - 1, 2^(d+1) - 2]
+  int D = 0;
+  while ((2ull << D) - 1ull < (uint64_t)ni) ++D;  // deepest level with an internal node
+  for (int d = D; d >= 0; --d) {
+    const uint32_t first = (1u << d) - 1u;
+    const uint32_t last = std::min<uint32_t>((2u << d) - 2u, ni - 1u);
+    const uint32_t count = last - first + 1u;
+    hipLaunchKernelGGL(k_union_boxes, dim3((count + 255u) / 256u), dim3(256), 0, st, first, count, m.bmin, m.bmax);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
   return hipSuccess;
 }
 

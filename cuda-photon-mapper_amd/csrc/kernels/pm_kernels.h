@@ -31,6 +31,8 @@ struct DevMap {
   float4* tpts;      // n, tree order
   KdNode* nodes;     // nl - 1
   uint2* nrange;     // nl - 1: point range [lo, hi) of each internal node
+  float4* bmin;      // 2 nl - 1 (internal nodes, then leaves): tight box of each node's points, or null
+  float4* bmax;
   bool built;
 };
 
@@ -39,6 +41,8 @@ struct GatherMap {
   const float4* tpts;
   const KdNode* nodes;
   const uint2* nrange;
+  const float4* bmin;  // tight node boxes (2 nl - 1), or null
+  const float4* bmax;
   const void* payload;
   uint32_t n, nl;
   int bf16;
@@ -104,6 +108,8 @@ size_t kdbuild_temp_words(uint32_t n, uint32_t L);
 // left count lc, and lb = left counts of the same level's earlier nodes.
 hipError_t kd_build_impl(DevMap& m, uint32_t* temp, const uint32_t* host_lo, const uint32_t* host_hi,
                          const uint32_t* host_lc, const uint32_t* host_lb, hipStream_t st);
+// Tight boxes of every node's points into m.bmin / m.bmax (when allocated).
+hipError_t kd_node_boxes(const DevMap& m, hipStream_t st);
 
 // gather (pm_gather.hip)
 hipError_t launch_knn(GatherMap g, const float* q, uint32_t nq, uint32_t* idx, float* d2, uint32_t* count,

@@ -136,6 +136,106 @@ struct Sum16CVisitor : Sum16Visitor<Add> {
   }
 };
 
+// Squared distance from q to a node's tight box, as a lower bound of the
+// squared distance to any of its points: with the operation order of dist2,
+// and FP32 rounding monotonic, no point in the box has a smaller computed d^2.
+__device__ __forceinline__ float box_d2(V3f q, float4 mn, float4 mx) {
+  const float bx = fmaxf(fmaxf(mn.x - q.x, q.x - mx.x), 0.0f);
+  const float by = fmaxf(fmaxf(mn.y - q.y, q.y - mx.y), 0.0f);
+  const float bz = fmaxf(fmaxf(mn.z - q.z, q.z - mx.z), 0.0f);
+  return (bx * bx + by * by) + bz * bz;
+}
+
+// lane_visit_dual of pm_lane_knn4.h for visitors whose far-child test does not
+// change during the traversal (fixed search bounds), with speculative
+// traversal (T. Aila and S. Laine, "Understanding the Efficiency of Ray
+// Traversal on GPUs", HPG 2009): a lane that reaches a leaf while other lanes
+// are still descending keeps that leaf for later and walks on toward its next
+// leaf, instead of waiting; the leaves are processed once every active lane
+// holds one.  Each lane visits the same leaves in the same order as without
+// speculation, because the far-child test does not depend on what the leaves
+// held.  When the map has node boxes, a far child and a leaf are entered only
+// if the visitor's box test (V::near_box) passes: the points skipped are all
+// outside every search bound.
+template <int NB, class V>
+__device__ __forceinline__ void lane_visit_dual_spec(const GatherMap& g, float qx, float qy, float qz, V& v) {
+  const uint32_t n = g.n, nl = g.nl;
+  if (n == 0) return;
+  uint32_t h = 0, lo = 0, hi = n, from = 0;
+  uint32_t plo = 0, phi = 0;  // the kept leaf's point range
+  bool down = true, done = false, kept = false;
+  for (;;) {
+    for (;;) {
+      if (!done && !kept && down && h + 1u >= nl) {
+        // keep this leaf (unless its box is out of reach) and walk on
+        if (!g.bmin || v.near_box(g.bmin[h], g.bmax[h])) {
+          plo = lo;
+          phi = hi;
+          kept = true;
+        }
+        from = h;
+        down = false;
+      }
+      if (ballot(!done && !kept) == 0ull) break;
+      const bool step = !done && !(down && h + 1u >= nl);
+      if (!step) continue;
+      if (down) {
+        KdNode nd = g.nodes[h];
+        int a = kd_axis(nd.meta);
+        float qa = a == 0 ? qx : (a == 1 ? qy : qz);
+        uint32_t mid = lo + kd_left_count(nd.meta);
+        if (qa - nd.split < 0.0f) {
+          h = 2u * h + 1u;
+          hi = mid;
+        } else {
+          h = 2u * h + 2u;
+          lo = mid;
+        }
+        continue;
+      }
+      if (from == 0u) {
+        done = true;
+        continue;
+      }
+      uint32_t p = (from - 1u) >> 1;
+      KdNode nd = g.nodes[p];
+      int a = kd_axis(nd.meta);
+      float qa = a == 0 ? qx : (a == 1 ? qy : qz);
+      uint32_t nearc = qa - nd.split < 0.0f ? 2u * p + 1u : 2u * p + 2u;
+      const uint32_t farc = nearc == 2u * p + 1u ? 2u * p + 2u : 2u * p + 1u;
+      if (from == nearc && v.far(a, nd.split) && (!g.bmin || v.near_box(g.bmin[farc], g.bmax[farc]))) {
+        uint2 r = g.nrange[p];
+        uint32_t mid = r.x + kd_left_count(nd.meta);
+        if (nearc == 2u * p + 1u) {
+          h = 2u * p + 2u;
+          lo = mid;
+          hi = r.y;
+        } else {
+          h = 2u * p + 1u;
+          lo = r.x;
+          hi = mid;
+        }
+        down = true;
+      } else {
+        from = p;
+      }
+    }
+    if (ballot(kept) == 0ull) break;
+    if (!kept) continue;
+    for (uint32_t b0 = plo; b0 < phi; b0 += (uint32_t)NB) {
+      float4 p[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        uint32_t i = b0 + (uint32_t)j;
+        p[j] = g.tpts[i < phi ? i : phi - 1u];
+      }
+      const uint32_t left = phi - b0;
+      v.template points<NB>(p, left < (uint32_t)NB ? (int)left : NB);
+    }
+    kept = false;
+  }
+}
+
 // First pass of the new point (counting) and second pass of the pending point
 // (sums) over one traversal.
 template <class Add>
@@ -147,6 +247,10 @@ struct DualHistVisitor {
     const float dn = axis_of(qn, a) - split, dp = axis_of(qp, a) - split;
     const bool same_side = (dn < 0.0f) == (dp < 0.0f);
     return dn * dn <= u2f(hv->bound()) || !same_side || dp * dp <= u2f(sv->bound());
+  }
+  // a node's box can hold points under either search bound
+  __device__ bool near_box(float4 mn, float4 mx) const {
+    return box_d2(qn, mn, mx) <= u2f(hv->bound()) || box_d2(qp, mn, mx) <= u2f(sv->bound());
   }
   template <int NB>
   __device__ void points(const float4* p, int cnt) {
@@ -219,7 +323,7 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
           HistVisitor hv{(lds_u32*)this->hp.H, this->hp.lane, K0, top < caph ? top : caph, 0u, 0u};
           hv.clear();
           DualHistVisitor<VolAdd> dv{x, this->pend_x, &hv, &sv};
-          lane_visit_dual<4>(gv, x.x, x.y, x.z, dv);
+          lane_visit_dual_spec<4>(gv, x.x, x.y, x.z, dv);
           uint32_t Q;
           if (hv.select((uint32_t)gv.k, Q)) {
             heap = false;
