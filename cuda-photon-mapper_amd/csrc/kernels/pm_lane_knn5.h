@@ -265,6 +265,27 @@ struct DualHistVisitor {
   }
 };
 
+// The pending point's second pass on its own (at the end of a march segment,
+// or for a point with fewer than k neighbours within the cap): its bound is
+// fixed, so it runs on the speculative traversal with node boxes.
+template <class Add>
+struct SumOnlyVisitor {
+  V3f q;
+  Sum16CVisitor<Add>* sv;
+  __device__ bool far(int a, float split) const {
+    const float d = axis_of(q, a) - split;
+    return d * d <= u2f(sv->bound());
+  }
+  __device__ bool near_box(float4 mn, float4 mx) const { return box_d2(q, mn, mx) <= u2f(sv->bound()); }
+  template <int NB>
+  __device__ void points(const float4* p, int cnt) {
+    uint32_t d[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) d[j] = f2u(dist2(q.x, q.y, q.z, p[j].x, p[j].y, p[j].z));
+    sv->template points<NB>(p, d, cnt);
+  }
+};
+
 template <bool CONE>
 struct LaneGather5 : LaneGather4<CONE, 4> {
   using Base = LaneGather4<CONE, 4>;
@@ -272,6 +293,57 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
   // The window reaches kWinUp buckets above the pending point's bucket Q'
   // (a factor 2^(kWinUp/256) in radius) and 256 - kWinUp below it.
   static constexpr uint32_t kWinUp = 96;
+  // The counting window around bucket Qc and its search bound.
+  __device__ HistVisitor window(uint32_t Qc, uint32_t caph) const {
+    const uint32_t hiq = Qc + kWinUp;
+    const uint32_t K0 = hiq > HistVisitor::kBuckets ? hiq - HistVisitor::kBuckets : 0u;
+    const uint32_t top = dq16_top(K0 + HistVisitor::kBuckets - 1u);
+    return HistVisitor{(lds_u32*)this->hp.H, this->hp.lane, K0, top < caph ? top : caph, 0u, 0u};
+  }
+  // Q from the counts (full), or the neighbour count when every point within
+  // the cap was counted and there are fewer than k; false: take the heap pass.
+  __device__ bool resolve(const HistVisitor& hv, uint32_t caph, Knn16Visitor& kv) {
+    uint32_t Q;
+    if (hv.select((uint32_t)this->gv.k, Q)) {
+      kv.full = true;
+      kv.root = Q;
+      this->hp.size = this->gv.k;
+      return true;
+    }
+    if (hv.top_h == caph && hv.below + hv.inwin < (uint32_t)this->gv.k) {
+      this->hp.size = (int)(hv.below + hv.inwin);
+      return true;
+    }
+    return false;
+  }
+
+  // Second-pass state of the pending point.
+  __device__ Sum16CVisitor<VolAdd> pending_sums(VolAdd& add) const {
+    Sum16CVisitor<VolAdd> sv;
+    sv.Q = this->pend_Q;
+    sv.cap_h = f2u(this->gv.rcap2);
+    sv.inclusive = !this->pend_full;
+    sv.n_less = 0u;
+    sv.d0 = sv.i0 = sv.d1 = sv.i1 = LKEY_INF;
+    sv.add = &add;
+    return sv;
+  }
+
+  __device__ bool volume_flush(V3f& c) {
+    if (!this->pend) return false;
+    this->pend = false;
+    if (this->pend_m == 0) {
+      c = mul(this->pend_w, v3(0.0f, 0.0f, 0.0f));
+      return true;
+    }
+    VolAdd add{&this->gv, this->pend_wo, this->hg_g, {}};
+    nsum3_init(add.s);
+    Sum16CVisitor<VolAdd> sv = pending_sums(add);
+    SumOnlyVisitor<VolAdd> w{this->pend_x, &sv};
+    lane_visit_dual_spec<4>(this->gv, this->pend_x.x, this->pend_x.y, this->pend_x.z, w);
+    c = this->finish_pending(sv, add);
+    return true;
+  }
 
   // The heap pass of x: seeded with bh, then with the cap when fewer than k
   // points fall under the seed (one traversal call site for both).
@@ -307,36 +379,18 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
       } else {
         VolAdd add{&gv, this->pend_wo, this->hg_g, {}};
         nsum3_init(add.s);
-        Sum16CVisitor<VolAdd> sv;
-        sv.Q = this->pend_Q;
-        sv.cap_h = caph;
-        sv.inclusive = !this->pend_full;
-        sv.n_less = 0u;
-        sv.d0 = sv.i0 = sv.d1 = sv.i1 = LKEY_INF;
-        sv.add = &add;
+        Sum16CVisitor<VolAdd> sv = pending_sums(add);
         if (this->pend_full) {
           // counting window below and around the pending point's bucket; the
           // search bound is the top of the window (or the cap)
-          const uint32_t hiq = this->pend_Q + kWinUp;
-          const uint32_t K0 = hiq > HistVisitor::kBuckets ? hiq - HistVisitor::kBuckets : 0u;
-          const uint32_t top = dq16_top(K0 + HistVisitor::kBuckets - 1u);
-          HistVisitor hv{(lds_u32*)this->hp.H, this->hp.lane, K0, top < caph ? top : caph, 0u, 0u};
+          HistVisitor hv = window(this->pend_Q, caph);
           hv.clear();
           DualHistVisitor<VolAdd> dv{x, this->pend_x, &hv, &sv};
           lane_visit_dual_spec<4>(gv, x.x, x.y, x.z, dv);
-          uint32_t Q;
-          if (hv.select((uint32_t)gv.k, Q)) {
-            heap = false;
-            kv.full = true;
-            kv.root = Q;
-            this->hp.size = gv.k;
-          } else if (hv.top_h == caph && hv.below + hv.inwin < (uint32_t)gv.k) {
-            // every point within the cap was counted: fewer than k neighbours
-            heap = false;
-            this->hp.size = (int)(hv.below + hv.inwin);
-          }
+          heap = !resolve(hv, caph, kv);
         } else {
-          lane_visit<4>(gv, this->pend_x.x, this->pend_x.y, this->pend_x.z, sv);  // fewer than k within the cap
+          SumOnlyVisitor<VolAdd> w{this->pend_x, &sv};  // fewer than k within the cap
+          lane_visit_dual_spec<4>(gv, this->pend_x.x, this->pend_x.y, this->pend_x.z, w);
         }
         c = this->finish_pending(sv, add);
       }
