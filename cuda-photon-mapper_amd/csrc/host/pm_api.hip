@@ -414,6 +414,7 @@ int pm_render_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev, 
   r.blocks_per_cu = c->blocks_per_cu;
   r.gather_mode = c->gather_mode;
   r.tile_regions = c->tile_regions;
+  r.sample_lanes = c->sample_lanes;
   HIPCHK(c->counters.ensure(16));
   r.counters = c->counters.as<unsigned long long>();
   HIPCHK(hipMemsetAsync(c->counters.p, 0, 16, c->stream));
@@ -476,6 +477,10 @@ int pm_set_option(pm_context* c, int option, int value) {
     case PM_OPT_TILE_REGIONS:
       if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8) return PM_ERR_INVALID;
       c->tile_regions = value;
+      return PM_OK;
+    case PM_OPT_SAMPLE_LANES:
+      if (value != 0 && value != 1) return PM_ERR_INVALID;
+      c->sample_lanes = value;
       return PM_OK;
     default:
       return PM_ERR_INVALID;

@@ -346,8 +346,70 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
   }
 }
 
+// Mode 7 with the samples of a pixel in adjacent lanes.  For spp = 2, 4, 8 or
+// 16 a wave renders a tile of 64 / spp pixels (8x4, 4x4, 4x2 or 2x2), and lane
+// p * spp + s renders sample s of pixel p on its own; the pixel is the
+// compensated sum of its samples in order s = 0, 1, ..., as in the one-lane
+// sample loop.  The lanes of a pixel march nearly the same ray, so their
+// traversals and leaf batches line up.
+__device__ __forceinline__ int sample_tile_w(int spp) {
+  const int P = 64 / spp;
+  return P == 32 ? 8 : (P == 16 ? 4 : (P == 8 ? 4 : 2));
+}
+
+template <bool FP64T, bool CONE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_render_lane7s(RenderLaunch r) {
+  extern __shared__ uint16_t heap16_lds[];  // [64 slots][64 lanes]
+  const int lane = lane_id();
+  const int spp = r.rp.spp, lg = __builtin_ctz((unsigned)spp);
+  const int pw = sample_tile_w(spp), ph = (64 / spp) / pw;
+  const int p = lane >> lg, s = lane & (spp - 1);
+  const int tiles_x = (r.w + pw - 1) / pw;
+  const int tx = (int)blockIdx.x % tiles_x, ty = (int)blockIdx.x / tiles_x;
+  const int lx = tx * pw + p % pw, ly = ty * ph + p / pw;
+  if (lx >= r.w || ly >= r.h) return;  // all lanes of a pixel leave together
+  const uint32_t px = (uint32_t)(r.x0 + lx), py = (uint32_t)(r.y0 + ly);
+  LaneGather5<CONE> g;
+  g.gs = r.gs;
+  g.gv = r.gv;
+  g.hp.H = heap16_lds;
+  g.hp.lane = lane;
+  g.hp.k = 0;
+  g.hp.size = 0;
+  g.hg_g = r.sc.med.g;
+  g.nsurf = 0u;
+  g.nvol = 0u;
+  g.prev_r = -1.0f;
+  g.pend = false;
+  const V3f v = FP64T ? render_sample_t<float, double>(r.sc, r.rp, px, py, (uint32_t)s, g)
+                      : render_sample_t<float, float>(r.sc, r.rp, px, py, (uint32_t)s, g);
+  NSum3<float> acc;
+  nsum3_init(acc);
+  const int base = p << lg;
+  for (int t = 0; t < spp; ++t)
+    nsum3_add(acc, v3(lane_read_f(v.x, base + t), lane_read_f(v.y, base + t), lane_read_f(v.z, base + t)));
+  if (s == 0) store_pixel(r.rgb, r.u8, (size_t)ly * (size_t)r.w + (size_t)lx, nsum3_get(acc) * (1.0f / (float)spp),
+                          r.exposure);
+  if (r.counters) {
+    atomicAdd(&r.counters[0], (unsigned long long)g.nsurf);
+    atomicAdd(&r.counters[1], (unsigned long long)g.nvol);
+  }
+}
+
 hipError_t launch_render_lane7(const RenderLaunch& r, hipStream_t st) {
   if (r.w <= 0 || r.h <= 0) return hipSuccess;
+  const int spp = r.rp.spp;
+  if (r.sample_lanes && (spp == 2 || spp == 4 || spp == 8 || spp == 16)) {
+    const int pw = spp == 4 || spp == 8 ? 4 : (spp == 2 ? 8 : 2), ph = (64 / spp) / pw;
+    const unsigned tiles = (unsigned)(((r.w + pw - 1) / pw) * ((r.h + ph - 1) / ph));
+    size_t lds = 64 * 64 * sizeof(uint16_t);
+    const bool fp64 = r.rp.fp64_transmittance != 0, cone = r.estimator == 1;
+    if (fp64 && cone) hipLaunchKernelGGL((k_render_lane7s<true, true>), dim3(tiles), dim3(64), lds, st, r);
+    else if (fp64) hipLaunchKernelGGL((k_render_lane7s<true, false>), dim3(tiles), dim3(64), lds, st, r);
+    else if (cone) hipLaunchKernelGGL((k_render_lane7s<false, true>), dim3(tiles), dim3(64), lds, st, r);
+    else hipLaunchKernelGGL((k_render_lane7s<false, false>), dim3(tiles), dim3(64), lds, st, r);
+    return hipGetLastError();
+  }
   const unsigned tiles = lane_grid(r);
   size_t lds = 64 * 64 * sizeof(uint16_t);
   const bool fp64 = r.rp.fp64_transmittance != 0, cone = r.estimator == 1;

@@ -127,6 +127,7 @@ struct RenderLaunch {
   int blocks_per_cu;             // persistent grid: workgroups per compute unit
   int gather_mode;               // 0: wave per query; lane per query: 1 8-byte, 2 4-byte, 3 2-byte heap
   int tile_regions;              // lane modes: 0 raster tile order; rx = 1, 2, 4, 8: rx x (8 / rx) per-XCD regions
+  int sample_lanes;              // mode 7: a pixel's spp samples in adjacent lanes (spp 2, 4, 8 or 16)
 };
 hipError_t launch_render(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane(const RenderLaunch& r, hipStream_t st);

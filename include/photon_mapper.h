@@ -188,13 +188,18 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
  *                      0 = raster order (default), rx = 1, 2, 4 or 8 = the
  *                      frame cut into rx x (8 / rx) regions, one per XCD
  *                      (workgroups go to the 8 XCDs round-robin), so that each
- *                      XCD's L2 serves one compact part of the frame. */
+ *                      XCD's L2 serves one compact part of the frame;
+ * PM_OPT_SAMPLE_LANES  gather mode 7 with spp = 2, 4, 8 or 16: 1 = the samples of a
+ *                      pixel in adjacent lanes (default; a wave renders 64 / spp
+ *                      pixels in raster tile order), 0 = one lane loops over a
+ *                      pixel's samples (PM_OPT_TILE_REGIONS applies to this form). */
 enum {
   PM_OPT_GATHER_MODE = 1,
   PM_OPT_BLOCKS_PER_CU = 2,
   PM_OPT_CACHE_FUSED_INFER = 3,
   PM_OPT_CACHE_INFER_CHUNK = 4,
-  PM_OPT_TILE_REGIONS = 5
+  PM_OPT_TILE_REGIONS = 5,
+  PM_OPT_SAMPLE_LANES = 6
 };
 int pm_set_option(pm_context* ctx, int option, int value);
 

@@ -262,6 +262,7 @@ def test_tile_regions_same_image(pair1, regions):
     # crop is ragged (partial tiles on two sides)
     g, _ = pair1
     g.set_option(pmamd.PM_OPT_GATHER_MODE, 7)
+    g.set_option(pmamd.PM_OPT_SAMPLE_LANES, 0)
     try:
         g.set_option(pmamd.PM_OPT_TILE_REGIONS, 0)
         ref = g.render(13, 7, 101, 77)
@@ -269,6 +270,7 @@ def test_tile_regions_same_image(pair1, regions):
         img = g.render(13, 7, 101, 77)
     finally:
         g.set_option(pmamd.PM_OPT_TILE_REGIONS, 0)
+        g.set_option(pmamd.PM_OPT_SAMPLE_LANES, 1)
     assert np.array_equal(bits(img), bits(ref))
 
 
@@ -279,3 +281,43 @@ def test_tile_regions_option_checked(pair1):
             g.set_option(pmamd.PM_OPT_TILE_REGIONS, bad)
     with pytest.raises(pmamd.PmError):
         g.set_option(pmamd.PM_OPT_GATHER_MODE, 8)
+
+
+@pytest.mark.parametrize("spp", [2, 4, 8, 16])
+def test_sample_lanes_same_image(pair2, spp):
+    # a pixel's samples in adjacent lanes, summed in sample order: the same
+    # pixels as one lane looping over the samples (ragged crop)
+    g, _ = pair2
+    c = g.cfg
+    g.set_option(pmamd.PM_OPT_GATHER_MODE, 7)
+    try:
+        g.set_render(c.width, c.height, spp, c.k_surface, c.k_volume, c.rcap_surface, c.rcap_volume)
+        g.set_option(pmamd.PM_OPT_SAMPLE_LANES, 0)
+        ref = g.render(5, 3, 61, 37)
+        q0 = g.stats()
+        g.set_option(pmamd.PM_OPT_SAMPLE_LANES, 1)
+        img = g.render(5, 3, 61, 37)
+        q1 = g.stats()
+    finally:
+        g.set_option(pmamd.PM_OPT_SAMPLE_LANES, 1)
+        g.set_render(c.width, c.height, c.spp, c.k_surface, c.k_volume, c.rcap_surface, c.rcap_volume)
+    assert np.array_equal(bits(img), bits(ref))
+    assert (q0.surface_queries, q0.volume_queries) == (q1.surface_queries, q1.volume_queries)
+
+
+def test_sample_lanes_matches_oracle(pair2):
+    g, o = pair2
+    g.set_option(pmamd.PM_OPT_GATHER_MODE, 7)
+    try:
+        g.set_option(pmamd.PM_OPT_SAMPLE_LANES, 0)
+        loop = g.render()
+        g.set_option(pmamd.PM_OPT_SAMPLE_LANES, 1)
+        img = g.render()
+    finally:
+        g.set_option(pmamd.PM_OPT_SAMPLE_LANES, 1)
+    assert np.array_equal(bits(img), bits(loop))
+    assert_radiance_close(img, o.render())
+    s = g.stats()
+    assert (s.surface_queries, s.volume_queries) == o.last_queries
+    with pytest.raises(pmamd.PmError):
+        g.set_option(pmamd.PM_OPT_SAMPLE_LANES, 2)
