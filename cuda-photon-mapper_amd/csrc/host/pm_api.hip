@@ -59,7 +59,7 @@ int alloc_map(MapBufs& mb, uint32_t n, int bf16, int L) {
   HIPCHK(mb.y.ensure(4ull * std::max<uint32_t>(n, 1)));
   HIPCHK(mb.z.ensure(4ull * std::max<uint32_t>(n, 1)));
   HIPCHK(mb.payload.ensure((bf16 ? 16ull : 32ull) * std::max<uint32_t>(n, 1)));
-  HIPCHK(mb.tpts.ensure(16ull * std::max<uint32_t>(n, 1)));
+  HIPCHK(mb.tpts.ensure(16ull * ((uint64_t)n + kTptsPad)));
   uint32_t nl = kd_num_leaves(n, (uint32_t)L);
   HIPCHK(mb.nodes.ensure(sizeof(KdNode) * std::max<uint32_t>(nl, 1)));
   HIPCHK(mb.nrange.ensure(sizeof(uint2) * std::max<uint32_t>(nl, 1)));

@@ -167,7 +167,7 @@ int size_records(pm_batch* b) {
   HIPCHK(b->y.ensure(4 * N));
   HIPCHK(b->z.ensure(4 * N));
   HIPCHK(b->payload.ensure(stride * N));
-  HIPCHK(b->tpts.ensure(16 * N));
+  HIPCHK(b->tpts.ensure(16 * (N + kTptsPad)));
   for (int v = 0; v < 2; ++v) {
     DevMap& m = v ? b->mv : b->ms;
     memset(&m, 0, sizeof(m));

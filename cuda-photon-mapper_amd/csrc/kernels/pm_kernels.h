@@ -36,6 +36,11 @@ struct DevMap {
   bool built;
 };
 
+// Tree-order point arrays carry this many entries past the last point, so that
+// a leaf batch reads up to 8 consecutive entries from one address without
+// clamping; entries past a leaf's end are read but never used.
+static constexpr uint32_t kTptsPad = 8;
+
 // Gather parameters of one map (passed by value to kernels).
 struct GatherMap {
   const float4* tpts;

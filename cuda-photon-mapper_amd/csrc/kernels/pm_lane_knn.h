@@ -129,11 +129,9 @@ __device__ __forceinline__ int lane_knn(const GatherMap& g, float qx, float qy, 
     // issued before any is used, then the passing keys are inserted.
     for (uint32_t b0 = lo; b0 < hi; b0 += 8u) {
       float4 p[8];
+      const float4* q = g.tpts + b0;  // padded: entries past hi are read, not used
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        uint32_t i = b0 + (uint32_t)j;
-        p[j] = g.tpts[i < hi ? i : hi - 1u];
-      }
+      for (int j = 0; j < 8; ++j) p[j] = q[j];
       uint32_t dh[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) dh[j] = f2u(dist2(qx, qy, qz, p[j].x, p[j].y, p[j].z));

@@ -119,11 +119,9 @@ __device__ __forceinline__ void lane_visit(const GatherMap& g, float qx, float q
     if (!at_leaf) continue;
     for (uint32_t b0 = lo; b0 < hi; b0 += (uint32_t)NB) {
       float4 p[NB];
+      const float4* q = g.tpts + b0;  // padded: entries past hi are read, not used
 #pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        uint32_t i = b0 + (uint32_t)j;
-        p[j] = g.tpts[i < hi ? i : hi - 1u];
-      }
+      for (int j = 0; j < NB; ++j) p[j] = q[j];
       uint32_t dh[NB];
 #pragma unroll
       for (int j = 0; j < NB; ++j) dh[j] = f2u(dist2(qx, qy, qz, p[j].x, p[j].y, p[j].z));

@@ -75,11 +75,9 @@ __device__ __forceinline__ void lane_visit_dual(const GatherMap& g, float qx, fl
     if (!at_leaf) continue;
     for (uint32_t b0 = lo; b0 < hi; b0 += (uint32_t)NB) {
       float4 p[NB];
+      const float4* q = g.tpts + b0;  // padded: entries past hi are read, not used
 #pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        uint32_t i = b0 + (uint32_t)j;
-        p[j] = g.tpts[i < hi ? i : hi - 1u];
-      }
+      for (int j = 0; j < NB; ++j) p[j] = q[j];
       const uint32_t left = hi - b0;
       v.template points<NB>(p, left < (uint32_t)NB ? (int)left : NB);
     }

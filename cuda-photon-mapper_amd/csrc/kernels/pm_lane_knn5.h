@@ -224,11 +224,9 @@ __device__ __forceinline__ void lane_visit_dual_spec(const GatherMap& g, float q
     if (!kept) continue;
     for (uint32_t b0 = plo; b0 < phi; b0 += (uint32_t)NB) {
       float4 p[NB];
+      const float4* q = g.tpts + b0;  // padded: entries past phi are read, not used
 #pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        uint32_t i = b0 + (uint32_t)j;
-        p[j] = g.tpts[i < phi ? i : phi - 1u];
-      }
+      for (int j = 0; j < NB; ++j) p[j] = q[j];
       const uint32_t left = phi - b0;
       v.template points<NB>(p, left < (uint32_t)NB ? (int)left : NB);
     }
