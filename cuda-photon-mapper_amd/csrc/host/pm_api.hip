@@ -59,6 +59,7 @@ int alloc_map(MapBufs& mb, uint32_t n, int bf16, int L) {
   HIPCHK(mb.y.ensure(4ull * std::max<uint32_t>(n, 1)));
   HIPCHK(mb.z.ensure(4ull * std::max<uint32_t>(n, 1)));
   HIPCHK(mb.payload.ensure((bf16 ? 16ull : 32ull) * std::max<uint32_t>(n, 1)));
+  HIPCHK(mb.tpayload.ensure((bf16 ? 16ull : 32ull) * std::max<uint32_t>(n, 1)));
   HIPCHK(mb.tpts.ensure(16ull * ((uint64_t)n + kTptsPad)));
   uint32_t nl = kd_num_leaves(n, (uint32_t)L);
   HIPCHK(mb.nodes.ensure(sizeof(KdNode) * std::max<uint32_t>(nl, 1)));
@@ -81,6 +82,7 @@ int alloc_map(MapBufs& mb, uint32_t n, int bf16, int L) {
   m.nrange = mb.nrange.as<uint2>();
   m.bmin = mb.bmin.as<float4>();
   m.bmax = mb.bmax.as<float4>();
+  m.tpayload = mb.tpayload.p;
   m.built = false;
   return PM_OK;
 }
@@ -232,6 +234,7 @@ GatherMap gather_map(const pm_context* c, const DevMap& m, int k, float rcap) {
   g.bmin = m.bmin;
   g.bmax = m.bmax;
   g.payload = m.payload;
+  g.tpayload = m.tpayload;
   g.n = m.n;
   g.nl = m.nl;
   g.bf16 = m.bf16;

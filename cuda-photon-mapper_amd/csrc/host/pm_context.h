@@ -40,11 +40,11 @@ struct DevBuf {
 };
 
 struct MapBufs {
-  DevBuf x, y, z, payload, tpts, nodes, nrange, bmin, bmax;
+  DevBuf x, y, z, payload, tpts, nodes, nrange, bmin, bmax, tpayload;
   DevMap m;
   void release() {
     x.release(); y.release(); z.release(); payload.release(); tpts.release(); nodes.release(); nrange.release();
-    bmin.release(); bmax.release();
+    bmin.release(); bmax.release(); tpayload.release();
   }
 };
 

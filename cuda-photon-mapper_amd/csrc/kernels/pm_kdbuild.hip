@@ -343,6 +343,7 @@ hipError_t kd_build_impl(DevMap& m, uint32_t* temp, const uint32_t* host_lo, con
   for (uint64_t f = 0, w = 1; f < ni; f = 2 * f + 1, w *= 2) off.push_back((uint32_t)std::min<uint64_t>(f + w, ni));
   if ((e = kd_build_forest(F, off.data(), (uint32_t)off.size() - 1u, temp, st)) != hipSuccess) return e;
   if ((e = kd_node_boxes(m, st)) != hipSuccess) return e;
+  if ((e = kd_tree_payload(m, st)) != hipSuccess) return e;
   m.built = true;
   return hipSuccess;
 }
@@ -406,6 +407,21 @@ hipError_t kd_node_boxes(const DevMap& m, hipStream_t st) {
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+// tpayload[i] = payload[record index of tpts[i]] (16 B or 32 B per record).
+__global__ void k_tree_payload(const float4* tpts, const uint4* payload, uint4* tpayload, uint32_t n, int words) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t r = __float_as_uint(tpts[i].w);
+  for (int w = 0; w < words; ++w) tpayload[(size_t)words * i + w] = payload[(size_t)words * r + w];
+}
+
+hipError_t kd_tree_payload(const DevMap& m, hipStream_t st) {
+  if (!m.tpayload || m.n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tree_payload, dim3((m.n + 255u) / 256u), dim3(256), 0, st, (const float4*)m.tpts,
+                     (const uint4*)m.payload, (uint4*)m.tpayload, m.n, m.bf16 ? 1 : 2);
+  return hipGetLastError();
 }
 
 }  // namespace pm

@@ -33,6 +33,7 @@ struct DevMap {
   uint2* nrange;     // nl - 1: point range [lo, hi) of each internal node
   float4* bmin;      // 2 nl - 1 (internal nodes, then leaves): tight box of each node's points, or null
   float4* bmax;
+  void* tpayload;    // payload in tree order (tpayload[i] = payload[record of tpts[i]]), or null
   bool built;
 };
 
@@ -49,6 +50,7 @@ struct GatherMap {
   const float4* bmin;  // tight node boxes (2 nl - 1), or null
   const float4* bmax;
   const void* payload;
+  const void* tpayload;  // payload in tree order, or null
   uint32_t n, nl;
   int bf16;
   int k;
@@ -115,6 +117,8 @@ hipError_t kd_build_impl(DevMap& m, uint32_t* temp, const uint32_t* host_lo, con
                          const uint32_t* host_lc, const uint32_t* host_lb, hipStream_t st);
 // Tight boxes of every node's points into m.bmin / m.bmax (when allocated).
 hipError_t kd_node_boxes(const DevMap& m, hipStream_t st);
+// The payload in tree order into m.tpayload (when allocated).
+hipError_t kd_tree_payload(const DevMap& m, hipStream_t st);
 
 // gather (pm_gather.hip)
 hipError_t launch_knn(GatherMap g, const float* q, uint32_t nq, uint32_t* idx, float* d2, uint32_t* count,

@@ -175,6 +175,23 @@ __device__ __forceinline__ void load_payload_l(const GatherMap& g, uint32_t idx,
   }
 }
 
+// The same from the tree-ordered copy (tree position pos).
+__device__ __forceinline__ void load_payload_tree(const GatherMap& g, uint32_t pos, V3f& pw, V3f& dir) {
+  if (g.bf16) {
+    uint4 w = ((const uint4*)g.tpayload)[pos];
+    pw = v3(bf16_to_f32((uint16_t)(w.x & 0xffffu)), bf16_to_f32((uint16_t)(w.x >> 16)),
+            bf16_to_f32((uint16_t)(w.y & 0xffffu)));
+    dir = v3(bf16_to_f32((uint16_t)(w.y >> 16)), bf16_to_f32((uint16_t)(w.z & 0xffffu)),
+             bf16_to_f32((uint16_t)(w.z >> 16)));
+  } else {
+    const float4* q = (const float4*)g.tpayload;
+    float4 a = q[2 * (size_t)pos];
+    float2 b = *(const float2*)&q[2 * (size_t)pos + 1];
+    pw = v3(a.x, a.y, a.z);
+    dir = v3(a.w, b.x, b.y);
+  }
+}
+
 template <bool CONE>
 struct LaneGather {
   GatherMap gs, gv;

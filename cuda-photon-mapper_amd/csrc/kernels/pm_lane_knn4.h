@@ -155,7 +155,8 @@ struct LaneGather4 {
   // Bucket selection of the pending point after its second pass: adds the
   // (k - #below) smallest bucket keys (extra exact passes beyond two) and
   // returns w * value.
-  __device__ V3f finish_pending(Sum16Visitor<VolAdd>& sv, VolAdd& add) {
+  template <class SV, class A>
+  __device__ V3f finish_pending(SV& sv, A& add) {
     const uint32_t cap_h = f2u(gv.rcap2);
     float r2 = u2f(cap_h);
     if (pend_full) {

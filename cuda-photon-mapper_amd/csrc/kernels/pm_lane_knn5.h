@@ -102,15 +102,37 @@ struct HistVisitor {
   }
 };
 
+// Volume contributions, by record index (the tie picks of the exact
+// selection) or by tree position (the second pass, which reads the payload
+// beside the leaf points it has just read).
+struct VolAddT {
+  const GatherMap* g;
+  V3f wo;
+  float hg;
+  NSum3<float> s;
+  __device__ void add(V3f pw, V3f dir) { nsum3_add(s, pw * phase_hg(dot(dir, wo), hg)); }
+  __device__ void operator()(uint32_t idx, uint32_t) {
+    V3f pw, dir;
+    load_payload_l(*g, idx, pw, dir);
+    add(pw, dir);
+  }
+  __device__ void tree(uint32_t pos) {
+    V3f pw, dir;
+    load_payload_tree(*g, pos, pw, dir);
+    add(pw, dir);
+  }
+};
+
 // Second pass with the contributions of a batch compacted per lane: a lane's
 // neighbours among the batch's points are gathered first, then the
 // contribution code runs max-over-lanes times instead of once per point
-// position that any lane takes (same points, same order).  For contributions
-// that do not use d^2 (the volume estimate).
+// position that any lane takes (same points, same order).  The batch's points
+// are tree positions base, base + 1, ...; contributions come from the
+// tree-ordered payload.
 template <class Add>
 struct Sum16CVisitor : Sum16Visitor<Add> {
   template <int NB>
-  __device__ void points(const float4* p, const uint32_t* dh, int cnt) {
+  __device__ void points(const float4* p, const uint32_t* dh, int cnt, uint32_t base) {
     uint32_t sel[NB];
 #pragma unroll
     for (int j = 0; j < NB; ++j) sel[j] = 0u;
@@ -118,20 +140,19 @@ struct Sum16CVisitor : Sum16Visitor<Add> {
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
       const uint32_t d = dh[j];
-      const uint32_t idx = f2u(p[j].w);
       const bool in = j < cnt;
       const bool take = in && (this->inclusive ? d <= this->cap_h : dq16(d) < this->Q);
 #pragma unroll
-      for (int t = 0; t <= j; ++t) sel[t] = take && nt == (uint32_t)t ? idx : sel[t];
+      for (int t = 0; t <= j; ++t) sel[t] = take && nt == (uint32_t)t ? base + (uint32_t)j : sel[t];
       nt += take ? 1u : 0u;
-      if (in && !take && !this->inclusive && dq16(d) == this->Q) this->keep(d, idx);
+      if (in && !take && !this->inclusive && dq16(d) == this->Q) this->keep(d, f2u(p[j].w));
     }
     this->n_less += nt;
     for (uint32_t t = 0; t < nt; ++t) {
-      uint32_t idx = sel[0];
+      uint32_t pos = sel[0];
 #pragma unroll
-      for (int s = 1; s < NB; ++s) idx = t == (uint32_t)s ? sel[s] : idx;
-      (*this->add)(idx, 0u);
+      for (int s = 1; s < NB; ++s) pos = t == (uint32_t)s ? sel[s] : pos;
+      this->add->tree(pos);
     }
   }
 };
@@ -228,7 +249,7 @@ __device__ __forceinline__ void lane_visit_dual_spec(const GatherMap& g, float q
 #pragma unroll
       for (int j = 0; j < NB; ++j) p[j] = q[j];
       const uint32_t left = phi - b0;
-      v.template points<NB>(p, left < (uint32_t)NB ? (int)left : NB);
+      v.template points<NB>(p, left < (uint32_t)NB ? (int)left : NB, b0);
     }
     kept = false;
   }
@@ -251,7 +272,7 @@ struct DualHistVisitor {
     return box_d2(qn, mn, mx) <= u2f(hv->bound()) || box_d2(qp, mn, mx) <= u2f(sv->bound());
   }
   template <int NB>
-  __device__ void points(const float4* p, int cnt) {
+  __device__ void points(const float4* p, int cnt, uint32_t base) {
     uint32_t dn[NB], dp[NB];
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
@@ -259,7 +280,7 @@ struct DualHistVisitor {
       dp[j] = f2u(dist2(qp.x, qp.y, qp.z, p[j].x, p[j].y, p[j].z));
     }
     hv->template points<NB>(dn, cnt);
-    sv->template points<NB>(p, dp, cnt);
+    sv->template points<NB>(p, dp, cnt, base);
   }
 };
 
@@ -276,18 +297,18 @@ struct SumOnlyVisitor {
   }
   __device__ bool near_box(float4 mn, float4 mx) const { return box_d2(q, mn, mx) <= u2f(sv->bound()); }
   template <int NB>
-  __device__ void points(const float4* p, int cnt) {
+  __device__ void points(const float4* p, int cnt, uint32_t base) {
     uint32_t d[NB];
 #pragma unroll
     for (int j = 0; j < NB; ++j) d[j] = f2u(dist2(q.x, q.y, q.z, p[j].x, p[j].y, p[j].z));
-    sv->template points<NB>(p, d, cnt);
+    sv->template points<NB>(p, d, cnt, base);
   }
 };
 
 template <bool CONE>
 struct LaneGather5 : LaneGather4<CONE, 4> {
   using Base = LaneGather4<CONE, 4>;
-  using typename Base::VolAdd;
+  using VolAdd = VolAddT;
   // The window reaches kWinUp buckets above the pending point's bucket Q'
   // (a factor 2^(kWinUp/256) in radius) and 256 - kWinUp below it.
   static constexpr uint32_t kWinUp = 96;
