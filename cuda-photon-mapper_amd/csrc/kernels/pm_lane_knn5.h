@@ -148,12 +148,14 @@ struct Sum16CVisitor : Sum16Visitor<Add> {
       if (in && !take && !this->inclusive && dq16(d) == this->Q) this->keep(d, f2u(p[j].w));
     }
     this->n_less += nt;
-    for (uint32_t t = 0; t < nt; ++t) {
-      uint32_t pos = sel[0];
-#pragma unroll
-      for (int s = 1; s < NB; ++s) pos = t == (uint32_t)s ? sel[s] : pos;
-      this->add->tree(pos);
-    }
+    if (nt == 0u) return;
+    // the first two payloads are loaded together, then summed in order
+    V3f pw0, dir0, pw1, dir1;
+    load_payload_tree(*this->add->g, sel[0], pw0, dir0);
+    if (nt > 1u) load_payload_tree(*this->add->g, sel[1], pw1, dir1);
+    this->add->add(pw0, dir0);
+    if (nt > 1u) this->add->add(pw1, dir1);
+    for (uint32_t t = 2; t < nt; ++t) this->add->tree(t == 2u ? sel[2] : sel[3]);
   }
 };
 
