@@ -191,7 +191,9 @@ PM_HD void camera_ray(const RenderParams& rp, uint32_t px, uint32_t py, uint32_t
 // declares `static constexpr bool kDeferredVolume = true` and provides
 //   bool volume_defer(x, wo, w, c) -- queue this point; c = w * value of an
 //                                     earlier point when one is ready
-//   bool volume_flush(c)           -- the last queued point's w * value
+//   bool volume_flush(wo, c)       -- the last queued point's w * value (wo:
+//                                     the segment's view direction, which is
+//                                     also the queued point's)
 // The march adds the weighted values in step order either way, so the sum
 // is the same as with the immediate g.volume.
 template <class G, class = void>
@@ -238,7 +240,7 @@ PM_HD void march_segment(const Scene& sc, const RenderParams& rp, V3<T> o, V3<T>
   }
   if constexpr (DeferredVolume<G>::value) {
     V3<T> c;
-    if (g.volume_flush(c)) nsum3_add(L, c);
+    if (g.volume_flush(wo, c)) nsum3_add(L, c);
   }
   T Tseg;
   if (m.type == MED_HOMOGENEOUS) Tseg = (T)exp_(-st * (TT)(t1 - t0));

@@ -128,6 +128,7 @@ __device__ __forceinline__ V3f lane_pixel4(const Scene& sc, const RenderParams& 
   g.nvol = 0u;
   g.prev_r = -1.0f;
   g.pend = false;
+  g.pend_full = false;
   NSum3<float> acc;
   nsum3_init(acc);
   for (int s = 0; s < rp.spp; ++s) {
@@ -381,6 +382,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
   g.nvol = 0u;
   g.prev_r = -1.0f;
   g.pend = false;
+  g.pend_full = false;
   const V3f v = FP64T ? render_sample_t<float, double>(r.sc, r.rp, px, py, (uint32_t)s, g)
                       : render_sample_t<float, float>(r.sc, r.rp, px, py, (uint32_t)s, g);
   NSum3<float> acc;

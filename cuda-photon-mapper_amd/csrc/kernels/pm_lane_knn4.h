@@ -247,7 +247,7 @@ struct LaneGather4 {
     return true;
   }
 
-  __device__ bool volume_flush(V3f& c) {
+  __device__ bool volume_flush(V3f, V3f& c) {
     if (!pend) return false;
     pend = false;
     if (pend_m == 0) {

@@ -350,14 +350,16 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
     return sv;
   }
 
-  __device__ bool volume_flush(V3f& c) {
+  // The pending point lies on the current march segment, so its view
+  // direction is the wo of the call (it is not kept).
+  __device__ bool volume_flush(V3f wo, V3f& c) {
     if (!this->pend) return false;
     this->pend = false;
     if (this->pend_m == 0) {
       c = mul(this->pend_w, v3(0.0f, 0.0f, 0.0f));
       return true;
     }
-    VolAdd add{&this->gv, this->pend_wo, this->hg_g, {}};
+    VolAdd add{&this->gv, wo, this->hg_g, {}};
     nsum3_init(add.s);
     Sum16CVisitor<VolAdd> sv = pending_sums(add);
     SumOnlyVisitor<VolAdd> w{this->pend_x, &sv};
@@ -378,27 +380,31 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
     }
   }
 
+  // The point becomes pending (its view direction is not kept: see
+  // volume_flush).
+  __device__ void queue(V3f x, V3f w, const Knn16Visitor& kv) {
+    this->pend = true;
+    this->pend_full = kv.full;
+    this->pend_m = this->hp.size;
+    this->pend_Q = kv.root;
+    this->pend_x = x;
+    this->pend_w = w;
+  }
+
   __device__ bool volume_defer(V3f x, V3f wo, V3f w, V3f& c) {
     ++this->nvol;
     const GatherMap& gv = this->gv;
     const uint32_t caph = f2u(gv.rcap2);
-    uint32_t bh = caph;
-    if (this->prev_r >= 0.0f) {
-      float dx = x.x - this->prev_x.x, dy = x.y - this->prev_x.y, dz = x.z - this->prev_x.z;
-      float b = (this->prev_r + sqrt_((dx * dx + dy * dy) + dz * dz)) * 1.0001f;
-      float b2 = b * b;
-      if (b2 < gv.rcap2) bh = f2u(b2);
-    }
     this->hp.k = gv.k;
     this->hp.size = 0;
-    Knn16Visitor kv{&this->hp, bh, 0u, false};
+    Knn16Visitor kv{&this->hp, caph, 0u, false};
     const bool had = this->pend;
     bool heap = true;
     if (had) {
       if (this->pend_m == 0) {
         c = mul(this->pend_w, v3(0.0f, 0.0f, 0.0f));
       } else {
-        VolAdd add{&gv, this->pend_wo, this->hg_g, {}};
+        VolAdd add{&gv, wo, this->hg_g, {}};
         nsum3_init(add.s);
         Sum16CVisitor<VolAdd> sv = pending_sums(add);
         if (this->pend_full) {
@@ -416,8 +422,20 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
         c = this->finish_pending(sv, add);
       }
     }
-    if (heap) this->heap_pass(x, bh, caph, kv);
-    this->queue(x, wo, w, kv);
+    if (heap) {
+      // seeded with (r' + |x - x'|)^2 from the last queued point x' when it
+      // had k neighbours (r' = top of its bucket): an upper bound of r_k
+      uint32_t bh = caph;
+      if (this->pend_full) {
+        const float pr = sqrt_(u2f(dq16_top(this->pend_Q)));
+        float dx = x.x - this->pend_x.x, dy = x.y - this->pend_x.y, dz = x.z - this->pend_x.z;
+        float b = (pr + sqrt_((dx * dx + dy * dy) + dz * dz)) * 1.0001f;
+        float b2 = b * b;
+        if (b2 < gv.rcap2) bh = f2u(b2);
+      }
+      this->heap_pass(x, bh, caph, kv);
+    }
+    queue(x, w, kv);
     return had;
   }
 };
