@@ -321,3 +321,22 @@ def test_sample_lanes_matches_oracle(pair2):
     assert (s.surface_queries, s.volume_queries) == o.last_queries
     with pytest.raises(pmamd.PmError):
         g.set_option(pmamd.PM_OPT_SAMPLE_LANES, 2)
+
+
+@pytest.mark.parametrize("estimator", ["box", "cone"])
+def test_sample_lanes_bf16_fp64_transmittance(ncpu, estimator):
+    # BF16 power/direction storage, FP64 transmittance and both surface
+    # estimators with a pixel's 4 samples in adjacent lanes: the same pixels as
+    # the one-lane sample loop, and the oracle's within tolerance
+    c = pmamd.preset(3)
+    c.n_emit = 30000
+    c.width, c.height, c.spp = 64, 40, 4
+    c.estimator = pmamd.PM_EST_CONE if estimator == "cone" else pmamd.PM_EST_BOX
+    g, o = run_pair(c, ncpu)
+    g.set_option(pmamd.PM_OPT_GATHER_MODE, 7)
+    g.set_option(pmamd.PM_OPT_SAMPLE_LANES, 0)
+    loop = g.render()
+    g.set_option(pmamd.PM_OPT_SAMPLE_LANES, 1)
+    img = g.render()
+    assert np.array_equal(bits(img), bits(loop))
+    assert_radiance_close(img, o.render(bf16=True))
