@@ -187,6 +187,7 @@ int pm_cache_reset(pm_context* c, uint32_t seed) {
   HIPCHK(hipMemset(c->cache.adam_m.p, 0, 4 * mlp_param_count()));
   HIPCHK(hipMemset(c->cache.adam_v.p, 0, 4 * mlp_param_count()));
   c->cache.step = 0;
+  c->cache.frame = 0;
   return upload_params(c, p.data());
 }
 
@@ -328,6 +329,8 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   // 4: online training on a subset with exact kNN targets
   GatherMap gs = make_gather_map(c, c->ms.m, c->cfg.k_surface, c->cfg.rcap_surface);
   GatherMap gv = make_gather_map(c, c->mv.m, c->cfg.k_volume, c->cfg.rcap_volume);
+  // exact values with the gather of mode 7 under modes 7 and 5 (auto)
+  const int mode7 = c->gather_mode == 7 || c->gather_mode == 5 ? 1 : 0;
   const uint32_t ncell = 2u * CACHE_GRID * CACHE_GRID * CACHE_GRID;
   HIPCHK(k.err_sum.ensure(8ull * ncell));
   HIPCHK(k.err_cnt.ensure(4ull * ncell));
@@ -366,6 +369,7 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
     vl.in = k.sel_in.as<float>();
     vl.n = S;
     vl.out = k.sel_t.as<float>();
+    vl.mode7 = mode7;
     HIPCHK(launch_cache_values(vl, st));
     int r = cache_train_dev(c, k.sel_in.as<float>(), k.sel_t.as<float>(), S, k.batch, k.lr, &loss,
                             k.err_sum.as<unsigned long long>(), k.err_cnt.as<unsigned int>());
@@ -386,24 +390,35 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   fp.cache_surface = k.cache_surface;
   for (int j = 0; j < 3; ++j) fp.light[j] = c->scene.light[j];
   fp.near2 = rl.near2;
-  HIPCHK(launch_cache_flags(k.rec_in.as<float>(), N, k.err_sum.as<unsigned long long>(), k.err_cnt.as<unsigned int>(),
-                            fp, k.flags.as<uint32_t>(), st));
   HIPCHK(c->small.ensure(64));
-  HIPCHK(exclusive_scan(k.flags.as<uint32_t>(), k.flag_off.as<uint32_t>(), N, c->scratch.as<uint32_t>(),
-                        c->small.as<uint32_t>() + 8, st));
-  HIPCHK(launch_cache_compact(k.flags.as<uint32_t>(), k.flag_off.as<uint32_t>(), N, k.list.as<uint32_t>(), st));
-  uint32_t nfb = 0;
-  HIPCHK(hipMemcpyAsync(&nfb, c->small.as<uint32_t>() + 8, 4, hipMemcpyDeviceToHost, st));
+  // the list holds the surface records, then the volume records, each in
+  // record order: a wave then runs one kind of gather
+  uint32_t* ntot = c->small.as<uint32_t>() + 8;
+  for (int kind = 1; kind <= 2; ++kind) {
+    fp.kind = kind;
+    HIPCHK(launch_cache_flags(k.rec_in.as<float>(), N, k.err_sum.as<unsigned long long>(),
+                              k.err_cnt.as<unsigned int>(), fp, k.flags.as<uint32_t>(), st));
+    HIPCHK(exclusive_scan(k.flags.as<uint32_t>(), k.flag_off.as<uint32_t>(), N, c->scratch.as<uint32_t>(),
+                          ntot + kind - 1, st));
+    HIPCHK(launch_cache_compact(k.flags.as<uint32_t>(), k.flag_off.as<uint32_t>(), N, k.list.as<uint32_t>(),
+                                kind == 2 ? ntot : nullptr, st));
+  }
+  uint32_t nk2[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(nk2, ntot, 8, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  if (nfb) {
+  const uint32_t nfb = nk2[0] + nk2[1];
+  for (int kind = 1; kind <= 2; ++kind) {  // one launch per kind of row
+    if (nk2[kind - 1] == 0) continue;
     CacheValueLaunch vl;
     memset(&vl, 0, sizeof(vl));
     vl.gs = gs; vl.gv = gv; vl.hg_g = c->scene.med.g;
     vl.in = k.rec_in.as<float>();
-    vl.list = k.list.as<uint32_t>();
-    vl.n = nfb;
+    vl.list = k.list.as<uint32_t>() + (kind == 2 ? nk2[0] : 0u);
+    vl.n = nk2[kind - 1];
     vl.out = k.rec_val.as<float>();
     vl.out_by_list = 1;
+    vl.mode7 = mode7;
+    vl.kind = kind;
     HIPCHK(launch_cache_values(vl, st));
   }
   HIPCHK(hipEventRecord(ev[4], st));

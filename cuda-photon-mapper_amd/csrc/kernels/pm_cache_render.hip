@@ -18,7 +18,7 @@
 //   7. k_accumulate   : per pixel, compensated sum of weight x value / spp.
 // No reference counterpart (the reference has no cache); the plain path it
 // replaces is photon_mapping_kernel, photonMappingKernel.cu:1409-1462.
-#include "pm_lane_knn3.h"
+#include "pm_lane_knn5.h"
 #include "pm_mlp.h"
 
 namespace pm {
@@ -161,6 +161,52 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
   v.out[3 * o + 2] = val.z;
 }
 
+__device__ __forceinline__ void put_value(const CacheValueLaunch& v, uint32_t o, V3f val) {
+  v.out[3 * (size_t)o] = val.x;
+  v.out[3 * (size_t)o + 1] = val.y;
+  v.out[3 * (size_t)o + 2] = val.z;
+}
+
+// The same values with the gather of mode 7 (pm_lane_knn5.h), one row per
+// lane: a heap pass for Q on the speculative traversal with node boxes, then
+// the second pass (also speculative, from the tree-ordered payload), for
+// surface and volume rows alike.  A row has no previous march point to seed a
+// count window from.  Values equal k_values'.  KIND 1 or 2: every row is a
+// surface or a volume query (the other branch is not compiled in).
+template <int KIND>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_values7(CacheValueLaunch v) {
+  extern __shared__ uint32_t heap_lds[];
+  const int lane = lane_id();
+  const uint32_t i = blockIdx.x * 64u + (uint32_t)lane;
+  if (i >= v.n) return;
+  const uint32_t rec = v.list ? v.list[i] : i;
+  const uint32_t o = v.out_by_list ? rec : i;
+  const float* p = v.in + (size_t)rec * MLP_IN;
+  LaneGather5<false, true> g;
+  g.gs = v.gs;
+  g.gv = v.gv;
+  g.hp.H = (uint16_t*)heap_lds;
+  g.hp.lane = lane;
+  g.hp.k = 0;
+  g.hp.size = 0;
+  g.hg_g = v.hg_g;
+  g.nsurf = g.nvol = 0u;
+  g.prev_r = -1.0f;
+  g.pend = false;
+  g.pend_full = false;
+  const V3f one = v3(1.0f, 1.0f, 1.0f);
+  V3f x = v3(p[0], p[1], p[2]), d = v3(p[3], p[4], p[5]), nrm = v3(p[6], p[7], p[8]);
+  V3f c;
+  const bool surf = KIND == 0 ? !(nrm.x == 0.0f && nrm.y == 0.0f && nrm.z == 0.0f) : KIND == 1;
+  if (surf) {
+    c = g.surface_lone(x, nrm, one);
+  } else {
+    (void)g.volume_defer(x, d, one, c);  // nothing pending: queues x
+    (void)g.volume_flush(d, c);
+  }
+  put_value(v, o, c);
+}
+
 // Training subset, stratified by query kind: rows j < count/2 are drawn from
 // the surface records (list_s, n_s of them), the rest from the volume records
 // (list_v, n_v), each uniformly with a Philox draw (surface queries are few
@@ -229,6 +275,7 @@ __global__ void k_flags(const float* __restrict__ in, uint64_t n, const unsigned
   // mean over the cell's training rows of the per-channel squared log error
   const double mse = (double)esum[cell] / 16777216.0 / ((double)c * (double)MLP_OUT);
   bool fb = (surf && !fp.cache_surface) || near || c == 0u || mse > (double)fp.threshold;
+  if (fp.kind != 0) fb = fb && surf == (fp.kind == 1);
   flags[i] = fb ? 1u : 0u;
 }
 
@@ -248,9 +295,9 @@ __global__ void __launch_bounds__(256) k_sum_rows(const float* __restrict__ v, u
 }
 
 __global__ void k_compact(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ off, uint64_t n,
-                          uint32_t* __restrict__ list) {
+                          uint32_t* __restrict__ list, const uint32_t* __restrict__ base) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && flags[i]) list[off[i]] = (uint32_t)i;
+  if (i < n && flags[i]) list[(base ? *base : 0u) + off[i]] = (uint32_t)i;
 }
 
 // Per pixel: compensated sum over its records of weight * value, / spp.
@@ -290,7 +337,12 @@ hipError_t launch_cache_record(const CacheRecordLaunch& r, hipStream_t st) {
 
 hipError_t launch_cache_values(const CacheValueLaunch& v, hipStream_t st) {
   if (v.n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_values, dim3((v.n + 63) / 64), dim3(64), 64 * 64 * sizeof(uint16_t), st, v);
+  const size_t lds = 64 * 64 * sizeof(uint16_t);  // 2-byte heap of 64 keys per lane
+  const dim3 grid((v.n + 63) / 64);
+  if (!v.mode7) hipLaunchKernelGGL(k_values, grid, dim3(64), lds, st, v);
+  else if (v.kind == 1) hipLaunchKernelGGL(k_values7<1>, grid, dim3(64), lds, st, v);
+  else if (v.kind == 2) hipLaunchKernelGGL(k_values7<2>, grid, dim3(64), lds, st, v);
+  else hipLaunchKernelGGL(k_values7<0>, grid, dim3(64), lds, st, v);
   return hipGetLastError();
 }
 
@@ -328,10 +380,15 @@ hipError_t launch_cache_flags(const float* in, uint64_t n, const unsigned long l
 }
 
 hipError_t launch_cache_compact(const uint32_t* flags, const uint32_t* off, uint64_t n, uint32_t* list,
-                                hipStream_t st) {
+                                const uint32_t* base, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_compact, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, flags, off, n, list);
+  hipLaunchKernelGGL(k_compact, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, flags, off, n, list, base);
   return hipGetLastError();
+}
+
+hipError_t launch_cache_compact(const uint32_t* flags, const uint32_t* off, uint64_t n, uint32_t* list,
+                                hipStream_t st) {
+  return launch_cache_compact(flags, off, n, list, nullptr, st);
 }
 
 hipError_t launch_cache_accumulate(const uint32_t* off, const uint32_t* cnt, uint32_t npix, const float* wt,

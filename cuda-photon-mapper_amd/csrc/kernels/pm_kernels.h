@@ -211,6 +211,7 @@ struct CacheFlagParams {
   int cache_surface;         // 0: surface records always fall back
   float light[3];
   float near2;               // volume records this close to the light fall back
+  int kind;                  // flag 0: every fallback record, 1: surface ones only, 2: volume ones only
 };
 struct CacheValueLaunch {
   GatherMap gs, gv;
@@ -220,6 +221,8 @@ struct CacheValueLaunch {
   uint32_t n;
   float* out;                // values (3 floats)
   int out_by_list;           // 1: out[list[i]], 0: out[i]
+  int mode7;                 // 1: the gather of mode 7 (k_values7), 0: the 2-byte heap gather (k_values)
+  int kind;                  // rows are 0: of either kind, 1: all surface queries, 2: all volume queries
 };
 hipError_t launch_cache_record(const CacheRecordLaunch& r, hipStream_t st);
 hipError_t launch_cache_values(const CacheValueLaunch& v, hipStream_t st);
@@ -231,6 +234,9 @@ hipError_t launch_err_accum(const float* in, const float* row_err, uint32_t rows
 hipError_t launch_sum_rows(const float* v, uint32_t n, double* acc, hipStream_t st);
 hipError_t launch_cache_flags(const float* in, uint64_t n, const unsigned long long* esum, const unsigned int* ecnt,
                               const CacheFlagParams& fp, uint32_t* flags, hipStream_t st);
+// list[base + off[i]] = i where flags[i] (base: a device word, or null for 0)
+hipError_t launch_cache_compact(const uint32_t* flags, const uint32_t* off, uint64_t n, uint32_t* list,
+                                const uint32_t* base, hipStream_t st);
 hipError_t launch_cache_compact(const uint32_t* flags, const uint32_t* off, uint64_t n, uint32_t* list,
                                 hipStream_t st);
 hipError_t launch_cache_accumulate(const uint32_t* off, const uint32_t* cnt, uint32_t npix, const float* wt,

@@ -123,6 +123,27 @@ struct VolAddT {
   }
 };
 
+// Surface contributions without the cone filter (power of the photons that
+// arrived from the query's side), by record index or by tree position.
+struct SurfAddT {
+  const GatherMap* g;
+  V3f n;
+  NSum3<float> s;
+  __device__ void add(V3f pw, V3f dir) {
+    if (dot(dir, n) < 0.0f) nsum3_add(s, pw);
+  }
+  __device__ void operator()(uint32_t idx, uint32_t) {
+    V3f pw, dir;
+    load_payload_l(*g, idx, pw, dir);
+    add(pw, dir);
+  }
+  __device__ void tree(uint32_t pos) {
+    V3f pw, dir;
+    load_payload_tree(*g, pos, pw, dir);
+    add(pw, dir);
+  }
+};
+
 // Second pass with the contributions of a batch compacted per lane: a lane's
 // neighbours among the batch's points are gathered first, then the
 // contribution code runs max-over-lanes times instead of once per point
@@ -307,7 +328,33 @@ struct SumOnlyVisitor {
   }
 };
 
-template <bool CONE>
+// The heap pass of one point on the speculative traversal with node boxes.
+// Its bound shrinks as the heap fills, so a lane decides on far children and
+// kept leaves with the bound of that moment, which may be larger than the one
+// the leaves are later tested against: more leaves are visited than the
+// shrunk bound needs, never fewer.  The heap ends with the k smallest bucket
+// keys whatever the order of the points, so Q is that of lane_visit.
+struct HeapSpecVisitor {
+  V3f q;
+  Knn16Visitor* kv;
+  __device__ bool far(int a, float split) const {
+    const float d = axis_of(q, a) - split;
+    return d * d <= u2f(kv->bound());
+  }
+  __device__ bool near_box(float4 mn, float4 mx) const { return box_d2(q, mn, mx) <= u2f(kv->bound()); }
+  template <int NB>
+  __device__ void points(const float4* p, int cnt, uint32_t) {
+    uint32_t d[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) d[j] = f2u(dist2(q.x, q.y, q.z, p[j].x, p[j].y, p[j].z));
+    kv->template points<NB>(p, d, cnt);
+  }
+};
+
+// SPEC_HEAP: the heap pass on the speculative traversal (HeapSpecVisitor), for
+// points that are each gathered on their own; the march's heap passes are rare
+// and scattered, and run faster on the plain traversal.
+template <bool CONE, bool SPEC_HEAP = false>
 struct LaneGather5 : LaneGather4<CONE, 4> {
   using Base = LaneGather4<CONE, 4>;
   using VolAdd = VolAddT;
@@ -375,7 +422,12 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
     for (int t = 0; t < 2; ++t) {
       this->hp.size = 0;
       kv.bound_h = t == 0 ? bh : caph;
-      lane_visit<4>(this->gv, x.x, x.y, x.z, kv);
+      if constexpr (SPEC_HEAP) {
+        HeapSpecVisitor hs{x, &kv};
+        lane_visit_dual_spec<4>(this->gv, x.x, x.y, x.z, hs);
+      } else {
+        lane_visit<4>(this->gv, x.x, x.y, x.z, kv);
+      }
       if (kv.full || bh == caph) break;
     }
   }
@@ -389,6 +441,60 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
     this->pend_Q = kv.root;
     this->pend_x = x;
     this->pend_w = w;
+  }
+
+  // A surface query on its own, without the cone filter: heap pass and second
+  // pass on the speculative traversal, then the bucket selection of
+  // knn16_gather (pm_lane_knn3.h); the same value as surface() with CONE off.
+  __device__ V3f surface_lone(V3f x, V3f n, V3f albedo) {
+    static_assert(!CONE, "surface_lone has no cone filter");
+    ++this->nsurf;
+    const GatherMap& gs = this->gs;
+    const uint32_t caph = f2u(gs.rcap2);
+    this->hp.k = gs.k;
+    this->hp.size = 0;
+    Knn16Visitor kv{&this->hp, caph, 0u, false};
+    HeapSpecVisitor hs{x, &kv};
+    lane_visit_dual_spec<4>(gs, x.x, x.y, x.z, hs);
+    if (this->hp.size == 0) return v3(0.0f, 0.0f, 0.0f);
+    SurfAddT add{&gs, n, {}};
+    nsum3_init(add.s);
+    Sum16CVisitor<SurfAddT> sv;
+    sv.Q = kv.root;
+    sv.cap_h = caph;
+    sv.inclusive = !kv.full;
+    sv.n_less = 0u;
+    sv.d0 = sv.i0 = sv.d1 = sv.i1 = LKEY_INF;
+    sv.add = &add;
+    SumOnlyVisitor<SurfAddT> w{x, &sv};
+    lane_visit_dual_spec<4>(gs, x.x, x.y, x.z, w);
+    float r2 = u2f(caph);
+    if (kv.full) {
+      const uint32_t need = (uint32_t)gs.k - sv.n_less;
+      uint32_t kd = LKEY_INF, ki = LKEY_INF;
+      if (sv.d0 != LKEY_INF) {
+        add(sv.i0, sv.d0);
+        kd = sv.d0;
+        ki = sv.i0;
+      }
+      if (need >= 2u && sv.d1 != LKEY_INF) {
+        add(sv.i1, sv.d1);
+        kd = sv.d1;
+        ki = sv.i1;
+      }
+      for (uint32_t r = 2; r < need; ++r) {
+        Bucket16Visitor bv{sv.Q, kd, ki, LKEY_INF, LKEY_INF};
+        lane_visit<4>(gs, x.x, x.y, x.z, bv);
+        if (bv.best_i == LKEY_INF) break;
+        add(bv.best_i, bv.best_d);
+        kd = bv.best_d;
+        ki = bv.best_i;
+      }
+      if (kd != LKEY_INF) r2 = u2f(kd);
+    }
+    V3f sum = nsum3_get(add.s);
+    V3f sc = v3(gs.scale[0], gs.scale[1], gs.scale[2]);
+    return mul(mul(sum, albedo), sc) * surface_norm(r2, false);
   }
 
   __device__ bool volume_defer(V3f x, V3f wo, V3f w, V3f& c) {

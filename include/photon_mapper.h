@@ -210,7 +210,9 @@ int pm_set_option(pm_context* ctx, int option, int value);
  * weights and Adam state.  Inputs are rows of 9 floats: position, direction,
  * normal (zero for volume queries).  No reference counterpart. */
 uint32_t pm_cache_param_count(void);
-/* (Re)initialise the weights deterministically from `seed`; clears Adam state. */
+/* (Re)initialise the weights deterministically from `seed`; clears Adam state
+ * and restarts the cache render's frame count (which seeds the draws of its
+ * training subsets), so that a reset context repeats the same frames. */
 int pm_cache_reset(pm_context* ctx, uint32_t seed);
 /* Copy the FP32 parameters to (set = 0) or from (set = 1) host memory; the
  * flat layout is, per layer l = 0..4, W_l [nout x kin] row-major then b_l,

@@ -3,6 +3,9 @@
 * With the fallback threshold at 0 every query is gathered exactly, so the
   pipeline must reproduce the plain render (1e-4 relative, as the plain
   parity bar): this pins recording, weights and accumulation.
+* The exact values of the training targets and the fallback come from
+  k_values (2-byte heap gather) under gather mode 3 and from k_values7 (the
+  gather of mode 7) under modes 7 and auto: cache frames must be the same bits.
 * After online training on the cache scene, the cache render must stay within
   3% (display RMSE) of the plain render on the same eye samples and add at
   most 15% to the plain render's error against a 64-spp plain render (the
@@ -55,6 +58,29 @@ def test_all_fallback_equals_plain(scene):
     assert s.fallbacks == s.records > 0
     err = np.abs(img.astype(np.float64) - ref)
     assert (err <= 1e-4 * np.abs(ref) + 1e-6).all(), float(err.max())
+
+
+def test_value_kernels_agree(scene):
+    say("case: cache frames with the exact values of gather mode 3 (k_values) and mode 7 (k_values7)")
+    g = scene
+    imgs, stats = [], []
+    try:
+        for mode in (3, 7):
+            g.set_option(pmamd.PM_OPT_GATHER_MODE, mode)
+            g.cache_reset(1)
+            g.cache_configure(threshold=0.05, train_samples=16384, batch=4096, march_stride=4)
+            for _ in range(3):
+                img = g.render_cache()
+            imgs.append(img)
+            stats.append(g.cache_stats())
+    finally:
+        g.set_option(pmamd.PM_OPT_GATHER_MODE, 5)
+    say("fallbacks %d / %d of %d records" % (stats[0].fallbacks, stats[1].fallbacks, stats[0].records))
+    # training targets and fallback values are the same bits, so is everything after them
+    assert 0 < stats[0].fallbacks < stats[0].records
+    assert stats[0].fallbacks == stats[1].fallbacks
+    assert stats[0].train_loss == stats[1].train_loss
+    assert np.array_equal(imgs[0], imgs[1])
 
 
 def test_cache_accuracy_cache_scene():
