@@ -159,8 +159,8 @@ def run_batch(steps, tune_stride, n_scenes=64, seed=1):
     t = time.perf_counter()
     best, tune_ms = b.autotune(tune_stride)
     t_tune = time.perf_counter() - t
-    log("batch: %d scenes, setup %.1f s, autotune %.1f s -> leaf %d, block %d, heap slots %d, key bytes %d"
-        % (n_scenes, t_setup, t_tune, best[0], best[1], best[2], best[3]))
+    log("batch: %d scenes, setup %.1f s, autotune %.1f s -> leaf %d, block %d, heap slots %d, key bytes %d, "
+        "gather mode %d" % (n_scenes, t_setup, t_tune, best[0], best[1], best[2], best[3], best[4]))
     b.submit()
     per = []
     for i in range(steps):
@@ -172,7 +172,8 @@ def run_batch(steps, tune_stride, n_scenes=64, seed=1):
     ms = float(np.median(per))
     b.close()
     alone = 0.0
-    mode = {8: 1, 4: 2, 2: 4}[best[3]]  # the one-scene gather with the same heap keys
+    # the one-scene gather with the same heap keys (or mode 7)
+    mode = 7 if best[4] == 7 else {8: 1, 4: 2, 2: 4}[best[3]]
     for c in cfgs:
         c.leaf_size = best[0]
         g = pmamd.PhotonMapper(c)
@@ -193,7 +194,7 @@ def run_batch(steps, tune_stride, n_scenes=64, seed=1):
         "volume_records": int(st.volume_records), "radiance_queries": int(q),
         "ms_per_frame": round(ms, 3), "radiance_queries_per_s": round(q / (ms * 1e-3), 1),
         "tuning": {"leaf_size": best[0], "block_threads": best[1], "heap_slots": best[2], "heap_key_bytes": best[3],
-                   "autotune_sample_stride": tune_stride, "autotune_s": round(t_tune, 2)},
+                   "gather_mode": best[4], "autotune_sample_stride": tune_stride, "autotune_s": round(t_tune, 2)},
         "graph_nodes": int(st.graph_nodes), "work_items": int(st.work_items),
         "scenes_one_at_a_time_ms": round(alone, 3), "speedup_vs_one_at_a_time": round(alone / ms, 3),
     }

@@ -48,6 +48,11 @@ struct pm_batch {
   std::vector<uint32_t> level_off, tree_nbase, tree_nl, tree_pbase_h, tree_n;
   DevBuf nodes, nrange, point_tree, tree_pbase, tree_root, node_lo, node_hi, node_lc, node_lb, node_child,
       node_pbase, level_nodes, build_tmp;
+  // gather mode 7: node boxes and the tree-ordered payload of every tree
+  std::vector<KdTreeDesc> tree_desc;
+  std::vector<uint32_t> box_level_total;
+  uint32_t box_depth = 0, nleaves = 0;
+  DevBuf bmin, bmax, tpayload, tree_desc_d, leaf_pref_d, level_pref_d;
   // render
   std::vector<BatchRenderScene> rsc;
   std::vector<uint64_t> img_off;
@@ -168,6 +173,7 @@ int size_records(pm_batch* b) {
   HIPCHK(b->z.ensure(4 * N));
   HIPCHK(b->payload.ensure(stride * N));
   HIPCHK(b->tpts.ensure(16 * (N + kTptsPad)));
+  HIPCHK(b->tpayload.ensure(stride * N));
   for (int v = 0; v < 2; ++v) {
     DevMap& m = v ? b->mv : b->ms;
     memset(&m, 0, sizeof(m));
@@ -194,6 +200,9 @@ GatherMap tree_gather_map(const pm_batch* b, int scene, int vol, uint32_t tree) 
   g.nodes = b->nodes.as<KdNode>() + b->tree_nbase[tree];
   g.nrange = b->nrange.as<uint2>() + b->tree_nbase[tree];
   g.payload = (const char*)b->payload.p + stride * b->tree_pbase_h[tree];
+  g.bmin = b->bmin.as<float4>() + b->tree_desc[tree].bbase;
+  g.bmax = b->bmax.as<float4>() + b->tree_desc[tree].bbase;
+  g.tpayload = (const char*)b->tpayload.p + stride * b->tree_pbase_h[tree];
   g.n = b->tree_n[tree];
   g.nl = b->tree_nl[tree];
   g.bf16 = b->bf16;
@@ -276,6 +285,48 @@ int setup_forest(pm_batch* b, int L) {
   for (uint32_t t = 0; t < nt; ++t)
     HIPCHK(launch_fill_u32(b->point_tree.as<uint32_t>() + b->tree_pbase_h[t], t, b->tree_n[t], b->stream));
   HIPCHK(b->build_tmp.ensure(4 * kdforest_temp_words((uint32_t)N, nn)));
+  // node boxes (gather mode 7): 2 nl - 1 per tree, leaves and internal levels
+  b->tree_desc.resize(nt);
+  std::vector<uint32_t> lpref(nt + 1, 0u);
+  uint32_t nbox = 0, bdepth = 0;
+  for (uint32_t t = 0; t < nt; ++t) {
+    KdTreeDesc& D = b->tree_desc[t];
+    D.n = b->tree_n[t];
+    D.nl = b->tree_nl[t];
+    D.nbase = b->tree_nbase[t];
+    D.pbase = b->tree_pbase_h[t];
+    D.bbase = nbox;
+    D.pad = 0;
+    nbox += D.nl ? 2u * D.nl - 1u : 0u;
+    lpref[t + 1] = lpref[t] + D.nl;
+    const uint32_t ni = D.nl ? D.nl - 1u : 0u;
+    uint32_t d = 0;
+    while (d < 32u && (1ull << d) - 1ull < ni) ++d;  // levels 0 .. d - 1 hold internal nodes
+    bdepth = std::max(bdepth, d);
+  }
+  std::vector<uint32_t> lvpref((size_t)bdepth * (nt + 1), 0u);
+  b->box_level_total.assign(bdepth, 0u);
+  for (uint32_t d = 0; d < bdepth; ++d) {
+    uint32_t* row = lvpref.data() + (size_t)d * (nt + 1);
+    for (uint32_t t = 0; t < nt; ++t) {
+      const uint32_t ni = b->tree_desc[t].nl ? b->tree_desc[t].nl - 1u : 0u;
+      const uint64_t first = (1ull << d) - 1ull, last = std::min<uint64_t>((2ull << d) - 2ull, (uint64_t)ni - 1ull);
+      const uint32_t c = ni > first ? (uint32_t)(last - first + 1ull) : 0u;
+      row[t + 1] = row[t] + c;
+    }
+    b->box_level_total[d] = row[nt];
+  }
+  b->box_depth = bdepth;
+  b->nleaves = lpref[nt];
+  HIPCHK(b->bmin.ensure(16ull * std::max<uint32_t>(nbox, 1)));
+  HIPCHK(b->bmax.ensure(16ull * std::max<uint32_t>(nbox, 1)));
+  HIPCHK(b->tree_desc_d.ensure(sizeof(KdTreeDesc) * nt));
+  HIPCHK(hipMemcpy(b->tree_desc_d.p, b->tree_desc.data(), sizeof(KdTreeDesc) * nt, hipMemcpyHostToDevice));
+  HIPCHK(b->leaf_pref_d.ensure(4ull * (nt + 1)));
+  HIPCHK(hipMemcpy(b->leaf_pref_d.p, lpref.data(), 4ull * (nt + 1), hipMemcpyHostToDevice));
+  HIPCHK(b->level_pref_d.ensure(4ull * std::max<size_t>(lvpref.size(), 1)));
+  if (!lvpref.empty())
+    HIPCHK(hipMemcpy(b->level_pref_d.p, lvpref.data(), 4ull * lvpref.size(), hipMemcpyHostToDevice));
   HIPCHK(hipStreamSynchronize(b->stream));
   b->L = L;
   // per-scene render descriptors (the gather maps point into the forest)
@@ -332,7 +383,7 @@ int setup_render(pm_batch* b) {
 
 int set_grid(pm_batch* b) {
   const int occ = render_batch_occupancy(b->fp64, b->cone, b->tune.block_threads, b->tune.heap_slots,
-                                         b->tune.heap_key_bytes);
+                                         b->tune.heap_key_bytes, b->tune.gather_mode == 7);
   if (occ < 1) return PM_ERR_INVALID;
   b->grid = b->cus * occ;
   return PM_OK;
@@ -375,6 +426,7 @@ hipError_t enqueue_frame(pm_batch* b, const uint2* items, uint32_t nitems, hipEv
   R.counters = b->counters.as<unsigned long long>();
   R.heap_slots = b->tune.heap_slots;
   R.key_bytes = b->tune.heap_key_bytes;
+  R.counting = b->tune.gather_mode == 7;
   hipError_t e;
   if ((e = launch_trace_count_batch(b->tsc_d.as<BatchTraceScene>(), b->block_scene_d.as<uint32_t>(), b->tblocks,
                                     b->cnt_s.as<uint32_t>(), b->cnt_v.as<uint32_t>(), st)) != hipSuccess)
@@ -396,6 +448,26 @@ hipError_t enqueue_frame(pm_batch* b, const uint2* items, uint32_t nitems, hipEv
   if ((e = kd_build_forest(F, b->level_off.data(), (uint32_t)b->level_off.size() - 1u, b->build_tmp.as<uint32_t>(),
                            st)) != hipSuccess)
     return e;
+  {  // node boxes (every gather prunes with them); the tree-ordered payload for mode 7
+    KdForestBoxes fb;
+    fb.tpts = F.tpts;
+    fb.nodes = F.nodes;
+    fb.nrange = F.nrange;
+    fb.trees = b->tree_desc_d.as<KdTreeDesc>();
+    fb.ntrees = F.ntrees;
+    fb.leaf_pref = b->leaf_pref_d.as<uint32_t>();
+    fb.level_pref = b->level_pref_d.as<uint32_t>();
+    fb.level_total = b->box_level_total.data();
+    fb.depth = b->box_depth;
+    fb.nleaves = b->nleaves;
+    fb.bmin = b->bmin.as<float4>();
+    fb.bmax = b->bmax.as<float4>();
+    if ((e = kd_forest_boxes(fb, st)) != hipSuccess) return e;
+    if (R.counting &&
+        (e = kd_forest_payload(F.tpts, F.point_tree, F.tree_pbase, b->payload.p, b->tpayload.p, F.n, b->bf16, st)) !=
+            hipSuccess)
+      return e;
+  }
   if (ev_built && (e = hipEventRecord(ev_built, st)) != hipSuccess) return e;
   if ((e = launch_fill_u32(b->head.as<uint32_t>(), 0u, 4, st)) != hipSuccess) return e;
   if ((e = launch_fill_u32(b->counters.as<uint32_t>(), 0u, 4, st)) != hipSuccess) return e;
@@ -441,6 +513,8 @@ int apply_tuning(pm_batch* b, const pm_batch_tuning& t, bool capture_graph) {
     return PM_ERR_INVALID;
   if ((t.heap_slots != 32 && t.heap_slots != 64) || t.heap_slots < b->kmax) return PM_ERR_INVALID;
   if (t.heap_key_bytes != 8 && t.heap_key_bytes != 4 && t.heap_key_bytes != 2) return PM_ERR_INVALID;
+  if (t.gather_mode != 0 && t.gather_mode != 7) return PM_ERR_INVALID;
+  if (t.gather_mode == 7 && (t.heap_key_bytes != 2 || t.heap_slots != 64)) return PM_ERR_INVALID;
   int r;
   b->tune = t;
   if ((r = set_grid(b)) != PM_OK) return r;
@@ -460,7 +534,8 @@ void destroy(pm_batch* b) {
                     &b->payload, &b->tpts, &b->nodes, &b->nrange, &b->point_tree, &b->tree_pbase, &b->tree_root,
                     &b->node_lo, &b->node_hi, &b->node_lc, &b->node_lb, &b->node_child, &b->node_pbase,
                     &b->level_nodes, &b->build_tmp, &b->rsc_d, &b->items_d, &b->items_tune_d, &b->head, &b->rgb,
-                    &b->counters})
+                    &b->counters, &b->bmin, &b->bmax, &b->tpayload, &b->tree_desc_d, &b->leaf_pref_d,
+                    &b->level_pref_d})
     d->release();
   if (b->e0) (void)hipEventDestroy(b->e0);
   if (b->e1) (void)hipEventDestroy(b->e1);
@@ -531,6 +606,7 @@ int pm_batch_create(const pm_config* cfgs, int n, pm_batch** out) {
   t.block_threads = 64;
   t.heap_slots = b->kmax <= 32 ? 32 : 64;
   t.heap_key_bytes = 2;
+  t.gather_mode = 0;
   int r = setup_scenes(b);
   if (r == PM_OK) r = size_records(b);
   if (r == PM_OK) r = setup_render(b);
@@ -613,9 +689,18 @@ int pm_batch_autotune(pm_batch* b, int sample_stride, pm_batch_tuning* best_out,
       pm_batch_tuning t = start;
       t.heap_key_bytes = kb;
       t.heap_slots = sl;
+      t.gather_mode = 0;
       int r = consider(t);
       if (r != PM_OK) return r;
     }
+  {  // gather mode 7 (2-byte keys, 64 slots of LDS for its counters)
+    pm_batch_tuning t = start;
+    t.heap_key_bytes = 2;
+    t.heap_slots = 64;
+    t.gather_mode = 7;
+    int r = consider(t);
+    if (r != PM_OK) return r;
+  }
   const pm_batch_tuning s1 = best;
   for (int L : leaves) {
     if (L == s1.leaf_size) continue;

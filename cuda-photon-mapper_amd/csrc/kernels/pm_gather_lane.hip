@@ -451,11 +451,13 @@ hipError_t launch_render_lane(const RenderLaunch& r, hipStream_t st) {
 // queue (an atomic counter over an item list ordered by decreasing expected
 // cost) until the list is exhausted, so scenes of very different sizes keep
 // all waves busy.  Per pixel the arithmetic is that of the one-scene lane
-// render with the same heap keys (KB = 8: mode 1, 4: mode 2, 2: mode 4), so
-// every scene's image is bit-identical to its own render.  heap_slots (32 or
-// 64 keys per lane) and KB size each wave's LDS heap.
-template <bool FP64T, bool CONE, int KB>
+// render with the same gather (G = 8: mode 1, 4: mode 2, 2: mode 4, 7: mode 7
+// with 2-byte keys), so every scene's image is bit-identical to its own render
+// in that mode.  heap_slots (32 or 64 keys per lane) and the key width size
+// each wave's LDS heap; mode 7 counts in the same 8 KB (64 slots).
+template <bool FP64T, bool CONE, int G>
 __global__ void __launch_bounds__(256) k_render_batch(BatchRenderLaunch r) {
+  constexpr int KB = G == 7 ? 2 : G;
   extern __shared__ uint32_t lds[];
   const int lane = lane_id();
   const int wave = (int)(threadIdx.x >> 6);
@@ -482,6 +484,9 @@ __global__ void __launch_bounds__(256) k_render_batch(BatchRenderLaunch r) {
                                     nv);
       else if (KB == 4)
         v = lane_pixel2<FP64T, CONE>(S.sc, S.rp, S.gs, S.gv, heap, lane, (uint32_t)lx, (uint32_t)ly, ns, nv);
+      else if (G == 7)
+        v = lane_pixel4<FP64T, CONE, LaneGather5<CONE>>(S.sc, S.rp, S.gs, S.gv, (uint16_t*)heap, lane, (uint32_t)lx,
+                                                        (uint32_t)ly, ns, nv);
       else
         v = lane_pixel3<FP64T, CONE, 4>(S.sc, S.rp, S.gs, S.gv, (uint16_t*)heap, lane, (uint32_t)lx, (uint32_t)ly, ns,
                                         nv);
@@ -500,10 +505,13 @@ size_t render_batch_lds_bytes(int block_threads, int heap_slots, int key_bytes) 
   return (size_t)(block_threads / 64) * (size_t)heap_slots * 64u * (size_t)key_bytes;
 }
 
+// Gather kind of a launch: the heap key width, or 7 for mode 7.
+static int batch_gather(const BatchRenderLaunch& r) { return r.counting ? 7 : r.key_bytes; }
+
 template <int KB>
 static hipError_t launch_render_batch_kb(const BatchRenderLaunch& r, int fp64, int cone, int block_threads, int grid,
                                          hipStream_t st) {
-  const size_t lds = render_batch_lds_bytes(block_threads, r.heap_slots, KB);
+  const size_t lds = render_batch_lds_bytes(block_threads, r.heap_slots, KB == 7 ? 2 : KB);
   if (fp64 && cone) hipLaunchKernelGGL((k_render_batch<true, true, KB>), dim3(grid), dim3(block_threads), lds, st, r);
   else if (fp64) hipLaunchKernelGGL((k_render_batch<true, false, KB>), dim3(grid), dim3(block_threads), lds, st, r);
   else if (cone) hipLaunchKernelGGL((k_render_batch<false, true, KB>), dim3(grid), dim3(block_threads), lds, st, r);
@@ -514,8 +522,11 @@ static hipError_t launch_render_batch_kb(const BatchRenderLaunch& r, int fp64, i
 hipError_t launch_render_batch(const BatchRenderLaunch& r, int fp64, int cone, int block_threads, int grid,
                                hipStream_t st) {
   if (r.nitems == 0) return hipSuccess;
-  if (r.key_bytes == 8) return launch_render_batch_kb<8>(r, fp64, cone, block_threads, grid, st);
-  if (r.key_bytes == 4) return launch_render_batch_kb<4>(r, fp64, cone, block_threads, grid, st);
+  const int G = batch_gather(r);
+  if (G == 7 && (r.key_bytes != 2 || r.heap_slots != 64)) return hipErrorInvalidValue;
+  if (G == 7) return launch_render_batch_kb<7>(r, fp64, cone, block_threads, grid, st);
+  if (G == 8) return launch_render_batch_kb<8>(r, fp64, cone, block_threads, grid, st);
+  if (G == 4) return launch_render_batch_kb<4>(r, fp64, cone, block_threads, grid, st);
   return launch_render_batch_kb<2>(r, fp64, cone, block_threads, grid, st);
 }
 
@@ -534,8 +545,9 @@ static int occupancy_kb(int fp64, int cone, int block_threads, size_t lds) {
   return e == hipSuccess ? nb : 0;
 }
 
-int render_batch_occupancy(int fp64, int cone, int block_threads, int heap_slots, int key_bytes) {
+int render_batch_occupancy(int fp64, int cone, int block_threads, int heap_slots, int key_bytes, int counting) {
   const size_t lds = render_batch_lds_bytes(block_threads, heap_slots, key_bytes);
+  if (counting) return occupancy_kb<7>(fp64, cone, block_threads, lds);
   if (key_bytes == 8) return occupancy_kb<8>(fp64, cone, block_threads, lds);
   if (key_bytes == 4) return occupancy_kb<4>(fp64, cone, block_threads, lds);
   return occupancy_kb<2>(fp64, cone, block_threads, lds);

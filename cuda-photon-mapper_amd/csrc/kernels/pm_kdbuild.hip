@@ -424,4 +424,94 @@ hipError_t kd_tree_payload(const DevMap& m, hipStream_t st) {
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- forests
+// Tree t of entry i, for a prefix table with pref[t] <= i < pref[t + 1].
+__device__ __forceinline__ uint32_t forest_tree_of(const uint32_t* pref, uint32_t nt, uint32_t i) {
+  uint32_t lo = 0u, hi = nt;
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (pref[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void box_of_points(const float4* tpts, uint32_t lo, uint32_t hi, float4& mn, float4& mx) {
+  mn = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+  mx = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+  for (uint32_t j = lo; j < hi; ++j) {
+    const float4 q = tpts[j];
+    mn.x = fminf(mn.x, q.x); mn.y = fminf(mn.y, q.y); mn.z = fminf(mn.z, q.z);
+    mx.x = fmaxf(mx.x, q.x); mx.y = fmaxf(mx.y, q.y); mx.z = fmaxf(mx.z, q.z);
+  }
+}
+
+// Leaf i of the forest: k_leaf_boxes in its own tree.
+__global__ void k_forest_leaf_boxes(KdForestBoxes f) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= f.nleaves) return;
+  const uint32_t t = forest_tree_of(f.leaf_pref, f.ntrees, i);
+  const KdTreeDesc D = f.trees[t];
+  const uint32_t h = D.nl - 1u + (i - f.leaf_pref[t]);
+  uint32_t lo = 0u, hi = D.n;
+  if (h > 0u) {
+    const uint32_t p = (h - 1u) >> 1;
+    const uint2 r = f.nrange[D.nbase + p];
+    const uint32_t mid = r.x + kd_left_count(f.nodes[D.nbase + p].meta);
+    if (h == 2u * p + 1u) {
+      lo = r.x;
+      hi = mid;
+    } else {
+      lo = mid;
+      hi = r.y;
+    }
+  }
+  float4 mn, mx;
+  box_of_points(f.tpts + D.pbase, lo, hi, mn, mx);
+  f.bmin[D.bbase + h] = mn;
+  f.bmax[D.bbase + h] = mx;
+}
+
+// Internal nodes at level d (heap nodes 2^d - 1 ..) of every tree.
+__global__ void k_forest_union_boxes(KdForestBoxes f, uint32_t d, uint32_t count) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t* pref = f.level_pref + (size_t)d * (f.ntrees + 1u);
+  const uint32_t t = forest_tree_of(pref, f.ntrees, i);
+  const uint32_t bb = f.trees[t].bbase;
+  const uint32_t h = (1u << d) - 1u + (i - pref[t]), l = 2u * h + 1u, r = 2u * h + 2u;
+  const float4 a = f.bmin[bb + l], b = f.bmin[bb + r], c = f.bmax[bb + l], e = f.bmax[bb + r];
+  f.bmin[bb + h] = make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), 0.0f);
+  f.bmax[bb + h] = make_float4(fmaxf(c.x, e.x), fmaxf(c.y, e.y), fmaxf(c.z, e.z), 0.0f);
+}
+
+hipError_t kd_forest_boxes(const KdForestBoxes& f, hipStream_t st) {
+  if (f.nleaves == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_forest_leaf_boxes, dim3((f.nleaves + 255u) / 256u), dim3(256), 0, st, f);
+  hipError_t e = hipGetLastError();
+  for (uint32_t d = f.depth; d-- > 0u && e == hipSuccess;) {
+    const uint32_t count = f.level_total[d];
+    if (count == 0) continue;
+    hipLaunchKernelGGL(k_forest_union_boxes, dim3((count + 255u) / 256u), dim3(256), 0, st, f, d, count);
+    e = hipGetLastError();
+  }
+  return e;
+}
+
+__global__ void k_forest_payload(const float4* tpts, const uint32_t* point_tree, const uint32_t* tree_pbase,
+                                 const uint4* payload, uint4* tpayload, uint32_t n, int words) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t r = tree_pbase[point_tree[i]] + __float_as_uint(tpts[i].w);
+  for (int w = 0; w < words; ++w) tpayload[(size_t)words * i + w] = payload[(size_t)words * r + w];
+}
+
+hipError_t kd_forest_payload(const float4* tpts, const uint32_t* point_tree, const uint32_t* tree_pbase,
+                             const void* payload, void* tpayload, uint32_t n, int bf16, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_forest_payload, dim3((n + 255u) / 256u), dim3(256), 0, st, tpts, point_tree, tree_pbase,
+                     (const uint4*)payload, (uint4*)tpayload, n, bf16 ? 1 : 2);
+  return hipGetLastError();
+}
+
 }  // namespace pm

@@ -119,6 +119,32 @@ hipError_t kd_build_impl(DevMap& m, uint32_t* temp, const uint32_t* host_lo, con
 hipError_t kd_node_boxes(const DevMap& m, hipStream_t st);
 // The payload in tree order into m.tpayload (when allocated).
 hipError_t kd_tree_payload(const DevMap& m, hipStream_t st);
+// The same two for a forest (trees laid out one after another).
+struct KdTreeDesc {
+  uint32_t n, nl;  // points, leaves
+  uint32_t nbase;  // first internal node in the forest's node arrays
+  uint32_t pbase;  // first point
+  uint32_t bbase;  // first box: a tree has 2 nl - 1 boxes, heap nodes 0 .. 2 nl - 2
+  uint32_t pad;
+};
+struct KdForestBoxes {
+  const float4* tpts;
+  const KdNode* nodes;
+  const uint2* nrange;           // tree-local point ranges
+  const KdTreeDesc* trees;
+  uint32_t ntrees;
+  const uint32_t* leaf_pref;     // [ntrees + 1] leaves of the earlier trees
+  const uint32_t* level_pref;    // [depth][ntrees + 1] internal nodes at level d of the earlier trees
+  const uint32_t* level_total;   // host: [depth] internal nodes at level d of all trees
+  uint32_t depth;                // internal levels of the deepest tree
+  uint32_t nleaves;
+  float4* bmin;
+  float4* bmax;
+};
+hipError_t kd_forest_boxes(const KdForestBoxes& f, hipStream_t st);
+// tpayload[i] = payload[tree_pbase[point_tree[i]] + tree-local index of tpts[i]]
+hipError_t kd_forest_payload(const float4* tpts, const uint32_t* point_tree, const uint32_t* tree_pbase,
+                             const void* payload, void* tpayload, uint32_t n, int bf16, hipStream_t st);
 
 // gather (pm_gather.hip)
 hipError_t launch_knn(GatherMap g, const float* q, uint32_t nq, uint32_t* idx, float* d2, uint32_t* count,
@@ -183,9 +209,11 @@ struct BatchRenderLaunch {
   unsigned long long* counters;    // [surface queries, volume queries]
   int heap_slots;                  // LDS heap keys per lane (>= every scene's k)
   int key_bytes;                   // heap key width: 8, 4 or 2 (gather modes 1, 2, 4)
+  int counting;                    // 1: gather mode 7 (key_bytes 2, heap_slots 64; maps with boxes and tpayload)
 };
 size_t render_batch_lds_bytes(int block_threads, int heap_slots, int key_bytes);
-int render_batch_occupancy(int fp64, int cone, int block_threads, int heap_slots, int key_bytes);  // per CU
+int render_batch_occupancy(int fp64, int cone, int block_threads, int heap_slots, int key_bytes,
+                           int counting);  // per CU
 hipError_t launch_tonemap(const float* rgb, size_t npix, float exposure, uchar4* out, hipStream_t st);
 hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t n, uint32_t* dst, hipStream_t st);
 hipError_t launch_render_batch(const BatchRenderLaunch& r, int fp64, int cone, int block_threads, int grid,

@@ -61,8 +61,21 @@ struct LaneHeap {
   }
 };
 
+// Squared distance from q to a node's tight box, as a lower bound of the
+// squared distance to any of its points: with the operation order of dist2,
+// and FP32 rounding monotonic, no point in the box has a smaller computed d^2.
+__device__ __forceinline__ float box_d2(V3f q, float4 mn, float4 mx) {
+  const float bx = fmaxf(fmaxf(mn.x - q.x, q.x - mx.x), 0.0f);
+  const float by = fmaxf(fmaxf(mn.y - q.y, q.y - mx.y), 0.0f);
+  const float bz = fmaxf(fmaxf(mn.z - q.z, q.z - mx.z), 0.0f);
+  return (bx * bx + by * by) + bz * bz;
+}
+
 // Exact kNN of q for this lane; the heap holds the result (unordered), its
-// size is returned.  bound_h: see wave_knn in pm_gather.hip.
+// size is returned.  bound_h: see wave_knn in pm_gather.hip.  When the map has
+// node boxes, a far child or a leaf whose box lies beyond the current bound is
+// skipped: none of its points can enter the heap (box_d2 is a lower bound of
+// their d^2, and the heap takes (d^2, index) keys below its root).
 __device__ __forceinline__ int lane_knn(const GatherMap& g, float qx, float qy, float qz, uint32_t bound_h,
                                         LaneHeap& hp) {
   hp.size = 0;
@@ -105,7 +118,9 @@ __device__ __forceinline__ int lane_knn(const GatherMap& g, float qx, float qy, 
       float qa = a == 0 ? qx : (a == 1 ? qy : qz);
       float diff = qa - nd.split;
       uint32_t nearc = diff < 0.0f ? 2u * p + 1u : 2u * p + 2u;
-      if (from == nearc && diff * diff <= u2f(wh)) {
+      const uint32_t farc = nearc == 2u * p + 1u ? 2u * p + 2u : 2u * p + 1u;
+      if (from == nearc && diff * diff <= u2f(wh) &&
+          (!g.bmin || box_d2(v3(qx, qy, qz), g.bmin[farc], g.bmax[farc]) <= u2f(wh))) {
         uint2 r = g.nrange[p];
         uint32_t mid = r.x + kd_left_count(nd.meta);
         if (nearc == 2u * p + 1u) {
@@ -127,7 +142,8 @@ __device__ __forceinline__ int lane_knn(const GatherMap& g, float qx, float qy, 
     if (!at_leaf) continue;
     // leaf [lo, hi), at most 64 points, in batches of 8: all 8 loads are
     // issued before any is used, then the passing keys are inserted.
-    for (uint32_t b0 = lo; b0 < hi; b0 += 8u) {
+    const bool reach = !g.bmin || box_d2(v3(qx, qy, qz), g.bmin[h], g.bmax[h]) <= u2f(wh);
+    for (uint32_t b0 = lo; reach && b0 < hi; b0 += 8u) {
       float4 p[8];
       const float4* q = g.tpts + b0;  // padded: entries past hi are read, not used
 #pragma unroll

@@ -180,16 +180,6 @@ struct Sum16CVisitor : Sum16Visitor<Add> {
   }
 };
 
-// Squared distance from q to a node's tight box, as a lower bound of the
-// squared distance to any of its points: with the operation order of dist2,
-// and FP32 rounding monotonic, no point in the box has a smaller computed d^2.
-__device__ __forceinline__ float box_d2(V3f q, float4 mn, float4 mx) {
-  const float bx = fmaxf(fmaxf(mn.x - q.x, q.x - mx.x), 0.0f);
-  const float by = fmaxf(fmaxf(mn.y - q.y, q.y - mx.y), 0.0f);
-  const float bz = fmaxf(fmaxf(mn.z - q.z, q.z - mx.z), 0.0f);
-  return (bx * bx + by * by) + bz * bz;
-}
-
 // lane_visit_dual of pm_lane_knn4.h for visitors whose far-child test does not
 // change during the traversal (fixed search bounds), with speculative
 // traversal (T. Aila and S. Laine, "Understanding the Efficiency of Ray

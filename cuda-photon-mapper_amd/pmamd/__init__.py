@@ -130,6 +130,7 @@ class PmBatchTuning(ctypes.Structure):
         ("block_threads", ctypes.c_int),
         ("heap_slots", ctypes.c_int),
         ("heap_key_bytes", ctypes.c_int),
+        ("gather_mode", ctypes.c_int),
     ]
 
 
@@ -472,15 +473,15 @@ class Batch:
         except Exception:
             pass
 
-    def set_tuning(self, leaf_size, block_threads, heap_slots, heap_key_bytes=2):
-        t = PmBatchTuning(leaf_size, block_threads, heap_slots, heap_key_bytes)
+    def set_tuning(self, leaf_size, block_threads, heap_slots, heap_key_bytes=2, gather_mode=0):
+        t = PmBatchTuning(leaf_size, block_threads, heap_slots, heap_key_bytes, gather_mode)
         check(load().pm_batch_set_tuning(self.h, ctypes.byref(t)), "pm_batch_set_tuning")
 
     def autotune(self, sample_stride=16):
         t = PmBatchTuning()
         ms = ctypes.c_double()
         check(load().pm_batch_autotune(self.h, sample_stride, ctypes.byref(t), ctypes.byref(ms)), "pm_batch_autotune")
-        return (t.leaf_size, t.block_threads, t.heap_slots, t.heap_key_bytes), ms.value
+        return (t.leaf_size, t.block_threads, t.heap_slots, t.heap_key_bytes, t.gather_mode), ms.value
 
     def submit(self):
         check(load().pm_batch_submit(self.h), "pm_batch_submit")

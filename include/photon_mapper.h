@@ -275,6 +275,8 @@ typedef struct pm_batch_tuning {
   int block_threads;  /* render workgroup size: 64, 128, 192 or 256 */
   int heap_slots;     /* LDS heap keys per lane: 32 or 64 (>= every scene's k) */
   int heap_key_bytes; /* heap key width: 8 (d^2, index), 4 (d^2) or 2 (d^2 bucket); see PM_OPT_GATHER_MODE */
+  int gather_mode;    /* 0: the heap gather of heap_key_bytes (modes 1, 2, 4); 7: gather mode 7, which needs
+                         heap_key_bytes 2 and heap_slots 64 (its bucket counters take that LDS) */
 } pm_batch_tuning;
 
 typedef struct pm_batch_stats {
@@ -302,9 +304,9 @@ int pm_batch_destroy(pm_batch* b);
 int pm_batch_set_tuning(pm_batch* b, const pm_batch_tuning* t);
 /* Times the kd-build and a 1-in-`sample_stride` subset of the render work
  * items for a staged sweep: heap key width {8, 4, 2} x heap slots {32 (when
- * every k <= 32), 64}, then leaf size {4, 8, 16, 32}, then workgroup size
- * {64, 128, 256}, each stage from the best so far; keeps the fastest and
- * returns it (and its time). */
+ * every k <= 32), 64} and gather mode 7, then leaf size {4, 8, 16, 32}, then
+ * workgroup size {64, 128, 256}, each stage from the best so far; keeps the
+ * fastest and returns it (and its time). */
 int pm_batch_autotune(pm_batch* b, int sample_stride, pm_batch_tuning* best_out, double* best_ms_out);
 /* One frame of every scene (one graph launch); blocking. */
 int pm_batch_submit(pm_batch* b);

@@ -56,10 +56,10 @@ def batch():
 MODE_OF_KEY_BYTES = {8: 1, 4: 2, 2: 4}  # gather mode of the one-scene render with the same heap keys
 
 
-def run_alone(c, leaf, key_bytes=2):
+def run_alone(c, leaf, key_bytes=2, gather_mode=0):
     c.leaf_size = leaf
     g = pmamd.PhotonMapper(c)
-    g.set_option(pmamd.PM_OPT_GATHER_MODE, MODE_OF_KEY_BYTES[key_bytes])
+    g.set_option(pmamd.PM_OPT_GATHER_MODE, 7 if gather_mode == 7 else MODE_OF_KEY_BYTES[key_bytes])
     g.emit()
     g.build()
     return g, g.render()
@@ -70,11 +70,12 @@ def test_batch_equals_scenes_run_alone(batch):
     st = b.stats()
     say("graph nodes %d, work items %d, tuning %s" % (st.graph_nodes, st.work_items,
                                                       (st.tuning.leaf_size, st.tuning.block_threads,
-                                                       st.tuning.heap_slots, st.tuning.heap_key_bytes)))
+                                                       st.tuning.heap_slots, st.tuning.heap_key_bytes,
+                                                       st.tuning.gather_mode)))
     L = st.tuning.leaf_size
     tot_q = 0
     for i, c in enumerate(cfgs):
-        g, img = run_alone(c, L, st.tuning.heap_key_bytes)
+        g, img = run_alone(c, L, st.tuning.heap_key_bytes, st.tuning.gather_mode)
         s1, s2 = g.stats(), b.scene_stats(i)
         assert (s1.n_surface, s1.n_volume) == (s2.n_surface, s2.n_volume), i
         for name in ("absorbed", "escaped", "rr_gain"):
@@ -99,7 +100,8 @@ def assert_same_radiance(ref, img, bitwise, what):
         np.testing.assert_allclose(img, ref, rtol=1e-5, atol=1e-7, err_msg=what)
 
 
-@pytest.mark.parametrize("tuning", [(32, 256, 32, 2), (32, 128, 64, 8), (4, 256, 32, 4), (16, 192, 64, 2)])
+@pytest.mark.parametrize("tuning", [(32, 256, 32, 2, 0), (32, 128, 64, 8, 0), (4, 256, 32, 4, 0),
+                                    (16, 192, 64, 2, 0), (32, 64, 64, 2, 7), (8, 256, 64, 2, 7)])
 def test_batch_tuning_invariance(batch, tuning):
     cfgs, b = batch
     b.set_tuning(32, 64, 32, 2)
@@ -109,8 +111,8 @@ def test_batch_tuning_invariance(batch, tuning):
     b.submit()
     for i in range(len(cfgs)):
         assert_same_radiance(ref[i], b.image(i), tuning[0] == 32, "scene %d under tuning %s" % (i, tuning))
-    # with the same leaf size and heap keys the batch matches the scene's own run bit for bit
-    g, img = run_alone(cfgs[3], tuning[0], tuning[3])
+    # with the same leaf size and gather the batch matches the scene's own run bit for bit
+    g, img = run_alone(cfgs[3], tuning[0], tuning[3], tuning[4])
     assert np.array_equal(bits(img), bits(b.image(3)))
     gp, gs, ga = g.tree(pmamd.PM_MAP_SURFACE)
     bp, bs, ba = b.tree(3, pmamd.PM_MAP_SURFACE, tuning[0])
@@ -123,9 +125,11 @@ def test_batch_autotune(batch):
     b.submit()
     ref = [b.image(i) for i in range(len(cfgs))]
     best, ms = b.autotune(sample_stride=3)
-    say("autotune: leaf %d, block %d, heap slots %d, key bytes %d (%.2f ms on the sampled frame)" % (best + (ms,)))
+    say("autotune: leaf %d, block %d, heap slots %d, key bytes %d, gather mode %d (%.2f ms on the sampled frame)"
+        % (best + (ms,)))
     assert best[0] in (4, 8, 16, 32) and best[1] in (64, 128, 256) and best[2] in (32, 64) and ms > 0
-    assert best[3] in (8, 4, 2)
+    assert best[3] in (8, 4, 2) and best[4] in (0, 7)
+    assert best[4] == 0 or (best[3], best[2]) == (2, 64)
     b.submit()
     for i in range(len(cfgs)):
         assert_same_radiance(ref[i], b.image(i), best[0] == 32, "scene %d after autotune" % i)
