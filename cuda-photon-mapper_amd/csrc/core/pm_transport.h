@@ -205,6 +205,19 @@ struct DeferredVolume<G, decltype((void)G::kDeferredVolume, void())> {
   static constexpr bool value = G::kDeferredVolume;
 };
 
+// A gather type that only counts its queries (their number depends on the
+// path's geometry, never on its weights) declares
+// `static constexpr bool kCountOnly = true`: the march then skips the
+// transmittance, and passes zero weights.
+template <class G, class = void>
+struct CountOnly {
+  static constexpr bool value = false;
+};
+template <class G>
+struct CountOnly<G, decltype((void)G::kCountOnly, void())> {
+  static constexpr bool value = G::kCountOnly;
+};
+
 // Ray-march of the in-scattered radiance over [t0, t1) along o + t d:
 // samples at t_j = t0 + (j + xi) * step, each weighted by transmittance and
 // the step length.  Homogeneous: T = exp(-sigma_t * (t - t0)).  Density field:
@@ -222,6 +235,10 @@ PM_HD void march_segment(const Scene& sc, const RenderParams& rp, V3<T> o, V3<T>
     T t = t0 + (T(j) + xi) * ds;
     if (!(t < t1)) break;
     V3<T> x = o + d * t;
+    if constexpr (CountOnly<G>::value) {
+      (void)g.volume(x, wo, v3<T>(T(0), T(0), T(0)));
+      continue;
+    }
     T Tj;
     if (m.type == MED_HOMOGENEOUS) {
       Tj = (T)exp_(-st * (TT)(t - t0));
@@ -242,6 +259,7 @@ PM_HD void march_segment(const Scene& sc, const RenderParams& rp, V3<T> o, V3<T>
     V3<T> c;
     if (g.volume_flush(wo, c)) nsum3_add(L, c);
   }
+  if constexpr (CountOnly<G>::value) return;
   T Tseg;
   if (m.type == MED_HOMOGENEOUS) Tseg = (T)exp_(-st * (TT)(t1 - t0));
   else Tseg = (T)exp_(-tau);

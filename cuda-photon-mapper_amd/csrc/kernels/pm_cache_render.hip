@@ -103,8 +103,15 @@ struct RecordGather {
   }
 };
 
-// One lane per crop pixel, 8x8 tiles per wave (same pixel order as the lane render).
-template <bool FP64T>
+// The counting pass: the number of records of a path does not depend on its
+// weights, so the march skips the transmittance.
+struct RecordCount : RecordGather {
+  static constexpr bool kCountOnly = true;
+};
+
+// One lane per crop pixel, 8x8 tiles per wave (same pixel order as the lane
+// render).  COUNT: the counting pass (r.write == 0).
+template <bool FP64T, bool COUNT>
 __global__ void __launch_bounds__(64) k_rec(CacheRecordLaunch r) {
   const int lane = lane_id();
   const int tiles_x = (r.w + 7) / 8;
@@ -112,7 +119,7 @@ __global__ void __launch_bounds__(64) k_rec(CacheRecordLaunch r) {
   const int lx = (tile % tiles_x) * 8 + (lane & 7), ly = (tile / tiles_x) * 8 + (lane >> 3);
   if (lx >= r.w || ly >= r.h) return;
   const uint32_t pidx = (uint32_t)ly * (uint32_t)r.w + (uint32_t)lx;
-  RecordGather g;
+  typename std::conditional<COUNT, RecordCount, RecordGather>::type g;
   g.write = r.write != 0;
   g.n = 0;
   g.base = r.write ? r.offsets[pidx] : 0u;
@@ -330,8 +337,9 @@ __global__ void k_accumulate(const uint32_t* __restrict__ off, const uint32_t* _
 hipError_t launch_cache_record(const CacheRecordLaunch& r, hipStream_t st) {
   unsigned tiles = (unsigned)(((r.w + 7) / 8) * ((r.h + 7) / 8));
   if (tiles == 0) return hipSuccess;
-  if (r.rp.fp64_transmittance) hipLaunchKernelGGL((k_rec<true>), dim3(tiles), dim3(64), 0, st, r);
-  else hipLaunchKernelGGL((k_rec<false>), dim3(tiles), dim3(64), 0, st, r);
+  if (!r.write) hipLaunchKernelGGL((k_rec<false, true>), dim3(tiles), dim3(64), 0, st, r);
+  else if (r.rp.fp64_transmittance) hipLaunchKernelGGL((k_rec<true, false>), dim3(tiles), dim3(64), 0, st, r);
+  else hipLaunchKernelGGL((k_rec<false, false>), dim3(tiles), dim3(64), 0, st, r);
   return hipGetLastError();
 }
 
