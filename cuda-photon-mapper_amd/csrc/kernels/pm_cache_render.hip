@@ -33,7 +33,8 @@ namespace pm {
 // new record.  Within the near-light radius (4 s steps) of the point light,
 // where the in-scattered radiance grows as 1 / distance^2 and no low-order
 // rule over s steps holds, every march point keeps its own record (and is
-// gathered exactly, see k_flags).
+// gathered exactly, see k_flags).  The open volume record stays in registers
+// and is written once, when the next record starts or the pixel ends (flush).
 struct RecordGather {
   bool write;
   uint32_t n;        // records of this lane so far
@@ -44,27 +45,41 @@ struct RecordGather {
   uint32_t* pix;
   int stride;
   float reach2;      // (1.5 * stride * step)^2
-  int open;          // points merged into the open volume record (0: none)
+  bool pending;      // the open volume record is not written yet
+  uint32_t open_i;   // its index among this lane's records
+  int open;          // points merged into the open volume record (0: it takes no more)
   V3f open_x, open_d;
+  V3f open_w;        // sum of the merged points' weights
   float open_sw;     // sum of the merged points' scalar weights (w.x + w.y + w.z)
   V3f open_swx;      // sum of scalar weight x position
   V3f light;
   float near2;       // (4 s steps)^2: no merging closer than this to the light
-  __device__ void put(V3f x, V3f d, V3f nrm, V3f w) {
-    if (write) {
-      size_t i = (size_t)base + n;
-      float* p = in + i * MLP_IN;
-      p[0] = x.x; p[1] = x.y; p[2] = x.z;
-      p[3] = d.x; p[4] = d.y; p[5] = d.z;
-      p[6] = nrm.x; p[7] = nrm.y; p[8] = nrm.z;
-      wt[3 * i] = w.x; wt[3 * i + 1] = w.y; wt[3 * i + 2] = w.z;
-      pix[i] = pixel;
+  __device__ void store(uint32_t k, V3f x, V3f d, V3f nrm, V3f w) {
+    if (!write) return;
+    size_t i = (size_t)base + k;
+    float* p = in + i * MLP_IN;
+    p[0] = x.x; p[1] = x.y; p[2] = x.z;
+    p[3] = d.x; p[4] = d.y; p[5] = d.z;
+    p[6] = nrm.x; p[7] = nrm.y; p[8] = nrm.z;
+    wt[3 * i] = w.x; wt[3 * i + 1] = w.y; wt[3 * i + 2] = w.z;
+    pix[i] = pixel;
+  }
+  // Writes the open volume record: at the weighted centroid of its points
+  // (the first point while there is one, or no weight).
+  __device__ void flush() {
+    if (!pending) return;
+    pending = false;
+    V3f p = open_x;
+    if (open > 1 && open_sw > 0.0f) {
+      const float inv = 1.0f / open_sw;
+      p = v3(open_swx.x * inv, open_swx.y * inv, open_swx.z * inv);
     }
-    ++n;
+    store(open_i, p, open_d, v3(0.0f, 0.0f, 0.0f), open_w);
   }
   __device__ V3f surface(V3f x, V3f nrm, V3f albedo, V3f w, V3f wo) {
+    flush();
     open = 0;
-    put(x, wo, nrm, mul(w, albedo));
+    store(n++, x, wo, nrm, mul(w, albedo));
     return v3(0.0f, 0.0f, 0.0f);
   }
   __device__ V3f volume(V3f x, V3f wo, V3f w) {
@@ -76,27 +91,18 @@ struct RecordGather {
         const float sw = (w.x + w.y) + w.z;
         open_sw = open_sw + sw;
         open_swx = open_swx + x * sw;
-        if (write) {
-          size_t i = (size_t)base + n - 1u;
-          wt[3 * i] += w.x;
-          wt[3 * i + 1] += w.y;
-          wt[3 * i + 2] += w.z;
-          if (open_sw > 0.0f) {
-            const float inv = 1.0f / open_sw;
-            float* p = in + i * MLP_IN;
-            p[0] = open_swx.x * inv;
-            p[1] = open_swx.y * inv;
-            p[2] = open_swx.z * inv;
-          }
-        }
+        open_w = open_w + w;
         ++open;
         return v3(0.0f, 0.0f, 0.0f);
       }
     }
-    put(x, wo, v3(0.0f, 0.0f, 0.0f), w);
+    flush();
+    pending = true;
+    open_i = n++;
     open = near ? 0 : 1;
     open_x = x;
     open_d = wo;
+    open_w = w;
     open_sw = (w.x + w.y) + w.z;
     open_swx = x * open_sw;
     return v3(0.0f, 0.0f, 0.0f);
@@ -130,6 +136,7 @@ __global__ void __launch_bounds__(64) k_rec(CacheRecordLaunch r) {
   g.stride = r.stride > 0 ? r.stride : 1;
   const float reach = 1.5f * (float)g.stride * r.rp.march_step;
   g.reach2 = reach * reach;
+  g.pending = false;
   g.open = 0;
   g.light = v3(r.sc.light[0], r.sc.light[1], r.sc.light[2]);
   g.near2 = r.near2;
@@ -138,6 +145,7 @@ __global__ void __launch_bounds__(64) k_rec(CacheRecordLaunch r) {
     if (FP64T) (void)render_sample_t<float, double>(r.sc, r.rp, px, py, (uint32_t)s, g);
     else (void)render_sample_t<float, float>(r.sc, r.rp, px, py, (uint32_t)s, g);
   }
+  g.flush();
   if (!r.write) r.counts[pidx] = g.n;
 }
 
