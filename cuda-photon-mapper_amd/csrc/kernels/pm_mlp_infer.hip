@@ -223,13 +223,39 @@ __device__ __forceinline__ bf16x8 lds_frag8(const uint16_t* p) {
   return *(const bf16x8*)&v;
 }
 
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// Two FP32 values rounded to BF16 (nearest even) in one v_cvt_pk_bf16_f32:
+// a in the low half.  For finite inputs it equals f32_to_bf16 on each.
+__device__ __forceinline__ uint32_t cvt_pk_bf16(float a, float b) {
+  const bf16x2 r = __builtin_convertvector((f32x2){a, b}, bf16x2);
+  return *(const uint32_t*)&r;
+}
+
 __device__ __forceinline__ bf16x8 pack8(const float* f) {
   uint4 v;
-  v.x = (uint32_t)f32_to_bf16(f[0]) | ((uint32_t)f32_to_bf16(f[1]) << 16);
-  v.y = (uint32_t)f32_to_bf16(f[2]) | ((uint32_t)f32_to_bf16(f[3]) << 16);
-  v.z = (uint32_t)f32_to_bf16(f[4]) | ((uint32_t)f32_to_bf16(f[5]) << 16);
-  v.w = (uint32_t)f32_to_bf16(f[6]) | ((uint32_t)f32_to_bf16(f[7]) << 16);
+  v.x = cvt_pk_bf16(f[0], f[1]);
+  v.y = cvt_pk_bf16(f[2], f[3]);
+  v.z = cvt_pk_bf16(f[4], f[5]);
+  v.w = cvt_pk_bf16(f[6], f[7]);
   return *(const bf16x8*)&v;
+}
+
+// Bias + ReLU of a D fragment's 4 rows of one column, written to the
+// row-major stage as BF16 (two packed conversions, four 2-byte stores).
+__device__ __forceinline__ void stage_col4(uint16_t* stage, int stride, int q, int col, f32x4 acc, float bias) {
+  float v[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const float x = acc[e] + bias;
+    v[e] = x > 0.0f ? x : 0.0f;
+  }
+  const uint32_t p01 = cvt_pk_bf16(v[0], v[1]), p23 = cvt_pk_bf16(v[2], v[3]);
+  stage[(4 * q + 0) * stride + col] = (uint16_t)(p01 & 0xffffu);
+  stage[(4 * q + 1) * stride + col] = (uint16_t)(p01 >> 16);
+  stage[(4 * q + 2) * stride + col] = (uint16_t)(p23 & 0xffffu);
+  stage[(4 * q + 3) * stride + col] = (uint16_t)(p23 >> 16);
 }
 
 static constexpr int STS = MLP_HID + 8;  // stage row stride (bf16), 272 B
@@ -292,12 +318,7 @@ __global__ void __launch_bounds__(IW) k_mlp_l1x(const float* __restrict__ in, ui
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
         const int col = c * 16 + r16;
-        const float bias = b1[col];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          float v = acc[g][c][e] + bias;
-          stage[(4 * q + e) * STS + col] = f32_to_bf16(v > 0.0f ? v : 0.0f);
-        }
+        stage_col4(stage, STS, q, col, acc[g][c], b1[col]);
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       __builtin_amdgcn_wave_barrier();
@@ -370,12 +391,7 @@ __global__ void __launch_bounds__(IW25) k_mlp_l25x(L25Args A) {
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
           const int col = c * 16 + r16;
-          const float bias = A.b[l][col];
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            float v = acc[g][c][e] + bias;
-            stage[(4 * q + e) * WHS + col] = f32_to_bf16(v > 0.0f ? v : 0.0f);
-          }
+          stage_col4(stage, WHS, q, col, acc[g][c], A.b[l][col]);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
