@@ -337,9 +337,10 @@ __global__ void __launch_bounds__(IW) k_mlp_l1x(const float* __restrict__ in, ui
   }
 }
 
-// Layers 2..5 with K = 32 steps: 8 waves share the weights (2 per SIMD); each
-// wave converts its 32-row activations D -> A through a 16-row LDS stage.
-static constexpr int IW25 = 512;
+// Layers 2..5 with K = 32 steps: 12 waves share the weights (3 per SIMD; the
+// weights and the 12 stages take 157 KB of the 160 KB of LDS); each wave
+// converts its 32-row activations D -> A through a 16-row LDS stage.
+static constexpr int IW25 = 768;
 
 __global__ void __launch_bounds__(IW25) k_mlp_l25x(L25Args A) {
   extern __shared__ uint16_t lds[];
