@@ -261,13 +261,15 @@ __device__ __forceinline__ void stage_col4(uint16_t* stage, int stride, int q, i
 static constexpr int STS = MLP_HID + 8;  // stage row stride (bf16), 272 B
 
 // Layer 1: features on the fly, W1 in LDS, H1 written through a per-wave
-// 16-row stage as 16-byte row segments.
-__global__ void __launch_bounds__(IW) k_mlp_l1x(const float* __restrict__ in, uint32_t rows,
-                                                 const uint16_t* __restrict__ w1, const float* __restrict__ b1,
-                                                 uint16_t* __restrict__ h1) {
+// 16-row stage as 16-byte row segments.  12 waves share W1 (3 per SIMD).
+static constexpr int IW1X = 768;
+
+__global__ void __launch_bounds__(IW1X) k_mlp_l1x(const float* __restrict__ in, uint32_t rows,
+                                                   const uint16_t* __restrict__ w1, const float* __restrict__ b1,
+                                                   uint16_t* __restrict__ h1) {
   __shared__ __align__(16) uint16_t W[MLP_HID * W1S];
-  __shared__ __align__(16) uint16_t stage_all[(IW / 64) * 16 * STS];
-  for (int e = threadIdx.x; e < MLP_HID * (MLP_KPAD / 8); e += IW) {
+  __shared__ __align__(16) uint16_t stage_all[(IW1X / 64) * 16 * STS];
+  for (int e = threadIdx.x; e < MLP_HID * (MLP_KPAD / 8); e += IW1X) {
     int n = e / (MLP_KPAD / 8), k8 = e % (MLP_KPAD / 8);
     *(uint4*)&W[n * W1S + 8 * k8] = ((const uint4*)w1)[e];
   }
@@ -276,7 +278,7 @@ __global__ void __launch_bounds__(IW) k_mlp_l1x(const float* __restrict__ in, ui
   const int r16 = lane & 15, q = lane >> 4;
   uint16_t* stage = stage_all + wave * 16 * STS;
   const uint32_t tiles = (rows + ROWS_PER_WAVE - 1) / ROWS_PER_WAVE;
-  for (uint32_t t = blockIdx.x * (IW / 64) + wave; t < tiles; t += gridDim.x * (IW / 64)) {
+  for (uint32_t t = blockIdx.x * (IW1X / 64) + wave; t < tiles; t += gridDim.x * (IW1X / 64)) {
     const uint32_t row0 = t * ROWS_PER_WAVE;
     f32x4 acc[2][8];
 #pragma unroll
@@ -452,7 +454,8 @@ hipError_t mlp_infer(const MlpDev& w, const float* in, uint32_t n, uint16_t* h1,
     uint32_t g1 = std::min<uint32_t>((uint32_t)cus * 2u, want), g2 = std::min<uint32_t>((uint32_t)cus, want);
     const uint32_t want25 = (tiles + IW25 / 64 - 1) / (IW25 / 64);
     const uint32_t g25 = std::min<uint32_t>((uint32_t)cus, want25);
-    if (k32) hipLaunchKernelGGL(k_mlp_l1x, dim3(g1), dim3(IW), 0, st, in + (size_t)o * MLP_IN, rows, w.wb[0], w.b[0], h1);
+    const uint32_t g1x = std::min<uint32_t>((uint32_t)cus, (tiles + IW1X / 64 - 1) / (IW1X / 64));
+    if (k32) hipLaunchKernelGGL(k_mlp_l1x, dim3(g1x), dim3(IW1X), 0, st, in + (size_t)o * MLP_IN, rows, w.wb[0], w.b[0], h1);
     else hipLaunchKernelGGL(k_mlp_l1, dim3(g1), dim3(IW), 0, st, in + (size_t)o * MLP_IN, rows, w.wb[0], w.b[0], h1);
     L25Args a;
     a.h1 = h1;
