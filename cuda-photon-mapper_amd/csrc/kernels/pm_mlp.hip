@@ -179,22 +179,25 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
       }
 }
 
-// out[i] = sum over s of part[s * stride + i], fixed order.
-__global__ void k_sum_slices(const float* __restrict__ part, int slices, size_t stride, size_t n,
-                             float* __restrict__ out) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float s = 0.0f;
-  for (int z = 0; z < slices; ++z) s += part[(size_t)z * stride + i];
-  out[i] = s;
-}
-
-// Column sums of a BF16 matrix [rows x cols] (bias gradients): one block per
-// column, fixed-order tree reduction.
-__global__ void __launch_bounds__(256) k_colsum(const uint16_t* __restrict__ x, int rows, int cols, int ld,
-                                                 float* __restrict__ out) {
+// Both reductions of one layer's gradients in one launch.  Blocks below
+// nwb = ceil(n / 256) sum the split-K weight partials over the slices, in
+// slice order; the next `cols` blocks take the column sums of the BF16 delta
+// [rows x cols] (bias gradients), one block per column with a fixed-order
+// tree reduction.
+__global__ void __launch_bounds__(256) k_grad_reduce(const float* __restrict__ part, int slices, size_t stride,
+                                                     size_t n, unsigned nwb, float* __restrict__ gw,
+                                                     const uint16_t* __restrict__ x, int rows, int cols, int ld,
+                                                     float* __restrict__ gb) {
   __shared__ float red[256];
-  const int c = blockIdx.x;
+  if (blockIdx.x < nwb) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.0f;
+    for (int z = 0; z < slices; ++z) s += part[(size_t)z * stride + i];
+    gw[i] = s;
+    return;
+  }
+  const int c = (int)(blockIdx.x - nwb);
   float s = 0.0f;
   for (int r = threadIdx.x; r < rows; r += 256) s += bf2f(x[(size_t)r * ld + c]);
   red[threadIdx.x] = s;
@@ -203,7 +206,7 @@ __global__ void __launch_bounds__(256) k_colsum(const uint16_t* __restrict__ x, 
     if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[c] = red[0];
+  if (threadIdx.x == 0) gb[c] = red[0];
 }
 
 // Loss L = mean over (rows, 3 channels) of (y - t)^2 on y = net output,
@@ -229,12 +232,27 @@ __global__ void k_loss_grad(const float* __restrict__ y, const float* __restrict
 }
 
 // Adam on FP32 master parameters, refreshing the BF16 copy.
-__global__ void k_adam(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
-                       const float* __restrict__ grad, uint16_t* __restrict__ pb, size_t n, float lr, float b1,
-                       float b2, float eps, float c1, float c2) {
-  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  float gr = grad[i];
+// Adam over every weight and bias tensor in one launch: tensor s takes the
+// blocks [first[s], first[s + 1]).
+struct AdamSegs {
+  float* p[2 * MLP_LAYERS];
+  float* m[2 * MLP_LAYERS];
+  float* v[2 * MLP_LAYERS];
+  const float* g[2 * MLP_LAYERS];
+  uint16_t* pb[2 * MLP_LAYERS];
+  uint32_t n[2 * MLP_LAYERS];
+  uint32_t first[2 * MLP_LAYERS + 1];
+};
+
+__global__ void k_adam_all(AdamSegs S, float lr, float b1, float b2, float eps, float c1, float c2) {
+  int s = 0;
+  while (s + 1 < 2 * MLP_LAYERS && blockIdx.x >= S.first[s + 1]) ++s;
+  const size_t i = (size_t)(blockIdx.x - S.first[s]) * blockDim.x + threadIdx.x;
+  if (i >= S.n[s]) return;
+  float* p = S.p[s];
+  float* m = S.m[s];
+  float* v = S.v[s];
+  float gr = S.g[s][i];
   float mi = b1 * m[i] + (1.0f - b1) * gr;
   float vi = b2 * v[i] + (1.0f - b2) * gr * gr;
   m[i] = mi;
@@ -242,7 +260,7 @@ __global__ void k_adam(float* __restrict__ p, float* __restrict__ m, float* __re
   float mh = mi / c1, vh = vi / c2;
   float pi = p[i] - lr * mh / (sqrt_(vh) + eps);
   p[i] = pi;
-  if (pb) pb[i] = f2bf(pi);
+  if (S.pb[s]) S.pb[s][i] = f2bf(pi);
 }
 
 __global__ void k_to_bf16(const float* __restrict__ x, size_t n, uint16_t* __restrict__ y) {
@@ -341,9 +359,9 @@ hipError_t mlp_backward(const MlpDev& w, const uint16_t* X, uint32_t rows, uint1
     a.k_per_slice = kps;
     if ((e = gemm(a, slices, st)) != hipSuccess) return e;
     size_t nw = (size_t)nout * kin;
-    hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, tb.partial, slices,
-                       (size_t)nout * kin, nw, tb.gw[l]);
-    hipLaunchKernelGGL(k_colsum, dim3(nout), dim3(256), 0, st, delta, (int)rows, nout, dcols, tb.gb[l]);
+    const unsigned nwb = (unsigned)((nw + 255) / 256);
+    hipLaunchKernelGGL(k_grad_reduce, dim3(nwb + (unsigned)nout), dim3(256), 0, st, tb.partial, slices,
+                       (size_t)nout * kin, nw, nwb, tb.gw[l], delta, (int)rows, nout, dcols, tb.gb[l]);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (l == 0) break;
     // delta_{l-1} [rows x 128] = (delta [rows x nout] * W_l [nout x 128]) * (H[l-1] > 0)
@@ -363,13 +381,23 @@ hipError_t mlp_backward(const MlpDev& w, const uint16_t* X, uint32_t rows, uint1
 hipError_t mlp_adam(MlpDev& w, const MlpTrainBufs& tb, float lr, int step, hipStream_t st) {
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   float c1 = 1.0f - (float)std::pow(0.9, (double)step), c2 = 1.0f - (float)std::pow(0.999, (double)step);
+  AdamSegs S;
+  uint32_t blocks = 0;
   for (int l = 0; l <= MLP_HIDDEN_LAYERS; ++l) {
-    size_t nw = mlp_w_count(l), nb = mlp_b_count(l);
-    hipLaunchKernelGGL(k_adam, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, w.w[l], w.mw[l], w.vw[l],
-                       tb.gw[l], w.wb[l], nw, lr, b1, b2, eps, c1, c2);
-    hipLaunchKernelGGL(k_adam, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, w.b[l], w.mb[l], w.vb[l],
-                       tb.gb[l], (uint16_t*)nullptr, nb, lr, b1, b2, eps, c1, c2);
+    for (int j = 0; j < 2; ++j) {
+      const int s = 2 * l + j;
+      S.p[s] = j ? w.b[l] : w.w[l];
+      S.m[s] = j ? w.mb[l] : w.mw[l];
+      S.v[s] = j ? w.vb[l] : w.vw[l];
+      S.g[s] = j ? tb.gb[l] : tb.gw[l];
+      S.pb[s] = j ? nullptr : w.wb[l];
+      S.n[s] = (uint32_t)(j ? mlp_b_count(l) : mlp_w_count(l));
+      S.first[s] = blocks;
+      blocks += (S.n[s] + 255u) / 256u;
+    }
   }
+  S.first[2 * MLP_LAYERS] = blocks;
+  hipLaunchKernelGGL(k_adam_all, dim3(blocks), dim3(256), 0, st, S, lr, b1, b2, eps, c1, c2);
   return hipGetLastError();
 }
 
