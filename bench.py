@@ -94,6 +94,36 @@ def run_gpu(cfg, steps, warmup, gather_mode, tile_regions=0):
     return (t1 - t0) / steps, per, g.stats(), img, g
 
 
+def run_surface_gather(g, cfg, steps):
+    """The surface-gather measurement: the same context and photon maps as the
+    plain frame, rendered without the march, so every radiance query is a
+    surface estimate (k_surface neighbours of the 1M surface records) at the
+    diffuse hit of an eye path of the 1920x1080 x spp frame."""
+    lib = pmamd.load()
+    npx = cfg.width * cfg.height
+    rgb = pmamd.DeviceBuffer(npx * 12)
+    g.set_option(pmamd.PM_OPT_RENDER_MEDIA, 0)
+    ms = []
+    try:
+        for i in range(steps + 1):
+            pmamd.check(lib.pm_render(g.h, rgb.ptr, None), "pm_render")
+            s = g.stats()
+            if i > 0:
+                ms.append(s.ms_render)
+    finally:
+        g.set_option(pmamd.PM_OPT_RENDER_MEDIA, 1)
+    t = float(np.median(ms))
+    q = int(s.surface_queries)
+    bpq = gather_bytes_per_query(cfg.k_surface, cfg.storage == pmamd.PM_STORE_BF16)
+    log("surface gather: %.1f ms for %d queries" % (t, q))
+    return {"workload": "surface radiance estimates only (no march): the plain frame's %d surface records, k=%d, "
+                        "r<=%g, at the diffuse hits of the %dx%d x %d spp eye paths"
+                        % (int(s.n_surface), cfg.k_surface, cfg.rcap_surface, cfg.width, cfg.height, cfg.spp),
+            "ms_per_frame": round(t, 3), "surface_queries": q, "radiance_queries_per_s": round(q / (t * 1e-3), 1),
+            "bytes_per_query": bpq, "achieved_GBps": round(bpq * q / (t * 1e-3) / 1e9, 1),
+            "frac_of_hbm_peak": round(bpq * q / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def rel_rmse(a, b, exposure=None):
     if exposure is not None:  # displayed values: exposure, clamp to [0, 1]
         a, b = np.clip(a * exposure, 0, 1), np.clip(b * exposure, 0, 1)
@@ -259,6 +289,8 @@ def main():
     ap.add_argument("--tile-regions", type=int, default=0,
                     help="0: raster tile order; 1, 2, 4, 8: per-XCD regions, that many across")
     ap.add_argument("--frame", type=str, default="", help="WxHxSPP override of the frame (profiling runs only)")
+    ap.add_argument("--surface-steps", type=int, default=3,
+                    help="timed surface-only frames (0: skip the surface-gather leg)")
     ap.add_argument("--cache-steps", type=int, default=3, help="timed cache frames (0: skip the cache leg)")
     ap.add_argument("--cache-train-frames", type=int, default=48)
     ap.add_argument("--cache-batch", type=int, default=8192, help="Adam batch (2^18 training samples per frame)")
@@ -322,6 +354,8 @@ def main():
         "roofline": roof,
         "image_mean_rgb": [round(float(x), 5) for x in img.reshape(-1, 3).mean(0)],
     }
+    if args.surface_steps > 0:
+        out["surface_gather"] = run_surface_gather(g, cfg, args.surface_steps)
     if args.cache_steps > 0:
         ca = run_cache(g, cfg, args.cache_steps, args.cache_train_frames, args.cache_stride, args.cache_threshold,
                        args.cache_batch, img.reshape(-1, 3))

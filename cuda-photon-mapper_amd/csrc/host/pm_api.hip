@@ -407,6 +407,7 @@ int pm_render_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev, 
   memset(&r, 0, sizeof(r));
   r.sc = c->scene;
   r.rp = c->rp;
+  if (!c->render_media) r.rp.media_on = 0;
   r.gs = gather_map(c, c->ms.m, c->cfg.k_surface, c->cfg.rcap_surface);
   r.gv = gather_map(c, c->mv.m, c->cfg.k_volume, c->cfg.rcap_volume);
   r.estimator = c->cfg.estimator;
@@ -484,6 +485,10 @@ int pm_set_option(pm_context* c, int option, int value) {
     case PM_OPT_SAMPLE_LANES:
       if (value != 0 && value != 1) return PM_ERR_INVALID;
       c->sample_lanes = value;
+      return PM_OK;
+    case PM_OPT_RENDER_MEDIA:
+      if (value != 0 && value != 1) return PM_ERR_INVALID;
+      c->render_media = value;
       return PM_OK;
     default:
       return PM_ERR_INVALID;

@@ -111,6 +111,7 @@ struct pm_context {
   int blocks_per_cu = 4;
   int tile_regions = 0;  // per-XCD tile regions of the lane render (0: raster order)
   int sample_lanes = 1;  // mode 7: a pixel's samples in adjacent lanes
+  int render_media = 1;  // 0: the plain render gathers at surface hits only
   pm_stats stats;
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
   pm::CacheState cache;

@@ -487,6 +487,12 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
     return mul(mul(sum, albedo), sc) * surface_norm(r2, false);
   }
 
+  // Surface queries of the march: surface_lone without the cone filter.
+  __device__ V3f surface(V3f x, V3f n, V3f albedo, V3f = V3f(), V3f = V3f()) {
+    if constexpr (CONE) return Base::surface(x, n, albedo);
+    else return surface_lone(x, n, albedo);
+  }
+
   __device__ bool volume_defer(V3f x, V3f wo, V3f w, V3f& c) {
     ++this->nvol;
     const GatherMap& gv = this->gv;

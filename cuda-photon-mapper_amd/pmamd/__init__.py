@@ -35,6 +35,7 @@ PM_MAP_SURFACE, PM_MAP_VOLUME = 0, 1
 PM_OPT_GATHER_MODE, PM_OPT_BLOCKS_PER_CU, PM_OPT_CACHE_FUSED_INFER, PM_OPT_CACHE_INFER_CHUNK = 1, 2, 3, 4
 PM_OPT_TILE_REGIONS = 5
 PM_OPT_SAMPLE_LANES = 6
+PM_OPT_RENDER_MEDIA = 7
 
 
 class PmSphere(ctypes.Structure):

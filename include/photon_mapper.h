@@ -192,14 +192,19 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
  * PM_OPT_SAMPLE_LANES  gather mode 7 with spp = 2, 4, 8 or 16: 1 = the samples of a
  *                      pixel in adjacent lanes (default; a wave renders 64 / spp
  *                      pixels in raster tile order), 0 = one lane loops over a
- *                      pixel's samples (PM_OPT_TILE_REGIONS applies to this form). */
+ *                      pixel's samples (PM_OPT_TILE_REGIONS applies to this form);
+ * PM_OPT_RENDER_MEDIA  1 = pm_render / pm_render_crop march the medium (default,
+ *                      when the scene has one), 0 = they gather at surface hits
+ *                      only: the surface-gather measurement on the same photon
+ *                      maps (the volume map is built but not read). */
 enum {
   PM_OPT_GATHER_MODE = 1,
   PM_OPT_BLOCKS_PER_CU = 2,
   PM_OPT_CACHE_FUSED_INFER = 3,
   PM_OPT_CACHE_INFER_CHUNK = 4,
   PM_OPT_TILE_REGIONS = 5,
-  PM_OPT_SAMPLE_LANES = 6
+  PM_OPT_SAMPLE_LANES = 6,
+  PM_OPT_RENDER_MEDIA = 7
 };
 int pm_set_option(pm_context* ctx, int option, int value);
 

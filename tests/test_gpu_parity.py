@@ -323,6 +323,36 @@ def test_sample_lanes_matches_oracle(pair2):
         g.set_option(pmamd.PM_OPT_SAMPLE_LANES, 2)
 
 
+@pytest.mark.parametrize("mode", [1, 7])
+def test_render_without_march_matches_oracle(pair2, ncpu, mode):
+    # PM_OPT_RENDER_MEDIA 0: the same photon maps, radiance estimates at the
+    # surface hits only; the oracle renders the GPU's records with the medium
+    # switched off
+    g, _ = pair2
+    s = g.stats()
+    c = type(g.cfg).from_buffer_copy(g.cfg)
+    c.media_on = 0
+    o = O.Oracle(c, ncpu)
+    o.set_records(pmamd.PM_MAP_SURFACE, *g.records(pmamd.PM_MAP_SURFACE), s.n_emit_surface)
+    o.set_records(pmamd.PM_MAP_VOLUME, *g.records(pmamd.PM_MAP_VOLUME), s.n_emit_volume)
+    o.build()
+    g.set_option(pmamd.PM_OPT_GATHER_MODE, mode)
+    g.set_option(pmamd.PM_OPT_RENDER_MEDIA, 0)
+    try:
+        img = g.render()
+        q = g.stats()
+    finally:
+        g.set_option(pmamd.PM_OPT_RENDER_MEDIA, 1)
+        g.set_option(pmamd.PM_OPT_GATHER_MODE, 5)
+    ref = o.render()
+    say("surface queries %d, volume queries %d" % (q.surface_queries, q.volume_queries))
+    assert q.volume_queries == 0 and q.surface_queries > 0
+    assert (q.surface_queries, q.volume_queries) == o.last_queries
+    assert_radiance_close(img, ref)
+    with pytest.raises(pmamd.PmError):
+        g.set_option(pmamd.PM_OPT_RENDER_MEDIA, 2)
+
+
 @pytest.mark.parametrize("estimator", ["box", "cone"])
 def test_sample_lanes_bf16_fp64_transmittance(ncpu, estimator):
     # BF16 power/direction storage, FP64 transmittance and both surface
