@@ -109,7 +109,7 @@ namespace pm {
 void cache_release(CacheState& k) {
   for (DevBuf* b : {&k.params, &k.grads, &k.adam_m, &k.adam_v, &k.params_bf16, &k.X, &k.y, &k.dy, &k.d0, &k.d1,
                     &k.row_err, &k.partial, &k.rgb, &k.err_sum, &k.err_cnt, &k.rec_cnt, &k.rec_off, &k.rec_in,
-                    &k.rec_wt, &k.rec_pix, &k.rec_val, &k.sel_in, &k.sel_t, &k.flags, &k.flag_off, &k.list})
+                    &k.rec_wt, &k.rec_pix, &k.rec_val, &k.sel_in, &k.sel_t, &k.flags, &k.flags2, &k.flag_off, &k.list})
     b->release();
   for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) k.H[l].release();
   k.ready = false;
@@ -349,16 +349,15 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
     HIPCHK(c->small.ensure(64));
     uint32_t* list_s = k.list.as<uint32_t>();
     uint32_t* list_v = list_s + N;
-    for (int kind = 0; kind < 2; ++kind) {
-      HIPCHK(launch_kind_flags(k.rec_in.as<float>(), N, kind == 0, k.flags.as<uint32_t>(), st));
-      HIPCHK(exclusive_scan(k.flags.as<uint32_t>(), k.flag_off.as<uint32_t>(), N, c->scratch.as<uint32_t>(),
-                            c->small.as<uint32_t>() + 10 + kind, st));
-      HIPCHK(launch_cache_compact(k.flags.as<uint32_t>(), k.flag_off.as<uint32_t>(), N, kind == 0 ? list_s : list_v,
-                                  st));
-    }
+    // one flag pass: the volume records are the complement
+    HIPCHK(launch_kind_flags(k.rec_in.as<float>(), N, 1, k.flags.as<uint32_t>(), st));
+    HIPCHK(exclusive_scan(k.flags.as<uint32_t>(), k.flag_off.as<uint32_t>(), N, c->scratch.as<uint32_t>(),
+                          c->small.as<uint32_t>() + 10, st));
+    HIPCHK(launch_cache_split(k.flags.as<uint32_t>(), k.flag_off.as<uint32_t>(), N, list_s, list_v, st));
     uint32_t nk[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(nk, c->small.as<uint32_t>() + 10, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(nk, c->small.as<uint32_t>() + 10, 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    nk[1] = (uint32_t)N - nk[0];
     // without surface caching the subset is drawn from the volume records only
     const uint32_t ns_sel = k.cache_surface ? nk[0] : 0u;
     HIPCHK(launch_cache_select(k.rec_in.as<float>(), list_s, ns_sel, list_v, nk[1], S, c->cfg.seed, k.frame,
@@ -394,14 +393,15 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   // the list holds the surface records, then the volume records, each in
   // record order: a wave then runs one kind of gather
   uint32_t* ntot = c->small.as<uint32_t>() + 8;
+  HIPCHK(k.flags2.ensure(4ull * std::max<uint64_t>(N, 1)));
+  fp.kind = 3;  // surface fallbacks in flags, volume ones in flags2, from one pass over the records
+  HIPCHK(launch_cache_flags(k.rec_in.as<float>(), N, k.err_sum.as<unsigned long long>(), k.err_cnt.as<unsigned int>(),
+                            fp, k.flags.as<uint32_t>(), st, k.flags2.as<uint32_t>()));
   for (int kind = 1; kind <= 2; ++kind) {
-    fp.kind = kind;
-    HIPCHK(launch_cache_flags(k.rec_in.as<float>(), N, k.err_sum.as<unsigned long long>(),
-                              k.err_cnt.as<unsigned int>(), fp, k.flags.as<uint32_t>(), st));
-    HIPCHK(exclusive_scan(k.flags.as<uint32_t>(), k.flag_off.as<uint32_t>(), N, c->scratch.as<uint32_t>(),
-                          ntot + kind - 1, st));
-    HIPCHK(launch_cache_compact(k.flags.as<uint32_t>(), k.flag_off.as<uint32_t>(), N, k.list.as<uint32_t>(),
-                                kind == 2 ? ntot : nullptr, st));
+    const uint32_t* fl = kind == 1 ? k.flags.as<uint32_t>() : k.flags2.as<uint32_t>();
+    HIPCHK(exclusive_scan(fl, k.flag_off.as<uint32_t>(), N, c->scratch.as<uint32_t>(), ntot + kind - 1, st));
+    HIPCHK(launch_cache_compact(fl, k.flag_off.as<uint32_t>(), N, k.list.as<uint32_t>(), kind == 2 ? ntot : nullptr,
+                                st));
   }
   uint32_t nk2[2] = {0, 0};
   HIPCHK(hipMemcpyAsync(nk2, ntot, 8, hipMemcpyDeviceToHost, st));

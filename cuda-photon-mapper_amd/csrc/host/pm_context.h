@@ -69,7 +69,7 @@ struct CacheState {
   int infer_chunk_log2 = 19;  // rows per fused-inference launch pair (log2)
   int stride = 1;          // march points per volume record
   int cache_surface = 0;   // 0: surface queries are always gathered exactly
-  DevBuf rec_cnt, rec_off, rec_in, rec_wt, rec_pix, rec_val, sel_in, sel_t, flags, flag_off, list;
+  DevBuf rec_cnt, rec_off, rec_in, rec_wt, rec_pix, rec_val, sel_in, sel_t, flags, flags2, flag_off, list;
   pm_cache_stats st;
 };
 

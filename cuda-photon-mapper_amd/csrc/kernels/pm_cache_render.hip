@@ -278,7 +278,8 @@ __global__ void k_err_accum(const float* __restrict__ in, const float* __restric
 // the radiance is steepest), and records in cells whose training error is
 // above the threshold or that saw no training row.
 __global__ void k_flags(const float* __restrict__ in, uint64_t n, const unsigned long long* __restrict__ esum,
-                        const unsigned int* __restrict__ ecnt, CacheFlagParams fp, uint32_t* __restrict__ flags) {
+                        const unsigned int* __restrict__ ecnt, CacheFlagParams fp, uint32_t* __restrict__ flags,
+                        uint32_t* __restrict__ flags2) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float* p = in + i * MLP_IN;
@@ -290,6 +291,11 @@ __global__ void k_flags(const float* __restrict__ in, uint64_t n, const unsigned
   // mean over the cell's training rows of the per-channel squared log error
   const double mse = (double)esum[cell] / 16777216.0 / ((double)c * (double)MLP_OUT);
   bool fb = (surf && !fp.cache_surface) || near || c == 0u || mse > (double)fp.threshold;
+  if (fp.kind == 3) {
+    flags[i] = fb && surf ? 1u : 0u;
+    flags2[i] = fb && !surf ? 1u : 0u;
+    return;
+  }
   if (fp.kind != 0) fb = fb && surf == (fp.kind == 1);
   flags[i] = fb ? 1u : 0u;
 }
@@ -389,9 +395,27 @@ hipError_t launch_sum_rows(const float* v, uint32_t n, double* acc, hipStream_t 
 }
 
 hipError_t launch_cache_flags(const float* in, uint64_t n, const unsigned long long* esum, const unsigned int* ecnt,
-                              const CacheFlagParams& fp, uint32_t* flags, hipStream_t st) {
+                              const CacheFlagParams& fp, uint32_t* flags, hipStream_t st, uint32_t* flags2) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, esum, ecnt, fp, flags);
+  if (fp.kind == 3 && !flags2) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, esum, ecnt, fp, flags,
+                     flags2);
+  return hipGetLastError();
+}
+
+__global__ void k_split(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ off, uint64_t n,
+                        uint32_t* __restrict__ list_a, uint32_t* __restrict__ list_b) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t o = off[i];
+  if (flags[i]) list_a[o] = (uint32_t)i;
+  else list_b[(uint32_t)i - o] = (uint32_t)i;
+}
+
+hipError_t launch_cache_split(const uint32_t* flags, const uint32_t* off, uint64_t n, uint32_t* list_a,
+                              uint32_t* list_b, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_split, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, flags, off, n, list_a, list_b);
   return hipGetLastError();
 }
 

@@ -239,7 +239,8 @@ struct CacheFlagParams {
   int cache_surface;         // 0: surface records always fall back
   float light[3];
   float near2;               // volume records this close to the light fall back
-  int kind;                  // flag 0: every fallback record, 1: surface ones only, 2: volume ones only
+  int kind;                  // flag 0: every fallback record, 1: surface ones only, 2: volume ones only,
+                             // 3: surface ones in flags, volume ones in flags2 (one pass)
 };
 struct CacheValueLaunch {
   GatherMap gs, gv;
@@ -261,7 +262,11 @@ hipError_t launch_err_accum(const float* in, const float* row_err, uint32_t rows
                             unsigned int* ecnt, hipStream_t st);
 hipError_t launch_sum_rows(const float* v, uint32_t n, double* acc, hipStream_t st);
 hipError_t launch_cache_flags(const float* in, uint64_t n, const unsigned long long* esum, const unsigned int* ecnt,
-                              const CacheFlagParams& fp, uint32_t* flags, hipStream_t st);
+                              const CacheFlagParams& fp, uint32_t* flags, hipStream_t st, uint32_t* flags2 = nullptr);
+// Both lists from one flag array: list_a[off[i]] = i where flags[i], else
+// list_b[i - off[i]] = i (the complement's exclusive scan is i - off[i]).
+hipError_t launch_cache_split(const uint32_t* flags, const uint32_t* off, uint64_t n, uint32_t* list_a,
+                              uint32_t* list_b, hipStream_t st);
 // list[base + off[i]] = i where flags[i] (base: a device word, or null for 0)
 hipError_t launch_cache_compact(const uint32_t* flags, const uint32_t* off, uint64_t n, uint32_t* list,
                                 const uint32_t* base, hipStream_t st);
