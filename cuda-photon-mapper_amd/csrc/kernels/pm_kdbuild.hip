@@ -93,6 +93,7 @@ struct LevelArgs {
   const uint32_t* node_axis;
   uint32_t nblocks;
   uint32_t* blockcnt;  // [3][nblocks]
+  uint8_t* flags;      // [n] per position: bit b = flag of list b, bit 7 = in a node of this level
 };
 
 // Flag of list b at position p: does the element order before the node's
@@ -108,6 +109,9 @@ __device__ __forceinline__ bool part_flag(const LevelArgs& A, int b, uint32_t p,
   return ki < km || (ki == km && i < m);
 }
 
+// Counts the flags per block and keeps them (A.flags) for k_level_scatter:
+// the flags of the two lists off the node's axis gather a coordinate of a
+// point in another list's order, the costly part of a level.
 __global__ void __launch_bounds__(KB_THREADS) k_level_count(LevelArgs A) {
   __shared__ uint32_t lw[KB_THREADS / 64];
   uint32_t cnt[3] = {0u, 0u, 0u};
@@ -116,8 +120,16 @@ __global__ void __launch_bounds__(KB_THREADS) k_level_count(LevelArgs A) {
     size_t p = base + r;
     if (p >= A.F.n) break;
     uint32_t h = A.seg[p];
-    if (h == KD_LEAF) continue;
-    for (int b = 0; b < 3; ++b) cnt[b] += part_flag(A, b, (uint32_t)p, h) ? 1u : 0u;
+    uint8_t m = 0;
+    if (h != KD_LEAF) {
+      m = 0x80;
+      for (int b = 0; b < 3; ++b)
+        if (part_flag(A, b, (uint32_t)p, h)) {
+          m |= (uint8_t)(1u << b);
+          cnt[b] += 1u;
+        }
+    }
+    A.flags[p] = m;
   }
   for (int b = 0; b < 3; ++b) {
     // reduce via wave sums + LDS
@@ -168,15 +180,9 @@ __global__ void __launch_bounds__(KB_THREADS) k_level_scatter(LevelArgs A, const
     fl[r] = 0;
     hs[r] = KD_LEAF;
     if (p >= A.F.n) continue;
-    uint32_t h = A.seg[p];
-    hs[r] = h;
-    if (h == KD_LEAF) continue;
-    uint8_t m = 0x80;
-    for (int b = 0; b < 3; ++b)
-      if (part_flag(A, b, (uint32_t)p, h)) {
-        m |= (uint8_t)(1u << b);
-        cnt[b] += 1u;
-      }
+    hs[r] = A.seg[p];
+    const uint8_t m = A.flags[p];  // from k_level_count
+    for (int b = 0; b < 3; ++b) cnt[b] += (m >> b) & 1u;
     fl[r] = m;
   }
   uint32_t run[3];
@@ -239,6 +245,7 @@ size_t kdforest_temp_words(uint32_t n, uint32_t nnodes) {
   w += scan_temp_words(3 * nb);
   w += radix_temp_words(n);
   w += 2 * (size_t)nnodes + 8;  // node_med, node_axis
+  w += ((size_t)n + 3) / 4;     // per-position flag bytes of a level
   return w + 64;
 }
 
@@ -262,6 +269,7 @@ hipError_t kd_build_forest(const KdForest& F, const uint32_t* level_off, uint32_
   uint32_t* sorttemp = p; p += radix_temp_words(n);
   uint32_t* node_med = p; p += F.nnodes + 1;
   uint32_t* node_axis = p; p += F.nnodes + 1;
+  uint8_t* pflags = (uint8_t*)p; p += ((size_t)n + 3) / 4;
 
   hipError_t e;
   const float* C[3] = {F.x, F.y, F.z};
@@ -288,6 +296,7 @@ hipError_t kd_build_forest(const KdForest& F, const uint32_t* level_off, uint32_
     A.seg = seg; A.seg_out = seg2;
     A.node_med = node_med; A.node_axis = node_axis;
     A.nblocks = (uint32_t)nb; A.blockcnt = blockcnt;
+    A.flags = pflags;
     hipLaunchKernelGGL(k_level_count, dim3((unsigned)nb), dim3(KB_THREADS), 0, st, A);
     if ((e = exclusive_scan(blockcnt, blockoff, 3 * nb, scantemp, nullptr, st)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_level_scatter, dim3((unsigned)nb), dim3(KB_THREADS), 0, st, A, (const uint32_t*)blockoff);
