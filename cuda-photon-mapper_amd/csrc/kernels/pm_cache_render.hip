@@ -330,7 +330,20 @@ __global__ void k_accumulate(const uint32_t* __restrict__ off, const uint32_t* _
   NSum3<float> acc;
   nsum3_init(acc);
   const size_t b = off[p], e = b + cnt[p];
-  for (size_t i = b; i < e; ++i)
+  size_t i = b;
+  // four records' loads issued before their adds (the adds stay in record order)
+  for (; i + 4 <= e; i += 4) {
+    V3f w[4], x[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const size_t r = i + (size_t)j;
+      w[j] = v3(wt[3 * r], wt[3 * r + 1], wt[3 * r + 2]);
+      x[j] = v3(val[3 * r], val[3 * r + 1], val[3 * r + 2]);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) nsum3_add(acc, mul(w[j], x[j]));
+  }
+  for (; i < e; ++i)
     nsum3_add(acc, mul(v3(wt[3 * i], wt[3 * i + 1], wt[3 * i + 2]), v3(val[3 * i], val[3 * i + 1], val[3 * i + 2])));
   V3f v = nsum3_get(acc) * (1.0f / (float)spp);
   if (rgb) {
