@@ -23,6 +23,10 @@
 // Counting costs one LDS add per point in the window and no data-dependent
 // loop, where the heap's insertions run sift loops that the whole wave waits
 // through for the few lanes that insert.
+// Queries gathered on their own (the march's surface hits with the box
+// estimate, and the radiance cache's exact values) have no previous point to
+// seed a window: they run the heap pass and the second pass on the
+// speculative traversal with node boxes (surface_lone, SPEC_HEAP).
 #pragma once
 
 #include "pm_lane_knn4.h"
