@@ -173,7 +173,6 @@ int size_records(pm_batch* b) {
   HIPCHK(b->z.ensure(4 * N));
   HIPCHK(b->payload.ensure(stride * N));
   HIPCHK(b->tpts.ensure(16 * (N + kTptsPad)));
-  HIPCHK(b->tpayload.ensure(stride * N));
   for (int v = 0; v < 2; ++v) {
     DevMap& m = v ? b->mv : b->ms;
     memset(&m, 0, sizeof(m));
@@ -202,7 +201,8 @@ GatherMap tree_gather_map(const pm_batch* b, int scene, int vol, uint32_t tree) 
   g.payload = (const char*)b->payload.p + stride * b->tree_pbase_h[tree];
   g.bmin = b->bmin.as<float4>() + b->tree_desc[tree].bbase;
   g.bmax = b->bmax.as<float4>() + b->tree_desc[tree].bbase;
-  g.tpayload = (const char*)b->tpayload.p + stride * b->tree_pbase_h[tree];
+  // the tree-ordered payload exists once gather mode 7 has been chosen
+  g.tpayload = b->tpayload.p ? (const char*)b->tpayload.p + stride * b->tree_pbase_h[tree] : nullptr;
   g.n = b->tree_n[tree];
   g.nl = b->tree_nl[tree];
   g.bf16 = b->bf16;
@@ -212,6 +212,8 @@ GatherMap tree_gather_map(const pm_batch* b, int scene, int vol, uint32_t tree) 
   for (int j = 0; j < 3; ++j) g.scale[j] = c.light_power[j] / (float)c.n_emit;
   return g;
 }
+
+int setup_scene_maps(pm_batch* b);
 
 // Forest tables for leaf size L, the per-scene gather maps, the build scratch.
 int setup_forest(pm_batch* b, int L) {
@@ -329,7 +331,12 @@ int setup_forest(pm_batch* b, int L) {
     HIPCHK(hipMemcpy(b->level_pref_d.p, lvpref.data(), 4ull * lvpref.size(), hipMemcpyHostToDevice));
   HIPCHK(hipStreamSynchronize(b->stream));
   b->L = L;
-  // per-scene render descriptors (the gather maps point into the forest)
+  return setup_scene_maps(b);
+}
+
+// Per-scene render descriptors (the gather maps point into the forest and,
+// for gather mode 7, into the tree-ordered payload).
+int setup_scene_maps(pm_batch* b) {
   b->rsc.resize(b->ns);
   for (int s = 0; s < b->ns; ++s) {
     const pm_config& c = b->cfg[s];
@@ -518,6 +525,12 @@ int apply_tuning(pm_batch* b, const pm_batch_tuning& t, bool capture_graph) {
   int r;
   b->tune = t;
   if ((r = set_grid(b)) != PM_OK) return r;
+  if (t.gather_mode == 7 && !b->tpayload.p) {
+    // the tree-ordered payload of gather mode 7, allocated on first use
+    const size_t N = std::max<size_t>((size_t)b->S_tot + b->V_tot, 1);
+    HIPCHK(b->tpayload.ensure((b->bf16 ? 16 : 32) * N));
+    if (t.leaf_size == b->L && (r = setup_scene_maps(b)) != PM_OK) return r;
+  }
   if (t.leaf_size != b->L && (r = setup_forest(b, t.leaf_size)) != PM_OK) return r;
   if (!capture_graph) {
     release_graph(b);
