@@ -116,12 +116,54 @@ def run_surface_gather(g, cfg, steps):
     q = int(s.surface_queries)
     bpq = gather_bytes_per_query(cfg.k_surface, cfg.storage == pmamd.PM_STORE_BF16)
     log("surface gather: %.1f ms for %d queries" % (t, q))
+    ctl = run_wall_control(cfg, steps, bpq)
     return {"workload": "surface radiance estimates only (no march): the plain frame's %d surface records, k=%d, "
                         "r<=%g, at the diffuse hits of the %dx%d x %d spp eye paths"
                         % (int(s.n_surface), cfg.k_surface, cfg.rcap_surface, cfg.width, cfg.height, cfg.spp),
             "ms_per_frame": round(t, 3), "surface_queries": q, "radiance_queries_per_s": round(q / (t * 1e-3), 1),
             "bytes_per_query": bpq, "achieved_GBps": round(bpq * q / (t * 1e-3) / 1e9, 1),
-            "frac_of_hbm_peak": round(bpq * q / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            "frac_of_hbm_peak": round(bpq * q / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "control_uniform_walls": ctl}
+
+
+def run_wall_control(cfg, steps, bpq, n=1000000):
+    """The surface gather on a control map: n records uniform on the room's five
+    walls (numpy default_rng(1)), each arriving along its wall's outward
+    direction, rendered with the same frame and estimator."""
+    lib = pmamd.load()
+    rng = np.random.default_rng(1)
+    w = rng.integers(0, 5, n)
+    u, v = rng.random(n, dtype=np.float32), rng.random(n, dtype=np.float32)
+    pos = np.zeros((n, 3), np.float32)
+    dirs = np.zeros((n, 3), np.float32)
+    axis = np.array([0, 0, 1, 1, 2])[w]
+    side = np.array([0.0, 1.0, 0.0, 1.0, 1.0], np.float32)[w]
+    others = np.array([[1, 2], [1, 2], [0, 2], [0, 2], [0, 1]])[w]
+    rows = np.arange(n)
+    pos[rows, axis] = side
+    pos[rows, others[:, 0]] = u
+    pos[rows, others[:, 1]] = v
+    dirs[rows, axis] = np.where(side > 0.5, 1.0, -1.0)
+    power = np.ones((n, 3), np.float32)
+    g = pmamd.PhotonMapper(cfg)
+    g.set_records(pmamd.PM_MAP_SURFACE, pos, power, dirs, n)
+    g.build()
+    g.set_option(pmamd.PM_OPT_RENDER_MEDIA, 0)
+    npx = cfg.width * cfg.height
+    rgb = pmamd.DeviceBuffer(npx * 12)
+    ms = []
+    for i in range(steps + 1):
+        pmamd.check(lib.pm_render(g.h, rgb.ptr, None), "pm_render")
+        s = g.stats()
+        if i > 0:
+            ms.append(s.ms_render)
+    g.close()
+    t = float(np.median(ms))
+    q = int(s.surface_queries)
+    log("surface gather, uniform-wall control: %.1f ms for %d queries" % (t, q))
+    return {"records": n, "generator": "numpy default_rng(1), uniform on the 5 walls", "ms_per_frame": round(t, 3),
+            "surface_queries": q, "radiance_queries_per_s": round(q / (t * 1e-3), 1),
+            "achieved_GBps": round(bpq * q / (t * 1e-3) / 1e9, 1)}
 
 
 def rel_rmse(a, b, exposure=None):

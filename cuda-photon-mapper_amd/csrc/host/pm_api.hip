@@ -509,6 +509,57 @@ int pm_get_records(pm_context* c, int map, float* pos, float* power, float* dir,
   return records_to_host(m, pos, power, dir);
 }
 
+int pm_set_records(pm_context* c, int map, const float* pos, const float* power, const float* dir, uint32_t n,
+                   uint32_t n_emit) {
+  if (!c || n_emit == 0 || (n > 0 && (!pos || !power || !dir))) return PM_ERR_INVALID;
+  if (map != PM_MAP_SURFACE && map != PM_MAP_VOLUME) return PM_ERR_INVALID;
+  MapBufs& mb = map == PM_MAP_VOLUME ? c->mv : c->ms;
+  const int bf16 = c->cfg.storage == PM_STORE_BF16 ? 1 : 0;
+  int r = alloc_map(mb, n, bf16, c->cfg.leaf_size);
+  if (r != PM_OK) return r;
+  mb.m.n_emit = n_emit;
+  if (n > 0) {
+    std::vector<float> x(n), y(n), z(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      x[i] = pos[3 * i];
+      y[i] = pos[3 * i + 1];
+      z[i] = pos[3 * i + 2];
+    }
+    HIPCHK(hipMemcpy(mb.m.x, x.data(), 4ull * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(mb.m.y, y.data(), 4ull * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(mb.m.z, z.data(), 4ull * n, hipMemcpyHostToDevice));
+    if (bf16) {
+      // 8 BF16 per record: power, direction, 2 zero pads (round to nearest even)
+      std::vector<uint16_t> p(8ull * n, 0);
+      for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+          p[8ull * i + k] = f32_to_bf16(power[3 * i + k]);
+          p[8ull * i + 3 + k] = f32_to_bf16(dir[3 * i + k]);
+        }
+      HIPCHK(hipMemcpy(mb.m.payload, p.data(), 16ull * n, hipMemcpyHostToDevice));
+    } else {
+      // 8 FP32 per record: power, direction, 2 zero pads
+      std::vector<float> p(8ull * n, 0.0f);
+      for (uint32_t i = 0; i < n; ++i)
+        for (int k = 0; k < 3; ++k) {
+          p[8ull * i + k] = power[3 * i + k];
+          p[8ull * i + 3 + k] = dir[3 * i + k];
+        }
+      HIPCHK(hipMemcpy(mb.m.payload, p.data(), 32ull * n, hipMemcpyHostToDevice));
+    }
+  }
+  if (map == PM_MAP_VOLUME) {
+    c->stats.n_volume = n;
+    c->stats.n_emit_volume = n_emit;
+  } else {
+    c->stats.n_surface = n;
+    c->stats.n_emit_surface = n_emit;
+  }
+  c->emitted = true;
+  c->built = false;
+  return PM_OK;
+}
+
 int pm_get_tree(pm_context* c, int map, uint32_t* perm, float* split, uint32_t* axis, uint32_t n, uint32_t ni) {
   if (!c || !perm) return PM_ERR_INVALID;
   if (!c->built) return PM_ERR_STATE;

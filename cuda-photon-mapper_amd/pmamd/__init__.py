@@ -181,6 +181,8 @@ _SIGS = {
                                      ctypes.c_int, ctypes.c_float, ctypes.c_float]),
     "pm_get_records": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_uint32]),
+    "pm_set_records": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     "pm_get_tree": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     "pm_knn": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int,
@@ -342,6 +344,15 @@ class PhotonMapper:
         if want_u8:
             return out, u8.download(np.uint8, w * h * 4).reshape(h, w, 4)
         return out
+
+    def set_records(self, which, pos, power, dirs, n_emit):
+        """Replace a map's records (host arrays [n, 3]); build() must follow."""
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        power = np.ascontiguousarray(power, np.float32).reshape(-1, 3)
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        assert pos.shape == power.shape == dirs.shape
+        check(load().pm_set_records(self.h, which, pos.ctypes.data, power.ctypes.data, dirs.ctypes.data,
+                                    pos.shape[0], int(n_emit)), "pm_set_records")
 
     def records(self, which):
         s = self.stats()

@@ -328,6 +328,13 @@ int pm_batch_get_tree(pm_batch* b, int scene, int map, uint32_t* perm_host, floa
 
 /* Inspection (copies to host memory; for tests and tools). */
 int pm_get_records(pm_context* ctx, int map, float* pos_host, float* power_host, float* dir_host, uint32_t n);
+/* Replaces map `map`'s records with n caller records (host arrays of 3 floats
+ * per record; power and direction rounded to BF16 under BF16 storage), whose
+ * flux normalisation is light_power / n_emit; pm_build must follow.  For
+ * measurements on given point sets (the other map is kept).  No reference
+ * counterpart (the reference has no photon map API). */
+int pm_set_records(pm_context* ctx, int map, const float* pos_host, const float* power_host, const float* dir_host,
+                   uint32_t n, uint32_t n_emit);
 int pm_get_tree(pm_context* ctx, int map, uint32_t* perm_host, float* split_host, uint32_t* axis_host,
                 uint32_t n, uint32_t n_internal);
 /* kNN over a map's kd-tree for nq device query points (xyz, FP32): idx_dev and

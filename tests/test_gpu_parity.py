@@ -353,6 +353,59 @@ def test_render_without_march_matches_oracle(pair2, ncpu, mode):
         g.set_option(pmamd.PM_OPT_RENDER_MEDIA, 2)
 
 
+def wall_points(rng, n):
+    # uniform on the room's five walls (x = 0, x = 1, y = 0, y = 1, z = 1), each
+    # arriving along the wall's outward direction
+    w = rng.integers(0, 5, n)
+    u, v = rng.random(n, dtype=np.float32), rng.random(n, dtype=np.float32)
+    pos = np.zeros((n, 3), np.float32)
+    dirs = np.zeros((n, 3), np.float32)
+    axis = np.array([0, 0, 1, 1, 2])[w]
+    side = np.array([0.0, 1.0, 0.0, 1.0, 1.0], np.float32)[w]
+    others = np.array([[1, 2], [1, 2], [0, 2], [0, 2], [0, 1]])[w]
+    rows = np.arange(n)
+    pos[rows, axis] = side
+    pos[rows, others[:, 0]] = u
+    pos[rows, others[:, 1]] = v
+    dirs[rows, axis] = np.where(side > 0.5, 1.0, -1.0)
+    return pos, dirs
+
+
+@pytest.mark.parametrize("storage", ["fp32", "bf16"])
+def test_set_records_matches_oracle(ncpu, storage):
+    # caller-given surface records (uniform on the walls), rendered without the
+    # march, against the oracle on the same records
+    c = small_cfg2(w=80, h=48, spp=1)
+    c.media_on = 0
+    if storage == "bf16":
+        c.storage = pmamd.PM_STORE_BF16
+    rng = np.random.default_rng(5)
+    n = 30000
+    pos, dirs = wall_points(rng, n)
+    power = rng.uniform(0.5, 1.5, (n, 3)).astype(np.float32)
+    g = pmamd.PhotonMapper(c)
+    g.set_records(pmamd.PM_MAP_SURFACE, pos, power, dirs, 60000)
+    g.build()
+    gp, gw, gd = g.records(pmamd.PM_MAP_SURFACE)
+    assert np.array_equal(bits(gp), bits(pos))
+    if storage == "fp32":
+        assert np.array_equal(bits(gw), bits(power)) and np.array_equal(bits(gd), bits(dirs))
+    s = g.stats()
+    assert (s.n_surface, s.n_emit_surface, s.n_volume) == (n, 60000, 0)
+    o = O.Oracle(c, ncpu)
+    o.set_records(pmamd.PM_MAP_SURFACE, gp, gw, gd, 60000)
+    o.build()
+    gp_, gs_, ga_ = g.tree(pmamd.PM_MAP_SURFACE)
+    op_, os_, oa_ = o.tree(pmamd.PM_MAP_SURFACE)
+    assert np.array_equal(gp_, op_) and np.array_equal(bits(gs_), bits(os_)) and np.array_equal(ga_, oa_)
+    img = g.render()
+    ref = o.render()
+    assert g.stats().surface_queries == o.last_queries[0] > 0
+    assert_radiance_close(img, ref)
+    with pytest.raises(pmamd.PmError):
+        g.set_records(2, pos, power, dirs, 60000)
+
+
 @pytest.mark.parametrize("estimator", ["box", "cone"])
 def test_sample_lanes_bf16_fp64_transmittance(ncpu, estimator):
     # BF16 power/direction storage, FP64 transmittance and both surface
