@@ -464,12 +464,11 @@ hipError_t enqueue_frame(pm_batch* b, const uint2* items, uint32_t nitems, hipEv
     fb.ntrees = F.ntrees;
     fb.leaf_pref = b->leaf_pref_d.as<uint32_t>();
     fb.level_pref = b->level_pref_d.as<uint32_t>();
-    fb.level_total = b->box_level_total.data();
     fb.depth = b->box_depth;
     fb.nleaves = b->nleaves;
     fb.bmin = b->bmin.as<float4>();
     fb.bmax = b->bmax.as<float4>();
-    if ((e = kd_forest_boxes(fb, st)) != hipSuccess) return e;
+    if ((e = kd_forest_boxes(fb, b->box_level_total.data(), st)) != hipSuccess) return e;
     if (R.counting &&
         (e = kd_forest_payload(F.tpts, F.point_tree, F.tree_pbase, b->payload.p, b->tpayload.p, F.n, b->bf16, st)) !=
             hipSuccess)

@@ -494,12 +494,12 @@ __global__ void k_forest_union_boxes(KdForestBoxes f, uint32_t d, uint32_t count
   f.bmax[bb + h] = make_float4(fmaxf(c.x, e.x), fmaxf(c.y, e.y), fmaxf(c.z, e.z), 0.0f);
 }
 
-hipError_t kd_forest_boxes(const KdForestBoxes& f, hipStream_t st) {
+hipError_t kd_forest_boxes(const KdForestBoxes& f, const uint32_t* level_total, hipStream_t st) {
   if (f.nleaves == 0) return hipSuccess;
   hipLaunchKernelGGL(k_forest_leaf_boxes, dim3((f.nleaves + 255u) / 256u), dim3(256), 0, st, f);
   hipError_t e = hipGetLastError();
   for (uint32_t d = f.depth; d-- > 0u && e == hipSuccess;) {
-    const uint32_t count = f.level_total[d];
+    const uint32_t count = level_total[d];
     if (count == 0) continue;
     hipLaunchKernelGGL(k_forest_union_boxes, dim3((count + 255u) / 256u), dim3(256), 0, st, f, d, count);
     e = hipGetLastError();

@@ -135,13 +135,13 @@ struct KdForestBoxes {
   uint32_t ntrees;
   const uint32_t* leaf_pref;     // [ntrees + 1] leaves of the earlier trees
   const uint32_t* level_pref;    // [depth][ntrees + 1] internal nodes at level d of the earlier trees
-  const uint32_t* level_total;   // host: [depth] internal nodes at level d of all trees
   uint32_t depth;                // internal levels of the deepest tree
   uint32_t nleaves;
   float4* bmin;
   float4* bmax;
 };
-hipError_t kd_forest_boxes(const KdForestBoxes& f, hipStream_t st);
+// level_total (host): [depth] internal nodes at level d of all trees
+hipError_t kd_forest_boxes(const KdForestBoxes& f, const uint32_t* level_total, hipStream_t st);
 // tpayload[i] = payload[tree_pbase[point_tree[i]] + tree-local index of tpts[i]]
 hipError_t kd_forest_payload(const float4* tpts, const uint32_t* point_tree, const uint32_t* tree_pbase,
                              const void* payload, void* tpayload, uint32_t n, int bf16, hipStream_t st);
