@@ -66,7 +66,7 @@ struct CacheState {
   float lr = 1e-3f;
   uint32_t frame = 0;
   int fused_infer = 2;     // pm_mlp_infer.hip kernels (when the LDS fits): 2 = 16x16x32 MFMA, 1 = 16x16x16; 0: generic GEMMs
-  int infer_chunk_log2 = 19;  // rows per fused-inference launch pair (log2)
+  int infer_chunk_log2 = 20;  // rows per fused-inference launch pair (log2)
   int stride = 1;          // march points per volume record
   int cache_surface = 0;   // 0: surface queries are always gathered exactly
   DevBuf rec_cnt, rec_off, rec_in, rec_wt, rec_pix, rec_val, sel_in, sel_t, flags, flags2, flag_off, list;

@@ -182,8 +182,9 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
  *                      kernels with 16x16x32 BF16 MFMA (default), 1 = the same
  *                      with 16x16x16 BF16 MFMA, 0 = through the generic GEMMs;
  * PM_OPT_CACHE_INFER_CHUNK log2 of the rows per fused-inference launch pair,
- *                      14..20 (default 19): smaller chunks keep the layer-1
- *                      activations in the on-die caches between the two launches;
+ *                      14..20 (default 20): smaller chunks keep the layer-1
+ *                      activations in the on-die caches between the two launches,
+ *                      larger ones launch fewer, fuller grids (faster at 1080p);
  * PM_OPT_TILE_REGIONS  order in which the lane modes' workgroups take 8x8 tiles:
  *                      0 = raster order (default), rx = 1, 2, 4 or 8 = the
  *                      frame cut into rx x (8 / rx) regions, one per XCD
