@@ -8,13 +8,16 @@
 //                       path weights (throughput x transmittance x step, with
 //                       the surface albedo folded in)
 //   4. training       : a Philox-chosen subset of the records gets exact kNN
-//                       targets (k_values) and trains the MLP (pm_mlp.hip);
+//                       targets (k_values7 under gather modes 7 and auto,
+//                       k_values otherwise) and trains the MLP (pm_mlp.hip);
 //                       each batch's pre-update errors are binned into a
 //                       16^3 x {surface, volume} grid (k_err_accum)
 //   5. inference      : MLP over all records (pm_mlp.hip)
 //   6. fallback       : records in cells whose mean training error exceeds
 //                       the threshold (or that saw no training sample) are
-//                       re-gathered exactly with kNN (k_flags, k_values)
+//                       re-gathered exactly with kNN (k_flags, then
+//                       k_values7 / k_values over the surface rows and the
+//                       volume rows)
 //   7. k_accumulate   : per pixel, compensated sum of weight x value / spp.
 // No reference counterpart (the reference has no cache); the plain path it
 // replaces is photon_mapping_kernel, photonMappingKernel.cu:1409-1462.
