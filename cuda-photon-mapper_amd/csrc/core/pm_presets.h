@@ -232,9 +232,25 @@ inline RenderParams render_params(const pm_config& c) {
   return r;
 }
 
-// Validates a configuration; returns 0 or PM_ERR_INVALID.
+inline bool finite_f(float v) { return __builtin_isfinite(v); }
+inline bool unit_f(float v) { return v >= 0.0f && v <= 1.0f; }  // false for NaN
+
+// Validates a configuration; returns 0 or PM_ERR_INVALID.  Every float that
+// reaches the kernels must be finite (NaN or Inf in a position, power or
+// coefficient would propagate silently into the photon maps and the frame).
 inline int validate_config(const pm_config& c) {
   if (c.n_spheres < 0 || c.n_spheres > PM_MAX_SPHERES) return PM_ERR_INVALID;
+  for (int f = 0; f < 5; ++f)
+    for (int k = 0; k < 3; ++k)
+      if (!unit_f(c.wall_albedo[f][k])) return PM_ERR_INVALID;
+  for (int k = 0; k < 3; ++k) {
+    if (!finite_f(c.light_pos[k]) || !finite_f(c.eye[k])) return PM_ERR_INVALID;
+    if (!finite_f(c.light_power[k]) || c.light_power[k] < 0.0f) return PM_ERR_INVALID;
+  }
+  if (!finite_f(c.sigma_s) || !finite_f(c.sigma_a) || c.sigma_s < 0.0f || c.sigma_a < 0.0f) return PM_ERR_INVALID;
+  if (!finite_f(c.hg_g) || !finite_f(c.anim_time)) return PM_ERR_INVALID;
+  if (!finite_f(c.focal) || !(c.focal > 0.0f) || !finite_f(c.exposure) || c.exposure < 0.0f) return PM_ERR_INVALID;
+  if (!finite_f(c.march_step) || !finite_f(c.rcap_surface) || !finite_f(c.rcap_volume)) return PM_ERR_INVALID;
   if (c.max_bounces < 1 || c.max_bounces > 64) return PM_ERR_INVALID;
   if (!(c.rr_survive > 0.0f) || c.rr_survive > 1.0f) return PM_ERR_INVALID;
   if (c.leaf_size < 1 || c.leaf_size > 64) return PM_ERR_INVALID;
@@ -252,7 +268,10 @@ inline int validate_config(const pm_config& c) {
   if (c.n_emit == 0 && c.cap_surface == 0) return PM_ERR_INVALID;
   for (int i = 0; i < c.n_spheres; ++i) {
     const pm_sphere& s = c.spheres[i];
-    if (!(s.radius > 0.0f) || s.material < 0 || s.material > 2) return PM_ERR_INVALID;
+    if (!finite_f(s.radius) || !(s.radius > 0.0f) || s.material < 0 || s.material > 2) return PM_ERR_INVALID;
+    for (int k = 0; k < 3; ++k)
+      if (!finite_f(s.center[k]) || !unit_f(s.albedo[k])) return PM_ERR_INVALID;
+    if (!finite_f(s.ior)) return PM_ERR_INVALID;
     if (s.material == PM_MAT_DIELECTRIC && !(s.ior > 1.0f)) return PM_ERR_INVALID;
   }
   if (c.storage != PM_STORE_FP32 && c.storage != PM_STORE_BF16) return PM_ERR_INVALID;

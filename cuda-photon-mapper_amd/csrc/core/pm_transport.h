@@ -219,17 +219,26 @@ struct CountOnly<G, decltype((void)G::kCountOnly, void())> {
 };
 
 // Ray-march of the in-scattered radiance over [t0, t1) along o + t d:
-// samples at t_j = t0 + (j + xi) * step, each weighted by transmittance and
-// the step length.  Homogeneous: T = exp(-sigma_t * (t - t0)).  Density field:
-// optical depth accumulated left-point (tau_j = sum_{i<j} sigma_t(x_i) step),
-// in TT precision (FP64 when the long-segment FP64 option is on).
+// samples at t_j = t0 + (j + xi) * step, each weighted by the transmittance
+// from t0 to t_j and the step length.  Homogeneous: T_j = exp(-sigma_t *
+// (t_j - t0)).  Density field: the optical depth to t_j over the intervals
+// [t0, t_0], [t_0, t_1], ..., [t_{j-1}, t_j], the first by sigma_t(x_0) times
+// its length and the others by the trapezoid rule:
+// tau_j = sigma_t(x_0) (t_0 - t0) + sum_{0<i<=j} (sigma_t(x_{i-1}) +
+// sigma_t(x_i)) / 2 (t_i - t_{i-1}); the segment's own transmittance adds
+// sigma_t(x_last) (t1 - t_last) (with no sample, sigma_t at the segment's
+// midpoint times its length).  At constant density both forms give
+// exp(-sigma_t (t - t0)) at every sample and at t1.  tau is accumulated in TT
+// precision.
 template <class T, class TT, class G>
 PM_HD void march_segment(const Scene& sc, const RenderParams& rp, V3<T> o, V3<T> d, T t0, T t1, T xi,
                          V3<T>& thr, NSum3<T>& L, G& g) {
   const Medium& m = sc.med;
   const T ds = T(rp.march_step);
   const TT st = TT(m.sigma_s) + TT(m.sigma_a);
-  TT tau = TT(0);
+  TT tau = TT(0), s_last = TT(0);
+  T t_last = t0;
+  bool sampled = false;
   const V3<T> wo = -d;
   for (int j = 0; j < rp.max_steps; ++j) {
     T t = t0 + (T(j) + xi) * ds;
@@ -243,8 +252,12 @@ PM_HD void march_segment(const Scene& sc, const RenderParams& rp, V3<T> o, V3<T>
     if (m.type == MED_HOMOGENEOUS) {
       Tj = (T)exp_(-st * (TT)(t - t0));
     } else {
+      const TT s = (TT)sigma_t_at(m, x);
+      tau = tau + (sampled ? (s_last + s) * TT(0.5) : s) * (TT)(t - t_last);
       Tj = (T)exp_(-tau);
-      tau = tau + (TT)sigma_t_at(m, x) * (TT)ds;
+      t_last = t;
+      s_last = s;
+      sampled = true;
     }
     V3<T> w = thr * (Tj * ds);
     if constexpr (DeferredVolume<G>::value) {
@@ -261,10 +274,20 @@ PM_HD void march_segment(const Scene& sc, const RenderParams& rp, V3<T> o, V3<T>
   }
   if constexpr (CountOnly<G>::value) return;
   T Tseg;
-  if (m.type == MED_HOMOGENEOUS) Tseg = (T)exp_(-st * (TT)(t1 - t0));
-  else Tseg = (T)exp_(-tau);
+  if (m.type == MED_HOMOGENEOUS) {
+    Tseg = (T)exp_(-st * (TT)(t1 - t0));
+  } else {
+    if (!sampled) s_last = (TT)sigma_t_at(m, o + d * ((t0 + t1) * T(0.5)));
+    Tseg = (T)exp_(-(tau + s_last * (TT)(t1 - t_last)));
+  }
   thr = thr * Tseg;
 }
+
+// Segments of at least this many march steps in a density field take the
+// FP64 transmittance when it is enabled (fp64_transmittance): there the
+// optical depth sums the most terms.  Shorter segments, and every segment of
+// a homogeneous medium (a closed form, no sum), stay in the frame's precision.
+constexpr int kLongSegmentSteps = 32;
 
 // One eye sample.  The gather object provides
 //   g.surface(x, n, albedo, w, wo) -> outgoing radiance of a diffuse point toward wo (n faces the viewer)
@@ -289,7 +312,16 @@ PM_HD V3<T> render_sample_t(const Scene& sc, const RenderParams& rp, uint32_t px
       T t0 = T(0);
       if (o.z < T(0)) t0 = d.z > T(0) ? -o.z / d.z : T(3.0e38);
       T t1 = h.kind != HIT_NONE ? h.t : (o.z < T(0) ? T(0) : t_exit_front(o, d));
-      if (t1 > t0) march_segment<T, TT>(sc, rp, o, d, t0, t1, xi, thr, L, g);
+      if (t1 > t0) {
+        if constexpr (sizeof(TT) > sizeof(T)) {
+          if (sc.med.type == MED_NOISE && t1 - t0 >= T(kLongSegmentSteps) * T(rp.march_step))
+            march_segment<T, TT>(sc, rp, o, d, t0, t1, xi, thr, L, g);
+          else
+            march_segment<T, T>(sc, rp, o, d, t0, t1, xi, thr, L, g);
+        } else {
+          march_segment<T, TT>(sc, rp, o, d, t0, t1, xi, thr, L, g);
+        }
+      }
     }
     if (h.kind == HIT_NONE) break;
     V3<T> x = o + d * h.t;
@@ -344,13 +376,17 @@ template <class T> PM_HD T volume_norm(T r2) {
 }
 template <class T> PM_HD T cone_weight(T d2, T r) { return T(1) - sqrt_(d2) / (T(1.1) * r); }
 
-// Display mapping of the FP32 radiance: exposure, explicit [0,1] clamp,
-// round to 8 bits (the reference writes (uchar)min(c*255,255) with no lower
-// clamp, photonMappingKernel.cu:1451-1453).
+// Display mapping of the FP32 radiance, the reference's frame-write rule
+// (photonMappingKernel.cu:1451-1453): c * 255.0 in double, capped at 255 and
+// truncated toward zero.  The FP32 radiance is first scaled by the exposure.
+// The reference has no lower bound (a negative value converted to unsigned
+// char is undefined behaviour); here values below 0, and NaN, give 0.
 PM_HD unsigned char tonemap_u8(float v, float exposure) {
-  float c = v * exposure;
-  c = c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
-  return (unsigned char)(int)rint_(c * 255.0f);
+  const float c = v * exposure;
+  double s = (double)c * 255.0;
+  s = s > 255.0 ? 255.0 : s;
+  if (!(s > 0.0)) s = 0.0;
+  return (unsigned char)(int)s;
 }
 
 }  // namespace pm

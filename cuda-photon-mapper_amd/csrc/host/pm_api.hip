@@ -102,10 +102,16 @@ int do_emit(pm_context* c) {
   HIPCHK(hipEventRecord(c->e0, st));
   HIPCHK(c->small.ensure(64));
   uint32_t* small = c->small.as<uint32_t>();
-  uint32_t N = fixed ? c->cfg.n_emit : (uint32_t)std::min<uint64_t>(std::max<uint64_t>(cap_s, 1u << 16), 1u << 30);
+  // Record counts, offsets and totals are 32-bit.  A photon stores at most
+  // max_bounces records, so at most UINT32_MAX / max_bounces photons are
+  // traced: no total can wrap, and k_find_cap sees each offset range once.
+  const uint64_t max_photons = std::min<uint64_t>(1ull << 31, 0xffffffffull / (uint64_t)c->cfg.max_bounces);
+  if (fixed && c->cfg.n_emit > max_photons) return PM_ERR_CAPACITY;
+  uint32_t N = fixed ? c->cfg.n_emit
+                     : (uint32_t)std::min<uint64_t>(std::min<uint64_t>(std::max<uint64_t>(cap_s, 1u << 16), 1u << 30),
+                                                    max_photons);
   uint32_t done = 0;
   uint32_t tot[2] = {0, 0};
-  const uint64_t max_photons = 1ull << 31;
   while (true) {
     int r;
     if ((r = grow_keep(c->cnt_s, N, done, st)) != PM_OK) return r;

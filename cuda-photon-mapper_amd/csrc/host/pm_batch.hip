@@ -522,15 +522,28 @@ int apply_tuning(pm_batch* b, const pm_batch_tuning& t, bool capture_graph) {
   if (t.gather_mode != 0 && t.gather_mode != 7) return PM_ERR_INVALID;
   if (t.gather_mode == 7 && (t.heap_key_bytes != 2 || t.heap_slots != 64)) return PM_ERR_INVALID;
   int r;
+  // the grid is checked first, then the tuning is committed; a later failure
+  // puts the previous tuning (and its grid) back
+  const pm_batch_tuning prev = b->tune;
+  const int prev_grid = b->grid;
   b->tune = t;
-  if ((r = set_grid(b)) != PM_OK) return r;
+  if ((r = set_grid(b)) != PM_OK) {
+    b->tune = prev;
+    b->grid = prev_grid;
+    return r;
+  }
+  auto restore = [&](int err) {
+    b->tune = prev;
+    b->grid = prev_grid;
+    return err;
+  };
   if (t.gather_mode == 7 && !b->tpayload.p) {
     // the tree-ordered payload of gather mode 7, allocated on first use
     const size_t N = std::max<size_t>((size_t)b->S_tot + b->V_tot, 1);
-    HIPCHK(b->tpayload.ensure((b->bf16 ? 16 : 32) * N));
-    if (t.leaf_size == b->L && (r = setup_scene_maps(b)) != PM_OK) return r;
+    if (b->tpayload.ensure((b->bf16 ? 16 : 32) * N) != hipSuccess) return restore(PM_ERR_NOMEM);
+    if (t.leaf_size == b->L && (r = setup_scene_maps(b)) != PM_OK) return restore(r);
   }
-  if (t.leaf_size != b->L && (r = setup_forest(b, t.leaf_size)) != PM_OK) return r;
+  if (t.leaf_size != b->L && (r = setup_forest(b, t.leaf_size)) != PM_OK) return restore(r);
   if (!capture_graph) {
     release_graph(b);
     return PM_OK;
@@ -587,13 +600,18 @@ int pm_batch_scene_config(uint32_t seed, int index, pm_config* out) {
 
 int pm_batch_create(const pm_config* cfgs, int n, pm_batch** out) {
   if (!cfgs || !out || n < 1 || n > 128) return PM_ERR_INVALID;
+  uint64_t max_records = 0;
   for (int i = 0; i < n; ++i) {
     const pm_config& c = cfgs[i];
     if (validate_config(c) != 0 || c.n_emit == 0 || c.cap_surface || c.cap_volume) return PM_ERR_INVALID;
     if (c.storage != cfgs[0].storage || c.estimator != cfgs[0].estimator ||
         c.fp64_transmittance != cfgs[0].fp64_transmittance)
       return PM_ERR_INVALID;
+    max_records += (uint64_t)c.n_emit * (uint64_t)c.max_bounces;
   }
+  // record offsets of the batch arena are 32-bit, and the arena holds fewer
+  // than 2^31 records: refuse any batch whose photons could store more
+  if (max_records >= (1ull << 31)) return PM_ERR_CAPACITY;
   pm_batch* b = new pm_batch();
   b->ns = n;
   b->cfg.assign(cfgs, cfgs + n);
@@ -683,6 +701,7 @@ int pm_batch_autotune(pm_batch* b, int sample_stride, pm_batch_tuning* best_out,
   auto consider = [&](const pm_batch_tuning& t) -> int {
     if (t.heap_slots < b->kmax) return PM_OK;
     int r = apply_tuning(b, t, false);
+    if (r == PM_ERR_INVALID) return PM_OK;  // a candidate this device cannot run: skipped
     if (r != PM_OK) return r;
     double ms = 0.0, ms2 = 0.0;
     if ((r = trial(&ms)) != PM_OK) return r;

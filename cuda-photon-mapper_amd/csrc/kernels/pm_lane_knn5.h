@@ -158,6 +158,9 @@ template <class Add>
 struct Sum16CVisitor : Sum16Visitor<Add> {
   template <int NB>
   __device__ void points(const float4* p, const uint32_t* dh, int cnt, uint32_t base) {
+    // the tail below names sel[2] and sel[3] directly (no dynamic register
+    // indexing), which covers batches of at most 4 points
+    static_assert(NB <= 4, "Sum16CVisitor handles leaf batches of at most 4 points");
     uint32_t sel[NB];
 #pragma unroll
     for (int j = 0; j < NB; ++j) sel[j] = 0u;

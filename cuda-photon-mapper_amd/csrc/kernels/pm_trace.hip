@@ -191,7 +191,9 @@ __global__ void k_find_cap(const uint32_t* __restrict__ off, const uint32_t* __r
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   uint64_t a = off[p], b = a + cnt[p];
-  if (a < cap && b >= cap) *out = p + 1u;
+  // one photon's range holds the cap-th record; the minimum keeps the
+  // answer defined even if several ranges matched
+  if (a < cap && b >= cap) atomicMin(out, p + 1u);
 }
 
 hipError_t launch_trace_count(const Scene& sc, const TraceParams& tp, uint32_t pid0, uint32_t npid, uint32_t* cnt_s,

@@ -174,8 +174,9 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
  *                      6 = as 4, with each march step's second pass merged
  *                          into the next step's first-pass traversal;
  *                      7 = as 6, with the march's first passes done by
- *                          counting in a 64-bucket window around the previous
- *                          step's k-th-neighbour bucket (heap pass on a miss);
+ *                          counting in a window of 256 buckets around the
+ *                          previous step's k-th-neighbour bucket (heap pass
+ *                          on a miss);
  *                      every mode gives the same neighbour sets and images;
  * PM_OPT_BLOCKS_PER_CU workgroups per compute unit of the persistent mode-0 grid;
  * PM_OPT_CACHE_FUSED_INFER 2 = cache inference through the fused LDS-resident

@@ -50,6 +50,13 @@ _SIGS = {
     "orc_collision": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_int, _P, _P, ctypes.c_float, _P]),
     "orc_kd_leaves_under": (ctypes.c_uint32, [ctypes.c_uint32, ctypes.c_uint32]),
     "orc_gather": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, ctypes.c_int, _P]),
+    "orc_tonemap": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_float, _P]),
+    "orc_set_density": (ctypes.c_int, [_P, _P, ctypes.c_uint32]),
+    "orc_march": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P,
+                                 _P, _P, _P]),
+    "orc_render_const": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P, _P]),
+    "orc_camera_ray": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P]),
+    "orc_validate": (ctypes.c_int, [ctypes.POINTER(PmConfig)]),
 }
 
 
@@ -112,6 +119,18 @@ def phase_hg(c, g):
 
 def kd_leaves_under(c, nl):
     return load().orc_kd_leaves_under(c, nl)
+
+
+def tonemap(v, exposure):
+    """The frame-write rule of the library's headers (tonemap_u8) on FP32 values."""
+    v = np.ascontiguousarray(v, np.float32)
+    out = np.empty(v.shape, np.uint8)
+    assert load().orc_tonemap(_ptr(v), v.size, exposure, _ptr(out)) == 0
+    return out
+
+
+def validate(cfg):
+    return load().orc_validate(ctypes.byref(cfg))
 
 
 class Oracle:
@@ -233,6 +252,40 @@ class Oracle:
         st = load().orc_gather(self.h, which, _ptr(x), _ptr(v), _ptr(a), 1 if fp64 else 0, _ptr(out))
         assert st == 0, st
         return out
+
+    def set_density(self, grid):
+        g = np.ascontiguousarray(grid, np.float32).ravel()
+        assert load().orc_set_density(self.h, _ptr(g), g.size) == 0
+
+    def march(self, o, d, t0, t1, xi, S, thr=(1.0, 1.0, 1.0), fp64=False):
+        """march_segment with a constant in-scatter S: (L, outgoing throughput, queries)."""
+        o = np.ascontiguousarray(o, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        S = np.ascontiguousarray(S, np.float32)
+        thr = np.ascontiguousarray(thr, np.float32)
+        L = np.empty(3, np.float32)
+        tout = np.empty(3, np.float32)
+        nq = np.zeros(1, np.uint32)
+        st = load().orc_march(self.h, 1 if fp64 else 0, _ptr(o), _ptr(d), t0, t1, xi, _ptr(S), _ptr(thr), _ptr(L),
+                              _ptr(tout), _ptr(nq))
+        assert st == 0, st
+        return L, tout, int(nq[0])
+
+    def render_const(self, px, py, s, Sv, Ss):
+        """One eye sample with a constant gather: (radiance, (surface, volume) queries)."""
+        Sv = np.ascontiguousarray(Sv, np.float32)
+        Ss = np.ascontiguousarray(Ss, np.float32)
+        out = np.empty(3, np.float32)
+        nq = np.zeros(2, np.uint32)
+        assert load().orc_render_const(self.h, px, py, s, _ptr(Sv), _ptr(Ss), _ptr(out), _ptr(nq)) == 0
+        return out, (int(nq[0]), int(nq[1]))
+
+    def camera_ray(self, px, py, s):
+        o = np.empty(3, np.float32)
+        d = np.empty(3, np.float32)
+        xi = ctypes.c_float()
+        load().orc_camera_ray(self.h, px, py, s, _ptr(o), _ptr(d), ctypes.byref(xi))
+        return o, d, xi.value
 
     def collision(self, pid, ev, o, d, tmax):
         o = np.ascontiguousarray(o, np.float32)

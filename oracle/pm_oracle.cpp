@@ -176,7 +176,9 @@ int emit(Ctx& c) {
   uint32_t pid = 0;
   bool done_s = false, done_v = !media;
   uint32_t chunk = fixed ? c.cfg.n_emit : 65536u;
-  const uint64_t max_photons = 1ull << 31;
+  // the library's bound: no 32-bit record total may wrap
+  const uint64_t max_photons = std::min<uint64_t>(1ull << 31, 0xffffffffull / (uint64_t)c.cfg.max_bounces);
+  if (fixed && c.cfg.n_emit > max_photons) return PM_ERR_CAPACITY;
   while (true) {
     uint32_t p1 = fixed ? c.cfg.n_emit : (uint32_t)std::min<uint64_t>((uint64_t)pid + chunk, max_photons);
     if (p1 <= pid) break;
@@ -418,6 +420,23 @@ struct OracleGather {
     V3<T> sc = v3<T>(T(c->scene.light_power[0]) / T(m.n_emit), T(c->scene.light_power[1]) / T(m.n_emit),
                      T(c->scene.light_power[2]) / T(m.n_emit));
     return mul(sum, sc) * norm;
+  }
+};
+
+// A gather with a constant answer: volume queries return Sv, surface queries
+// Sv_s.  With it the march's quadrature weights (transmittance times step
+// length) and the segment bounds can be checked against closed forms.
+template <class T>
+struct ConstGather {
+  V3<T> sv, ss;
+  uint64_t nsurf = 0, nvol = 0;
+  V3<T> surface(V3<T>, V3<T>, V3<T>, V3<T> = V3<T>(), V3<T> = V3<T>()) {
+    ++nsurf;
+    return ss;
+  }
+  V3<T> volume(V3<T>, V3<T>, V3<T> = V3<T>()) {
+    ++nvol;
+    return sv;
   }
 };
 
@@ -683,6 +702,75 @@ int orc_collision(void* h, uint32_t pid, int ev, const float* o, const float* d,
 }
 
 uint32_t orc_kd_leaves_under(uint32_t c, uint32_t nl) { return kd_leaves_under(c, nl); }
+
+// The frame-write rule (tonemap_u8) for n FP32 values.
+int orc_tonemap(const float* v, uint32_t n, float exposure, uint8_t* out) {
+  for (uint32_t i = 0; i < n; ++i) out[i] = tonemap_u8(v[i], exposure);
+  return 0;
+}
+
+// Replaces the value-noise density grid (res^3 vertex values, x fastest) of a
+// MED_NOISE context: the march and delta tracking then read `grid`.
+int orc_set_density(void* h, const float* grid, uint32_t n) {
+  Ctx* c = (Ctx*)h;
+  if (c->scene.med.type != MED_NOISE || n != c->density.size()) return PM_ERR_INVALID;
+  memcpy(c->density.data(), grid, sizeof(float) * n);
+  return 0;
+}
+
+// march_segment over [t0, t1) along o + t d with the march offset xi, a
+// constant in-scatter S and incoming throughput thr: L (the weighted sum) and
+// the outgoing throughput; tt64 != 0 accumulates the transmittance in FP64.
+int orc_march(void* h, int tt64, const float* o, const float* d, float t0, float t1, float xi, const float* S,
+              const float* thr_in, float* L_out, float* thr_out, uint32_t* nq) {
+  Ctx* c = (Ctx*)h;
+  if (c->scene.med.type == MED_NONE) return PM_ERR_INVALID;
+  ConstGather<float> g;
+  g.sv = v3(S[0], S[1], S[2]);
+  g.ss = v3(0.0f, 0.0f, 0.0f);
+  V3f thr = v3(thr_in[0], thr_in[1], thr_in[2]);
+  NSum3<float> L;
+  nsum3_init(L);
+  V3f oo = v3(o[0], o[1], o[2]), dd = v3(d[0], d[1], d[2]);
+  if (tt64) march_segment<float, double>(c->scene, c->rp, oo, dd, t0, t1, xi, thr, L, g);
+  else march_segment<float, float>(c->scene, c->rp, oo, dd, t0, t1, xi, thr, L, g);
+  V3f l = nsum3_get(L);
+  L_out[0] = l.x; L_out[1] = l.y; L_out[2] = l.z;
+  thr_out[0] = thr.x; thr_out[1] = thr.y; thr_out[2] = thr.z;
+  if (nq) *nq = (uint32_t)g.nvol;
+  return 0;
+}
+
+// One eye sample (pixel px, py, sample s) of the configured frame with a
+// constant gather: volume queries return Sv, surface queries Ss (each times
+// its path weight).  Counts: nq[0] surface, nq[1] volume queries.
+int orc_render_const(void* h, uint32_t px, uint32_t py, uint32_t s, const float* Sv, const float* Ss, float* out,
+                     uint32_t* nq) {
+  Ctx* c = (Ctx*)h;
+  ConstGather<float> g;
+  g.sv = v3(Sv[0], Sv[1], Sv[2]);
+  g.ss = v3(Ss[0], Ss[1], Ss[2]);
+  V3f v = render_sample<float>(c->scene, c->rp, px, py, s, g);
+  out[0] = v.x; out[1] = v.y; out[2] = v.z;
+  if (nq) {
+    nq[0] = (uint32_t)g.nsurf;
+    nq[1] = (uint32_t)g.nvol;
+  }
+  return 0;
+}
+
+// Camera ray of pixel (px, py), sample s: origin, direction, march offset.
+int orc_camera_ray(void* h, uint32_t px, uint32_t py, uint32_t s, float* o, float* d, float* xi) {
+  Ctx* c = (Ctx*)h;
+  V3f oo, dd;
+  camera_ray(c->rp, px, py, s, &oo, &dd, xi);
+  o[0] = oo.x; o[1] = oo.y; o[2] = oo.z;
+  d[0] = dd.x; d[1] = dd.y; d[2] = dd.z;
+  return 0;
+}
+
+// validate_config of the library's headers (0 or PM_ERR_INVALID).
+int orc_validate(const pm_config* cfg) { return cfg ? validate_config(*cfg) : PM_ERR_INVALID; }
 
 // One radiance estimate (FP64 when fp64 != 0) at x: which 0 = surface with
 // normal v and albedo alb, 1 = volume toward direction v.  out = RGB.
