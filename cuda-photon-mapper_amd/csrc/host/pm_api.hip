@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "pm_context.h"
@@ -405,7 +406,16 @@ int pm_set_render(pm_context* c, int width, int height, int spp, int k_s, int k_
   return PM_OK;
 }
 
-int pm_render_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev, pm_uchar4* out_dev) {
+}  // extern "C"
+
+namespace {
+
+// The render behind pm_render_crop and pm_render_crop_noalloc.  `counters`
+// null: the context's own query counters (allocated on first use, cleared
+// with hipMemsetAsync); otherwise two caller-side counters, cleared by a fill
+// kernel, and nothing is allocated.
+int render_crop_impl(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev, pm_uchar4* out_dev,
+                     unsigned long long* counters) {
   if (!c) return PM_ERR_INVALID;
   if (!c->built) return PM_ERR_STATE;
   if (w <= 0 || h <= 0 || x0 < 0 || y0 < 0 || x0 + w > c->cfg.width || y0 + h > c->cfg.height) return PM_ERR_INVALID;
@@ -425,9 +435,14 @@ int pm_render_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev, 
   r.gather_mode = c->gather_mode;
   r.tile_regions = c->tile_regions;
   r.sample_lanes = c->sample_lanes;
-  HIPCHK(c->counters.ensure(16));
-  r.counters = c->counters.as<unsigned long long>();
-  HIPCHK(hipMemsetAsync(c->counters.p, 0, 16, c->stream));
+  if (counters) {
+    r.counters = counters;
+    HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(counters), 0u, 4, c->stream));
+  } else {
+    HIPCHK(c->counters.ensure(16));
+    r.counters = c->counters.as<unsigned long long>();
+    HIPCHK(hipMemsetAsync(c->counters.p, 0, 16, c->stream));
+  }
   HIPCHK(hipEventRecord(c->e0, c->stream));
   if (r.gather_mode == 5) {
     // auto: the two-pass gather (march passes merged, first passes counted)
@@ -451,13 +466,63 @@ int pm_render_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev, 
   }
   HIPCHK(hipEventRecord(c->e1, c->stream));
   unsigned long long q[2] = {0, 0};
-  HIPCHK(hipMemcpyAsync(q, c->counters.p, 16, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(q, r.counters, 16, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->stats.ms_render = elapsed_ms(c->e0, c->e1);
   c->stats.ms_gather_kernel = c->stats.ms_render;
   c->stats.surface_queries = q[0];
   c->stats.volume_queries = q[1];
   return PM_OK;
+}
+
+// Scratch of pm_render_crop_noalloc: the render's two query counters
+// (surface, volume), 16 bytes in a device array that exists from library load
+// to process end, one copy per GPU.  Every call on a GPU shares that GPU's
+// copy: calls that overlap on one GPU (two host threads, two contexts) race
+// on it.  Each call clears it before its render.
+constexpr int kScratchCounters = 2;
+static __device__ __align__(256) unsigned long long scratch[kScratchCounters];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_dev[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on `device`, looked up once per device with
+// that device current (each GPU holds its own copy at its own address).
+int scratch_on(int device, unsigned long long** out) {
+  if (device < 0 || device >= kMaxScratchDevices) return PM_ERR_INVALID;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    int cur = -1;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != device) HIPCHK(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p) {
+      set_last_hip(e, "hipGetSymbolAddress(scratch)");
+      return PM_ERR_HIP;
+    }
+    g_scratch_dev[device] = static_cast<unsigned long long*>(p);
+  }
+  *out = g_scratch_dev[device];
+  return PM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pm_render_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev, pm_uchar4* out_dev) {
+  return render_crop_impl(c, x0, y0, w, h, rgb_dev, out_dev, nullptr);
+}
+
+int pm_render_crop_noalloc(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev, pm_uchar4* out_dev) {
+  if (!c) return PM_ERR_INVALID;
+  unsigned long long* counters = nullptr;
+  int r = scratch_on(c->device, &counters);
+  if (r != PM_OK) return r;
+  return render_crop_impl(c, x0, y0, w, h, rgb_dev, out_dev, counters);
 }
 
 int pm_render(pm_context* c, float* rgb_dev, pm_uchar4* out_dev) {

@@ -153,6 +153,16 @@ int pm_build(pm_context* ctx);
  * w*h*3 FP32 radiance (may be NULL), out_dev w*h uchar4 (may be NULL). */
 int pm_render(pm_context* ctx, float* rgb_dev, pm_uchar4* out_dev);
 int pm_render_crop(pm_context* ctx, int x0, int y0, int w, int h, float* rgb_dev, pm_uchar4* out_dev);
+/* pm_render_crop without any allocation: no reference counterpart.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops).  Same arguments, outputs and statistics as
+ * pm_render_crop.  The render's only scratch, its two 64-bit query counters
+ * (16 bytes), lives in a library-wide device array, one per GPU, held from
+ * library load to process end; the call clears it with a kernel before the
+ * render.  Calls that overlap on one GPU (several host threads or contexts)
+ * share that array and race on it.  The context must already be built (its
+ * photon maps and trees are allocated by pm_emit/pm_build). */
+int pm_render_crop_noalloc(pm_context* ctx, int x0, int y0, int w, int h, float* rgb_dev, pm_uchar4* out_dev);
 int pm_stats_get(pm_context* ctx, pm_stats* out);
 /* Change the per-render parameters (resolution, spp, k, caps) without re-emitting. */
 int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface, int k_volume,
