@@ -18,7 +18,8 @@ _REPO = os.path.dirname(_HERE)
 sys.path.insert(0, os.path.join(_REPO, "cuda-photon-mapper_amd"))
 from pmamd import PmConfig  # noqa: E402  (struct layout only)
 
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# ORACLE_LIB selects another build of the same source (the sanitizer build).
+LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 _P = ctypes.c_void_p

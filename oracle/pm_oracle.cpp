@@ -575,7 +575,7 @@ int orc_build(void* h) {
 int orc_get_tree(void* h, int map, uint32_t* perm, float* split, uint32_t* axis, uint32_t n, uint32_t ni) {
   Map& m = map_of(*(Ctx*)h, map);
   if (!m.built || n != m.size() || ni != (m.nl ? m.nl - 1 : 0)) return PM_ERR_INVALID;
-  memcpy(perm, m.perm.data(), sizeof(uint32_t) * n);
+  if (n) memcpy(perm, m.perm.data(), sizeof(uint32_t) * n);
   if (ni) {
     memcpy(split, m.split.data(), sizeof(float) * ni);
     memcpy(axis, m.axis.data(), sizeof(uint32_t) * ni);
@@ -687,7 +687,7 @@ int orc_density(void* h, const float* p, float* out, uint32_t n) {
 int orc_noise_grid(void* h, float* out, uint32_t n) {
   Ctx* c = (Ctx*)h;
   if (n != c->density.size()) return PM_ERR_INVALID;
-  memcpy(out, c->density.data(), sizeof(float) * n);
+  if (n) memcpy(out, c->density.data(), sizeof(float) * n);
   return 0;
 }
 
