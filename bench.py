@@ -10,14 +10,17 @@ the hit).  Inputs are synthetic (the scene is generated from Philox seed 1).
 Prints one JSON line.  `value` = radiance queries (surface + volume kNN
 density estimates) per second of whole-frame time.  `roofline` is for the
 dominant kernel (the render/gather kernel), with its duration measured by HIP
-events on the library's own stream inside libpm.so.  `cpu_baseline` times the
-CPU oracle (same per-thread logic, std::thread over the host cores) on a
-stratified sample of the same frame.  `cache` reports the learned-cache form of
-the same frame (queries recorded, MLP trained online on exact kNN targets,
-MLP inference with exact kNN fallback): ms/frame, speed-up over the plain
-frame, its error against the plain frame on the same eye samples, and the
-MFMA rate of the MLP inference.  `batch` reports the batched workload (64
-random scenes per frame through one hipGraph launch) against the same scenes
+events on the library's own stream inside libpm.so; it also carries the
+kernel's VALU issue rate from the committed counter profile, the roof it
+actually sits on.  `cpu_baseline` times the CPU oracle (same per-thread logic,
+std::thread over the host-core share of one GPU) on a stratified sample of the
+same frame, and the 256x256 unit-room workload in full.  `cache_bf16` reports
+the frame in its BF16-power + learned-cache form (queries recorded, MLP
+trained online on exact kNN targets, MLP inference with exact kNN fallback):
+ms/frame, speed-up over the plain FP32 frame, and its error against the FP64
+oracle render of stratified rows (also given for the plain FP32 frame as
+`plain_fp32_vs_fp64_oracle`).  `batch` reports the batched workload (64 random
+scenes per frame through one hipGraph launch) against the same scenes
 rendered one at a time.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
@@ -172,15 +175,30 @@ def rel_rmse(a, b, exposure=None):
     return float(np.sqrt(((a.astype(np.float64) - b) ** 2).mean()) / np.abs(b).mean())
 
 
-def run_cache(g, cfg, steps, train_frames, stride, threshold, batch, plain_img):
-    """Cache frames of the same workload: emit + build + cache render (record
-    queries, train on 2^18 exact kNN targets, MLP inference, exact fallback),
-    after `train_frames` frames of online training from a fresh network."""
+def northstar_cfg():
+    """The 1080p workload in the north-star precision form: photon power and
+    direction stored BF16 (positions and split planes FP32), FP64
+    transmittance on long segments through the density field."""
+    c = workload_cfg()
+    c.storage = pmamd.PM_STORE_BF16
+    c.fp64_transmittance = 1
+    return c
+
+
+def run_northstar(steps, train_frames, stride, thresholds, batch, rows):
+    """Cache frames of the 1080p workload in the BF16 form: emit + build + cache
+    render (record queries, train on 2^18 exact kNN targets, MLP inference,
+    exact fallback), after `train_frames` frames of online training from a
+    fresh network, at each fallback threshold in turn (training continues).
+    Returns the timings and the frame rows `rows` of each setting (their error
+    against the FP64 oracle is computed by the CPU leg)."""
     lib = pmamd.load()
+    cfg = northstar_cfg()
+    g = pmamd.PhotonMapper(cfg)
     npx = cfg.width * cfg.height
     rgb = pmamd.DeviceBuffer(npx * 12)
     g.cache_reset(1)
-    g.cache_configure(threshold=threshold, batch=batch, march_stride=stride)
+    g.cache_configure(threshold=thresholds[0], batch=batch, march_stride=stride)
 
     def frame():
         g.emit()
@@ -190,33 +208,58 @@ def run_cache(g, cfg, steps, train_frames, stride, threshold, batch, plain_img):
 
     for i in range(train_frames):
         cs = frame()
-        if i % 4 == 0:
+        if i % 8 == 0:
             log("cache training frame %d: %.1f ms, loss %.4g" % (i, cs.ms_total, cs.train_loss))
-    per = []
-    t0 = time.perf_counter()
-    for i in range(steps):
-        cs = frame()
-        per.append((cs.ms_record, cs.ms_train, cs.ms_infer, cs.ms_fallback, cs.ms_accumulate, cs.ms_total))
-        log("cache step %d: %.1f ms (infer %.1f ms), fallback %d of %d" % (i, cs.ms_total, cs.ms_infer, cs.fallbacks,
-                                                                          cs.records))
-    t1 = time.perf_counter()
-    img = rgb.download(np.float32, npx * 3).reshape(-1, 3)
-    m = np.mean(np.array(per), axis=0)
-    infer_flop = 154368.0 * cs.records
-    return {
-        "ms_per_frame": round((t1 - t0) / steps * 1e3, 3),
-        "breakdown_ms": {"record": round(m[0], 3), "train": round(m[1], 3), "infer": round(m[2], 3),
-                         "fallback": round(m[3], 3), "accumulate": round(m[4], 3), "render_total": round(m[5], 3)},
-        "march_stride": stride, "threshold": threshold, "adam_batch": batch, "train_samples_per_frame": 1 << 18,
-        "train_frames_before_timing": train_frames,
-        "cache_queries": int(cs.records), "fallback_queries": int(cs.fallbacks), "train_loss": round(cs.train_loss, 5),
-        "rmse_vs_plain_same_samples": {"display_rel": round(rel_rmse(img, plain_img, cfg.exposure), 5),
-                                       "linear_rel": round(rel_rmse(img, plain_img), 5)},
-        "mlp_inference_mfma": {"bound": "mfma", "achieved": round(infer_flop / (m[2] * 1e-3) / 1e12, 2),
-                               "peak": 2500.0, "unit": "TFLOP/s",
-                               "frac": round(infer_flop / (m[2] * 1e-3) / 1e12 / 2500.0, 4),
-                               "flop_per_query": 154368},
-    }
+    out = []
+    for th in thresholds:
+        g.cache_configure(threshold=th, batch=batch, march_stride=stride)
+        frame()
+        per = []
+        t0 = time.perf_counter()
+        for i in range(steps):
+            cs = frame()
+            per.append((cs.ms_record, cs.ms_train, cs.ms_infer, cs.ms_fallback, cs.ms_accumulate, cs.ms_total))
+            log("cache step %d (threshold %g): %.1f ms (infer %.1f ms), fallback %d of %d"
+                % (i, th, cs.ms_total, cs.ms_infer, cs.fallbacks, cs.records))
+        t1 = time.perf_counter()
+        img = rgb.download(np.float32, npx * 3).reshape(cfg.height, cfg.width, 3)
+        m = np.mean(np.array(per), axis=0)
+        infer_flop = 154368.0 * cs.records
+        out.append({
+            "threshold": th,
+            "ms_per_frame": round((t1 - t0) / steps * 1e3, 3),
+            "breakdown_ms": {"record": round(m[0], 3), "train": round(m[1], 3), "infer": round(m[2], 3),
+                             "fallback": round(m[3], 3), "accumulate": round(m[4], 3),
+                             "render_total": round(m[5], 3)},
+            "cache_queries": int(cs.records), "fallback_queries": int(cs.fallbacks),
+            "train_loss": round(cs.train_loss, 5),
+            "mlp_inference_mfma": {"bound": "mfma", "achieved": round(infer_flop / (m[2] * 1e-3) / 1e12, 2),
+                                   "peak": 2500.0, "unit": "TFLOP/s",
+                                   "frac": round(infer_flop / (m[2] * 1e-3) / 1e12 / 2500.0, 4),
+                                   "flop_per_query": 154368},
+            "_rows": img[rows].copy(),
+        })
+    g.close()
+    return {"workload": "the same frame with photon power and direction stored BF16 (positions and splits FP32), "
+                        "FP64 transmittance on segments of >= 32 march steps through the density field, and the "
+                        "learned radiance cache (MLP 216 -> 4 x 128 -> 3, BF16 MFMA, Adam lr 1e-3, 2^18 training "
+                        "samples per frame, exact k=64 fallback above the threshold)",
+            "march_stride": stride, "adam_batch": batch, "train_samples_per_frame": 1 << 18,
+            "train_frames_before_timing": train_frames, "settings": out}
+
+
+def row_errors(img_rows, ref_rows, exposure):
+    """Errors of FP32 frame rows against the FP64 oracle's rows: RMSE relative
+    to the mean of the reference, on linear radiance and on the displayed
+    values (exposure, clamp to [0, 1]); relative error quantiles."""
+    a = img_rows.astype(np.float64)
+    b = ref_rows
+    da, db = np.clip(a * exposure, 0, 1), np.clip(b * exposure, 0, 1)
+    rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-6 * np.abs(b).mean())
+    return {"rmse_linear_rel": round(float(np.sqrt(((a - b) ** 2).mean()) / np.abs(b).mean()), 6),
+            "rmse_display_rel": round(float(np.sqrt(((da - db) ** 2).mean()) / db.mean()), 6),
+            "rel_err_median": float("%.4g" % np.median(rel)), "rel_err_p99": float("%.4g" % np.quantile(rel, 0.99)),
+            "rel_err_max": float("%.4g" % rel.max())}
 
 
 def run_batch(steps, tune_stride, n_scenes=64, seed=1):
@@ -272,11 +315,34 @@ def run_batch(steps, tune_stride, n_scenes=64, seed=1):
     }
 
 
-def run_cpu(cfg, budget_s):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+# Host cores given to one GPU on the benchmark box: the oracle's threads are
+# sized to that share, whatever the machine's total.
+CPU_SHARE_PER_GPU = 16
+
+
+def run_cpu(cfg, budget_s, ref_rows=None):
+    """The cpu_baseline leg: the CPU oracle (the same per-thread logic, g++ -O2,
+    std::thread) timed on this host.  (1) The 1080p workload: full emit and
+    kd-build, then stratified rows of the frame until `budget_s` is spent
+    (extrapolated).  (2) The 256x256 unit-room workload (100k photons, k=32)
+    in full.  (3) As the checker, the FP64 oracle render of the frame rows
+    `ref_rows` (the reference for the BF16 + cache frame's error)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
 
-    cores = min(16, len(os.sched_getaffinity(0)))
+    aff = len(os.sched_getaffinity(0))
+    cores = min(CPU_SHARE_PER_GPU, aff)
     o = O.Oracle(cfg, cores)
     t = time.perf_counter()
     o.emit()
@@ -304,12 +370,37 @@ def run_cpu(cfg, budget_s):
         if t_render > budget_s:
             break
     qps = queries / t_render
-    return dict(qps=qps, cores=cores, rows=rows, queries=queries, t_render=t_render, t_emit=t_emit, t_build=t_build,
-                cpu=platform.processor() or platform.machine())
+    ref = None
+    t_ref = 0.0
+    if ref_rows is not None:
+        t = time.perf_counter()
+        ref = np.stack([o.render(0, int(y), cfg.width, 1, fp64=True)[0] for y in ref_rows])
+        t_ref = time.perf_counter() - t
+    del o
+    c1 = pmamd.preset(1)
+    o1 = O.Oracle(c1, cores)
+    t = time.perf_counter()
+    o1.emit()
+    t1_emit = time.perf_counter() - t
+    t = time.perf_counter()
+    o1.build()
+    t1_build = time.perf_counter() - t
+    t = time.perf_counter()
+    o1.render()
+    t1_render = time.perf_counter() - t
+    q1 = sum(o1.last_queries)
+    cfg1 = {"workload": "unit room, 100k photons, homogeneous medium, k=32, r<=0.05, 256x256 x 1 spp, in full",
+            "emit_s": round(t1_emit, 3), "build_s": round(t1_build, 3), "render_s": round(t1_render, 3),
+            "radiance_queries": q1, "radiance_queries_per_s": round(q1 / t1_render, 1),
+            "photons_traced_per_s": round(c1.n_emit / t1_emit, 1),
+            "ms_per_frame": round((t1_emit + t1_build + t1_render) * 1e3, 1)}
+    return dict(qps=qps, cores=cores, cores_affinity=aff, cores_total=os.cpu_count(), rows=rows, queries=queries,
+                t_render=t_render, t_emit=t_emit, t_build=t_build, cpu=cpu_model(), cfg1=cfg1, ref=ref, t_ref=t_ref)
 
 
-def read_traffic():
-    p = os.path.join(REPO, "profiles", "traffic_render_cfg2.json")
+def read_profile(name):
+    """A committed counter summary under profiles/ (None if absent)."""
+    p = os.path.join(REPO, "profiles", name)
     if not os.path.exists(p):
         return None
     try:
@@ -317,6 +408,11 @@ def read_traffic():
             return json.load(f)
     except Exception:
         return None
+
+
+# VALU issue peak of MI355X: 256 CUs x 4 SIMDs, one wave64 VALU instruction per
+# SIMD every 4 cycles at the 2.4 GHz peak engine clock.
+VALU_ISSUE_PEAK_PER_S = 256 * 4 * 2.4e9 / 4
 
 
 def main():
@@ -337,7 +433,10 @@ def main():
     ap.add_argument("--cache-train-frames", type=int, default=48)
     ap.add_argument("--cache-batch", type=int, default=8192, help="Adam batch (2^18 training samples per frame)")
     ap.add_argument("--cache-stride", type=int, default=4)
-    ap.add_argument("--cache-threshold", type=float, default=0.12)
+    ap.add_argument("--cache-thresholds", type=str, default="0.12,0.05",
+                    help="fallback thresholds timed in turn (comma-separated)")
+    ap.add_argument("--error-rows", type=int, default=16,
+                    help="stratified frame rows rendered by the FP64 oracle for the error of the BF16 + cache frame")
     ap.add_argument("--batch-steps", type=int, default=3, help="timed batch frames (0: skip the batch leg)")
     ap.add_argument("--batch-tune-stride", type=int, default=4)
     ap.add_argument("--copy-calibrate", type=int, default=0,
@@ -364,12 +463,20 @@ def main():
     kern_ms = float(np.mean([p[3] for p in per]))
     bpq = gather_bytes_per_query(cfg.k_volume, cfg.storage == pmamd.PM_STORE_BF16)
     achieved = bpq * q_frame / (kern_ms * 1e-3) / 1e9
-    tr = read_traffic()
+    tr = read_profile("traffic_render_cfg2.json")
+    vp = read_profile("valu_render_cfg2.json")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": (tr.get("bytes_per_launch") if tr else None),
-            "kernel": "k_render_lane7 (eye ray + march + kNN gather)", "bytes_per_query": bpq,
+            "kernel": "k_render_lane7s (eye ray + march + kNN gather, a pixel's samples in adjacent lanes)",
+            "bytes_per_query": bpq,
             "queries_per_launch": int(q_frame), "kernel_ms": round(kern_ms, 3)}
+    if vp:
+        # the roof the kernel actually sits on: VALU instruction issue
+        roof["valu_issue"] = {"insts_per_launch": vp["sq_insts_valu"], "peak_insts_per_s": VALU_ISSUE_PEAK_PER_S,
+                              "frac": round(vp["sq_insts_valu"] / (kern_ms * 1e-3) / VALU_ISSUE_PEAK_PER_S, 4),
+                              "salu_insts_per_launch": vp.get("sq_insts_salu"),
+                              "source": "profiles/valu_render_cfg2.json (rocprofv3 SQ_INSTS_VALU of the same kernel)"}
     photons_traced = stats.n_emit_total
     emit_ms = float(np.mean([p[0] for p in per]))
     out = {
@@ -385,7 +492,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "cfg2 plain FP32 frame: " + workload_desc(cfg),
+        "config": {"workload": "1080p workload, plain FP32 frame: " + workload_desc(cfg),
                    "photons_emitted": int(photons_traced), "surface_records": int(stats.n_surface),
                    "volume_records": int(stats.n_volume), "surface_queries": int(stats.surface_queries),
                    "volume_queries": int(stats.volume_queries), "sigma_a_used": cfg.sigma_a,
@@ -398,22 +505,49 @@ def main():
     }
     if args.surface_steps > 0:
         out["surface_gather"] = run_surface_gather(g, cfg, args.surface_steps)
+    g.close()
+    H = cfg.height
+    err_rows = [int((i + 0.5) * H / args.error_rows) for i in range(args.error_rows)]
+    plain_rows = img.reshape(H, cfg.width, 3)[err_rows].copy()
+    ns = None
     if args.cache_steps > 0:
-        ca = run_cache(g, cfg, args.cache_steps, args.cache_train_frames, args.cache_stride, args.cache_threshold,
-                       args.cache_batch, img.reshape(-1, 3))
-        ca["speedup_vs_plain_frame"] = round(out["ms_per_step"] / ca["ms_per_frame"], 3)
-        out["cache"] = ca
+        ths = [float(v) for v in args.cache_thresholds.split(",") if v]
+        ns = run_northstar(args.cache_steps, args.cache_train_frames, args.cache_stride, ths, args.cache_batch,
+                           err_rows)
+        for st in ns["settings"]:
+            st["speedup_vs_plain_fp32_frame"] = round(out["ms_per_step"] / st["ms_per_frame"], 3)
+            st["rmse_vs_plain_fp32_same_samples"] = {
+                k: v for k, v in row_errors(st["_rows"], plain_rows.astype(np.float64), cfg.exposure).items()
+                if k.startswith("rmse")}
     if args.batch_steps > 0:
         out["batch"] = run_batch(args.batch_steps, args.batch_tune_stride)
     if not args.no_cpu:
-        cb = run_cpu(cfg, args.cpu_seconds)
+        cb = run_cpu(cfg, args.cpu_seconds, err_rows if ns is not None else None)
         out["cpu_baseline"] = {
             "value": round(cb["qps"], 1), "unit": "radiance queries/s", "cores": cb["cores"], "kind": "port",
-            "sample": "CPU oracle (g++ -O2, std::thread x %d): full cfg2 emit (%.1f s) and kd-build (%.1f s), then "
-                      "render of %d stratified rows of the 1920x1080x4 frame (%d radiance queries in %.1f s)"
-                      % (cb["cores"], cb["t_emit"], cb["t_build"], cb["rows"], cb["queries"], cb["t_render"]),
+            "sample": "CPU oracle (g++ -O2, std::thread x %d, the host-core share of one GPU; the host has %d cores "
+                      "in its affinity mask, %d in total, %s): full emit (%.1f s) and kd-build (%.1f s) of the "
+                      "1080p workload, then render of %d stratified rows of its 1920x1080x4 frame (%d radiance "
+                      "queries in %.1f s), extrapolated to the frame"
+                      % (cb["cores"], cb["cores_affinity"], cb["cores_total"], cb["cpu"], cb["t_emit"], cb["t_build"],
+                         cb["rows"], cb["queries"], cb["t_render"]),
             "est_ms_per_frame": round(q_frame / cb["qps"] * 1e3 + (cb["t_emit"] + cb["t_build"]) * 1e3, 1),
+            "host_cores_affinity": cb["cores_affinity"], "host_cores_total": cb["cores_total"],
+            "cpu_model": cb["cpu"],
+            "unit_room_256_in_full": cb["cfg1"],
         }
+        if cb["ref"] is not None:
+            ref = cb["ref"]
+            out["plain_fp32_vs_fp64_oracle"] = row_errors(plain_rows, ref, cfg.exposure)
+            for st in ns["settings"]:
+                st["error_vs_fp64_oracle"] = row_errors(st["_rows"], ref, cfg.exposure)
+            ns["fp64_reference"] = ("oracle FP64 render (FP32 photon records, FP64 eye paths, march and estimates) "
+                                    "of %d stratified rows x 1920 px x 4 spp (%.1f s on the host)"
+                                    % (len(err_rows), cb["t_ref"]))
+    if ns is not None:
+        for st in ns["settings"]:
+            del st["_rows"]
+        out["cache_bf16"] = ns
     print(json.dumps(out), flush=True)
 
 
