@@ -123,6 +123,10 @@ int cache_predict_dev(pm_context* c, const float* in, uint32_t n, float* rgb) {
   if (k.fused_infer) {
     int maxlds = 0;
     HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+    if (k.fused_infer == 3 && (size_t)maxlds >= mlp_fused_lds_bytes()) {
+      HIPCHK(mlp_infer_fused(k.w, in, n, rgb, c->stream));
+      return PM_OK;
+    }
     if ((size_t)maxlds >= mlp_infer_lds_bytes()) {
       HIPCHK(mlp_infer(k.w, in, n, k.H[0].as<uint16_t>(), rgb, c->stream, k.fused_infer == 2 ? 1 : 0,
                        1u << k.infer_chunk_log2));

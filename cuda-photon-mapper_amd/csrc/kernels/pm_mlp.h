@@ -71,5 +71,8 @@ size_t mlp_infer_lds_bytes();
 // rows per launch pair (layer 1, layers 2..5) = min(chunk_rows, mlp_infer_h1_rows()).
 hipError_t mlp_infer(const MlpDev& w, const float* in, uint32_t n, uint16_t* h1, float* rgb, hipStream_t st,
                      int k32, uint32_t chunk_rows);
+// All five layers in one kernel with the activations in registers (pm_mlp_infer.hip).
+size_t mlp_fused_lds_bytes();
+hipError_t mlp_infer_fused(const MlpDev& w, const float* in, uint32_t n, float* rgb, hipStream_t st);
 
 }  // namespace pm

@@ -430,6 +430,203 @@ __global__ void __launch_bounds__(IW25) k_mlp_l25x(L25Args A) {
   }
 }
 
+// ---------------------------------------------------------------- one fused kernel
+// All five layers in one kernel, the activations never leaving registers.
+// The layers are computed transposed, out^T = W . H^T: the weights are the
+// A operand (read from LDS) and a wave's 64 rows are the B operand, as four
+// 16-row tiles.  The D fragment of m-tile c holds, in lane l, neurons
+// 16c + 4(l/16) + e (e = 0..3) of row l%16, and that is almost the B layout
+// the next layer needs (8 consecutive K values of row l%16 per lane).  The
+// K order inside an MFMA is free as long as A and B agree, so K step s of the
+// next layer takes, in lane group g = l/16, the neurons
+//   n(s, 8g + e) = 32s + 16(e / 4) + 4g + e%4,   e = 0..7,
+// i.e. the D fragments of m-tiles 2s and 2s + 1 of the same lane, and its
+// weights are stored in LDS with their columns in that order.  No LDS stage,
+// no cross-lane exchange.  Each weight fragment read from LDS feeds 4 MFMAs.
+//
+// LDS (bf16): every layer as K/32 blocks of [nout x 32] (64 B rows, four
+// 16-B chunks; chunk g of row r at position g ^ ((r / 4) % 4), so the 8
+// lanes of a read phase hit distinct banks): W1 7 x 128 rows, W2..W4 4 x 128,
+// W5 4 x 16.  159,744 B, one workgroup of 8 waves (2 per SIMD) per CU.
+static constexpr int IWF = 512;
+static constexpr int FT = 4;                       // 16-row tiles per wave
+static constexpr int FROWS = 16 * FT;              // rows per wave step
+static constexpr uint32_t FW1 = 0;                 // byte offsets of the layers
+static constexpr uint32_t FW2 = FW1 + 7u * 128u * 64u;
+static constexpr uint32_t FW3 = FW2 + 4u * 128u * 64u;
+static constexpr uint32_t FW4 = FW3 + 4u * 128u * 64u;
+static constexpr uint32_t FW5 = FW4 + 4u * 128u * 64u;
+static constexpr uint32_t FLDS = FW5 + 4u * 16u * 64u;
+
+__device__ __forceinline__ uint32_t fchunk(uint32_t nout, uint32_t ks, uint32_t r, uint32_t g) {
+  return ks * nout * 64u + r * 64u + 16u * (g ^ ((r >> 2) & 3u));
+}
+
+struct FusedArgs {
+  const float* in;
+  uint32_t rows;
+  const uint16_t* w[MLP_LAYERS];  // BF16 weights, row-major [nout x kin]
+  const float* b[MLP_LAYERS];
+  float* rgb;
+};
+
+// bias + ReLU of m-tiles 2s and 2s + 1 into the next layer's B fragment for K step s
+__device__ __forceinline__ bf16x8 next_b(const f32x4& d0, const f32x4& d1, const float* bias, int s, int g) {
+  const float4 b0 = *(const float4*)&bias[32 * s + 4 * g];
+  const float4 b1 = *(const float4*)&bias[32 * s + 16 + 4 * g];
+  float f[8] = {d0[0] + b0.x, d0[1] + b0.y, d0[2] + b0.z, d0[3] + b0.w,
+                d1[0] + b1.x, d1[1] + b1.y, d1[2] + b1.z, d1[3] + b1.w};
+#pragma unroll
+  for (int e = 0; e < 8; ++e) f[e] = f[e] > 0.0f ? f[e] : 0.0f;
+  return pack8(f);
+}
+
+__global__ void __launch_bounds__(IWF) k_mlp_fused(FusedArgs A) {
+  extern __shared__ __align__(16) unsigned char flds[];
+  // weights -> LDS: one 16-B chunk per step; layers 2..5 in the permuted K order
+  {
+    const uint32_t c1 = 128u * 28u, c24 = 3u * 128u * 16u, c5 = 16u * 16u;
+    for (uint32_t e = threadIdx.x; e < c1 + c24 + c5; e += IWF) {
+      uint4 v;
+      uint32_t dst;
+      if (e < c1) {
+        const uint32_t r = e / 28u, k8 = e % 28u;  // W1 row r, features 8 k8 .. 8 k8 + 7
+        v = *(const uint4*)&A.w[0][(size_t)r * MLP_KPAD + 8u * k8];
+        dst = FW1 + fchunk(128u, k8 >> 2, r, k8 & 3u);
+      } else {
+        const uint32_t e2 = e - c1;
+        const uint32_t l = e2 < c24 ? 1u + e2 / (128u * 16u) : 4u;
+        const uint32_t f = e2 < c24 ? e2 % (128u * 16u) : e2 - c24;
+        const uint32_t r = f / 16u, kc = f % 16u, ks = kc >> 2, g = kc & 3u;
+        const uint16_t* w = A.w[l] + (size_t)r * MLP_HID;
+        const uint2 lo = *(const uint2*)&w[32u * ks + 4u * g];
+        const uint2 hi = *(const uint2*)&w[32u * ks + 16u + 4u * g];
+        v = make_uint4(lo.x, lo.y, hi.x, hi.y);
+        const uint32_t base = l == 1u ? FW2 : (l == 2u ? FW3 : (l == 3u ? FW4 : FW5));
+        dst = base + fchunk(l == 4u ? 16u : 128u, ks, r, g);
+      }
+      *(uint4*)&flds[dst] = v;
+    }
+  }
+  __syncthreads();
+  const int lane = lane_id(), wave = threadIdx.x >> 6;
+  const int j = lane & 15, g = lane >> 4;
+  const uint32_t steps = (A.rows + FROWS - 1) / FROWS;
+  for (uint32_t t = blockIdx.x * (IWF / 64) + wave; t < steps; t += gridDim.x * (IWF / 64)) {
+    const uint32_t row0 = t * FROWS;
+    f32x4 acc[8][FT];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+      for (int u = 0; u < FT; ++u) acc[c][u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    // layer 1: the encoding computed as B fragments (4 bands of one input per lane and K step)
+    for (int ks = 0; ks < MLP_KPAD / 32; ++ks) {
+      const int f0 = ks * 32 + 8 * g;
+      bf16x8 bf[FT];
+#pragma unroll
+      for (int u = 0; u < FT; ++u) {
+        const uint32_t row = row0 + 16u * (uint32_t)u + (uint32_t)j;
+        float f[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (f0 < MLP_FEAT && row < A.rows) {
+          const int i = f0 / (2 * MLP_BANDS), b = (f0 % (2 * MLP_BANDS)) / 2;
+          const float v = A.in[(size_t)row * MLP_IN + i];
+          float sn, cs;
+          sincos_turns(v * u2f((uint32_t)(126 + b) << 23), &sn, &cs);  // 2^(b-1)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            f[2 * q] = sn;
+            f[2 * q + 1] = cs;
+            const float s2 = 2.0f * (sn * cs), c2 = (cs - sn) * (cs + sn);
+            sn = s2;
+            cs = c2;
+          }
+        }
+        bf[u] = pack8(f);
+      }
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const bf16x8 a = *(const bf16x8*)&flds[FW1 + fchunk(128u, (uint32_t)ks, (uint32_t)(16 * c + j), (uint32_t)g)];
+#pragma unroll
+        for (int u = 0; u < FT; ++u) acc[c][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bf[u], acc[c][u], 0, 0, 0);
+      }
+    }
+    bf16x8 hb[4][FT];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int u = 0; u < FT; ++u) hb[s][u] = next_b(acc[2 * s][u], acc[2 * s + 1][u], A.b[0], s, g);
+    // layers 2..4
+#pragma unroll 1
+    for (int l = 1; l < MLP_HIDDEN_LAYERS; ++l) {
+      const uint32_t base = l == 1 ? FW2 : (l == 2 ? FW3 : FW4);
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int u = 0; u < FT; ++u) acc[c][u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const bf16x8 a = *(const bf16x8*)&flds[base + fchunk(128u, (uint32_t)s, (uint32_t)(16 * c + j), (uint32_t)g)];
+#pragma unroll
+          for (int u = 0; u < FT; ++u) acc[c][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, hb[s][u], acc[c][u], 0, 0, 0);
+        }
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int u = 0; u < FT; ++u) hb[s][u] = next_b(acc[2 * s][u], acc[2 * s + 1][u], A.b[l], s, g);
+    }
+    // layer 5: 16 output rows (3 used), read by lane group 0
+    f32x4 y[FT];
+#pragma unroll
+    for (int u = 0; u < FT; ++u) y[u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const bf16x8 a = *(const bf16x8*)&flds[FW5 + fchunk(16u, (uint32_t)s, (uint32_t)j, (uint32_t)g)];
+#pragma unroll
+      for (int u = 0; u < FT; ++u) y[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, hb[s][u], y[u], 0, 0, 0);
+    }
+    if (g == 0) {
+      const float* b5 = A.b[MLP_HIDDEN_LAYERS];
+#pragma unroll
+      for (int u = 0; u < FT; ++u) {
+        const uint32_t row = row0 + 16u * (uint32_t)u + (uint32_t)j;
+        if (row < A.rows) {
+#pragma unroll
+          for (int e = 0; e < MLP_OUT; ++e) {
+            const float v = exp_(y[u][e] + b5[e]) - 1.0f;
+            A.rgb[(size_t)row * MLP_OUT + e] = v > 0.0f ? v : 0.0f;
+          }
+        }
+      }
+    }
+  }
+}
+
+size_t mlp_fused_lds_bytes() { return FLDS; }
+
+// Inference for n rows through the single fused kernel (no activation buffer).
+hipError_t mlp_infer_fused(const MlpDev& w, const float* in, uint32_t n, float* rgb, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  if (cus <= 0) cus = 1;
+  FusedArgs a;
+  a.in = in;
+  a.rows = n;
+  for (int l = 0; l < MLP_LAYERS; ++l) {
+    a.w[l] = w.wb[l];
+    a.b[l] = w.b[l];
+  }
+  a.rgb = rgb;
+  const uint32_t steps = (n + FROWS - 1) / FROWS;
+  const uint32_t grid = std::min<uint32_t>((uint32_t)cus, (steps + IWF / 64 - 1) / (IWF / 64));
+  hipLaunchKernelGGL(k_mlp_fused, dim3(grid), dim3(IWF), FLDS, st, a);
+  return hipGetLastError();
+}
+
 size_t mlp_infer_lds_bytes_x() { return (size_t)((3 * MLP_HID + MLP_OUTPAD) * WHS + (IW25 / 64) * 16 * WHS) * 2; }
 
 size_t mlp_infer_h1_rows() { return 1u << 20; }

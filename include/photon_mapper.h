@@ -189,9 +189,11 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
  *                          on a miss);
  *                      every mode gives the same neighbour sets and images;
  * PM_OPT_BLOCKS_PER_CU workgroups per compute unit of the persistent mode-0 grid;
- * PM_OPT_CACHE_FUSED_INFER 2 = cache inference through the fused LDS-resident
- *                      kernels with 16x16x32 BF16 MFMA (default), 1 = the same
- *                      with 16x16x16 BF16 MFMA, 0 = through the generic GEMMs;
+ * PM_OPT_CACHE_FUSED_INFER 3 = cache inference through one fused kernel, all
+ *                      five layers with the activations in registers and the
+ *                      weights in LDS (default), 2 = two LDS-resident kernels
+ *                      (layer 1, layers 2..5) with 16x16x32 BF16 MFMA, 1 = the
+ *                      same with 16x16x16 BF16 MFMA, 0 = the generic GEMMs;
  * PM_OPT_CACHE_INFER_CHUNK log2 of the rows per fused-inference launch pair,
  *                      14..20 (default 20): smaller chunks keep the layer-1
  *                      activations in the on-die caches between the two launches,
