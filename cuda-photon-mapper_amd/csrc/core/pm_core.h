@@ -102,13 +102,12 @@ PM_HD void sincos_turns(float u, float* s_out, float* c_out) {
   float f2 = f * f;
   float sp = f * (PM_S1 + f2 * (PM_S3 + f2 * (PM_S5 + f2 * (PM_S7 + f2 * PM_S9))));
   float cp = 1.0f + f2 * (PM_C2 + f2 * (PM_C4 + f2 * (PM_C6 + f2 * (PM_C8 + f2 * PM_C10))));
-  float s, c;
-  if (iq == 0) { s = sp; c = cp; }
-  else if (iq == 1) { s = cp; c = -sp; }
-  else if (iq == 2) { s = -sp; c = -cp; }
-  else { s = -cp; c = sp; }
-  *s_out = s;
-  *c_out = c;
+  // quadrant iq: (s, c) = (sp, cp), (cp, -sp), (-sp, -cp), (-cp, sp) for iq =
+  // 0..3, by selects and sign-bit flips (no branches, the same bits)
+  const bool odd = (iq & 1) != 0;
+  const float s = odd ? cp : sp, c = odd ? sp : cp;
+  *s_out = u2f(f2u(s) ^ ((uint32_t)(iq & 2) << 30));
+  *c_out = u2f(f2u(c) ^ ((uint32_t)((iq + 1) & 2) << 30));
 }
 
 // Natural log for finite x > 0.  x = m * 2^e with m in [sqrt(1/2), sqrt(2));

@@ -456,10 +456,24 @@ static constexpr uint32_t FW2 = FW1 + 7u * 128u * 64u;
 static constexpr uint32_t FW3 = FW2 + 4u * 128u * 64u;
 static constexpr uint32_t FW4 = FW3 + 4u * 128u * 64u;
 static constexpr uint32_t FW5 = FW4 + 4u * 128u * 64u;
-static constexpr uint32_t FLDS = FW5 + 4u * 16u * 64u;
+static constexpr uint32_t FB14 = FW5 + 4u * 16u * 64u;  // biases of layers 1..4, 4 x 128 FP32
+static constexpr uint32_t FLDS = FB14 + 4u * 128u * 4u;
 
 __device__ __forceinline__ uint32_t fchunk(uint32_t nout, uint32_t ks, uint32_t r, uint32_t g) {
   return ks * nout * 64u + r * 64u + 16u * (g ^ ((r >> 2) & 3u));
+}
+
+// Layer 1's K order in the fused kernel: lane group g takes inputs 2g and
+// 2g + 1 in K steps 0..5 (bands 0-3, 4-7, 8-11 of each), and K step 6 takes
+// bands 4g .. 4g + 3 of input 8 (g < 3; g = 3 is padding).  A lane then needs
+// three inputs per row and five sin/cos evaluations (bands 0 and 8 of two
+// inputs, band 4g of input 8), the other bands by the double-angle identities.  Returns the first of the 8 encoding columns
+// (24 input + 2 band + {sin, cos}) of step ks, group g, or -1 for padding.
+__host__ __device__ __forceinline__ int l1_input(int ks, int g) { return ks < 6 ? 2 * g + ks / 3 : (g < 3 ? 8 : -1); }
+__host__ __device__ __forceinline__ int l1_band0(int ks, int g) { return ks < 6 ? 4 * (ks % 3) : 4 * g; }
+__host__ __device__ __forceinline__ int l1_feature_col(int ks, int g) {
+  const int i = l1_input(ks, g);
+  return i < 0 ? -1 : 2 * MLP_BANDS * i + 2 * l1_band0(ks, g);
 }
 
 struct FusedArgs {
@@ -470,15 +484,24 @@ struct FusedArgs {
   float* rgb;
 };
 
-// bias + ReLU of m-tiles 2s and 2s + 1 into the next layer's B fragment for K step s
-__device__ __forceinline__ bf16x8 next_b(const f32x4& d0, const f32x4& d1, const float* bias, int s, int g) {
-  const float4 b0 = *(const float4*)&bias[32 * s + 4 * g];
-  const float4 b1 = *(const float4*)&bias[32 * s + 16 + 4 * g];
-  float f[8] = {d0[0] + b0.x, d0[1] + b0.y, d0[2] + b0.z, d0[3] + b0.w,
-                d1[0] + b1.x, d1[1] + b1.y, d1[2] + b1.z, d1[3] + b1.w};
-#pragma unroll
-  for (int e = 0; e < 8; ++e) f[e] = f[e] > 0.0f ? f[e] : 0.0f;
-  return pack8(f);
+// ReLU of m-tiles 2s and 2s + 1 (their accumulators started from the bias)
+// into the next layer's B fragment for K step s.  The ReLU is applied after
+// the BF16 rounding, as a packed 16-bit integer max with 0: rounding keeps the
+// sign, and a BF16 value is negative exactly when its bit pattern is as a
+// signed integer (-0 becomes +0), so the bits equal those of rounding max(x, 0).
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t relu_pk(uint32_t p) {
+  s16x2 v = *(const s16x2*)&p;
+  v = __builtin_elementwise_max(v, (s16x2){0, 0});
+  return *(const uint32_t*)&v;
+}
+__device__ __forceinline__ bf16x8 next_b(const f32x4& d0, const f32x4& d1) {
+  uint4 v;
+  v.x = relu_pk(cvt_pk_bf16(d0[0], d0[1]));
+  v.y = relu_pk(cvt_pk_bf16(d0[2], d0[3]));
+  v.z = relu_pk(cvt_pk_bf16(d1[0], d1[1]));
+  v.w = relu_pk(cvt_pk_bf16(d1[2], d1[3]));
+  return *(const bf16x8*)&v;
 }
 
 __global__ void __launch_bounds__(IWF) k_mlp_fused(FusedArgs A) {
@@ -490,9 +513,11 @@ __global__ void __launch_bounds__(IWF) k_mlp_fused(FusedArgs A) {
       uint4 v;
       uint32_t dst;
       if (e < c1) {
-        const uint32_t r = e / 28u, k8 = e % 28u;  // W1 row r, features 8 k8 .. 8 k8 + 7
-        v = *(const uint4*)&A.w[0][(size_t)r * MLP_KPAD + 8u * k8];
-        dst = FW1 + fchunk(128u, k8 >> 2, r, k8 & 3u);
+        // W1 row r, K step ks, lane group g: 8 features (4 bands of one input)
+        const uint32_t r = e / 28u, k8 = e % 28u, ks = k8 >> 2, g = k8 & 3u;
+        const int col = l1_feature_col((int)ks, (int)g);
+        v = col < 0 ? make_uint4(0u, 0u, 0u, 0u) : *(const uint4*)&A.w[0][(size_t)r * MLP_KPAD + col];
+        dst = FW1 + fchunk(128u, ks, r, g);
       } else {
         const uint32_t e2 = e - c1;
         const uint32_t l = e2 < c24 ? 1u + e2 / (128u * 16u) : 4u;
@@ -507,97 +532,144 @@ __global__ void __launch_bounds__(IWF) k_mlp_fused(FusedArgs A) {
       }
       *(uint4*)&flds[dst] = v;
     }
+    float* bl = (float*)&flds[FB14];
+    for (uint32_t e = threadIdx.x; e < 4u * 128u; e += IWF) bl[e] = A.b[e >> 7][e & 127u];
   }
   __syncthreads();
+  const float* bias_l = (const float*)&flds[FB14];
   const int lane = lane_id(), wave = threadIdx.x >> 6;
   const int j = lane & 15, g = lane >> 4;
   const uint32_t steps = (A.rows + FROWS - 1) / FROWS;
-  for (uint32_t t = blockIdx.x * (IWF / 64) + wave; t < steps; t += gridDim.x * (IWF / 64)) {
+  // a lane's three inputs of each tile (zero past the end): loaded one wave
+  // step ahead, so their latency hides behind the previous step's MFMAs
+  float x0[FT], x1[FT], v8[FT];
+  auto load_inputs = [&](uint32_t t) {
+#pragma unroll
+    for (int u = 0; u < FT; ++u) {
+      const uint32_t row = t * FROWS + 16u * (uint32_t)u + (uint32_t)j;
+      const bool ok = t < steps && row < A.rows;
+      const float* x = A.in + (size_t)(ok ? row : 0u) * MLP_IN;
+      x0[u] = ok ? x[2 * g] : 0.0f;
+      x1[u] = ok ? x[2 * g + 1] : 0.0f;
+      v8[u] = ok && g < 3 ? x[8] : 0.0f;
+    }
+  };
+  const uint32_t tstride = gridDim.x * (IWF / 64);
+  for (uint32_t t = blockIdx.x * (IWF / 64) + wave; t < steps; t += tstride) {
+    load_inputs(t);
     const uint32_t row0 = t * FROWS;
-    f32x4 acc[8][FT];
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
-#pragma unroll
-      for (int u = 0; u < FT; ++u) acc[c][u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    // layer 1: the encoding computed as B fragments (4 bands of one input per lane and K step)
-    for (int ks = 0; ks < MLP_KPAD / 32; ++ks) {
-      const int f0 = ks * 32 + 8 * g;
-      bf16x8 bf[FT];
+    // layer 1: the encoding computed as B fragments, in the K order of
+    // l1_feature_col: per tile, a lane's three inputs (bands 0 and 8 of the
+    // first two and band 4g of input 8 evaluated directly, the other bands by
+    // the double-angle identities: at most 7 doublings in a row keep the
+    // features within 1e-5 of direct values)
+    float sn[FT], cs[FT];
+    auto features = [&](int ks, bf16x8* bf) {
 #pragma unroll
       for (int u = 0; u < FT; ++u) {
-        const uint32_t row = row0 + 16u * (uint32_t)u + (uint32_t)j;
-        float f[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-        if (f0 < MLP_FEAT && row < A.rows) {
-          const int i = f0 / (2 * MLP_BANDS), b = (f0 % (2 * MLP_BANDS)) / 2;
-          const float v = A.in[(size_t)row * MLP_IN + i];
-          float sn, cs;
-          sincos_turns(v * u2f((uint32_t)(126 + b) << 23), &sn, &cs);  // 2^(b-1)
+        if (ks == 0) sincos_turns(x0[u] * 0.5f, &sn[u], &cs[u]);    // band 0: 2^-1 turns per unit
+        if (ks == 2) sincos_turns(x0[u] * 128.0f, &sn[u], &cs[u]);  // band 8: 2^7 turns per unit
+        if (ks == 3) sincos_turns(x1[u] * 0.5f, &sn[u], &cs[u]);    // the lane's second input
+        if (ks == 5) sincos_turns(x1[u] * 128.0f, &sn[u], &cs[u]);
+        if (ks == 6) sincos_turns(v8[u] * u2f((uint32_t)(126 + 4 * g) << 23), &sn[u], &cs[u]);  // band 4g
+        float f[8];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            f[2 * q] = sn;
-            f[2 * q + 1] = cs;
-            const float s2 = 2.0f * (sn * cs), c2 = (cs - sn) * (cs + sn);
-            sn = s2;
-            cs = c2;
-          }
+        for (int q = 0; q < 4; ++q) {
+          f[2 * q] = sn[u];
+          f[2 * q + 1] = cs[u];
+          const float s2 = 2.0f * (sn[u] * cs[u]), c2 = (cs[u] - sn[u]) * (cs[u] + sn[u]);
+          sn[u] = s2;
+          cs[u] = c2;
         }
+        // rows past the end (inputs 0) only feed their own outputs, which are
+        // not written, and the padding columns of W1 are zero in LDS: no masking
         bf[u] = pack8(f);
       }
+    };
+    // the accumulators start from the bias of their neurons 16c + 4g + e
+    auto bias4 = [&](const float* b, int c) { return *(const f32x4*)&b[16 * c + 4 * g]; };
+    f32x4 acc[8][FT];
+    {
+      // K steps fully unrolled, the features of step ks + 1 computed while the
+      // MFMAs of step ks run (two fragment buffers, no copies)
+      bf16x8 fa[FT], fb[FT];
+      features(0, fa);
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const bf16x8 a = *(const bf16x8*)&flds[FW1 + fchunk(128u, (uint32_t)ks, (uint32_t)(16 * c + j), (uint32_t)g)];
+      for (int ks = 0; ks < MLP_KPAD / 32; ++ks) {
+        bf16x8* cur = (ks & 1) ? fb : fa;
+        bf16x8* nxt = (ks & 1) ? fa : fb;
 #pragma unroll
-        for (int u = 0; u < FT; ++u) acc[c][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bf[u], acc[c][u], 0, 0, 0);
+        for (int c = 0; c < 8; ++c) {
+          const bf16x8 a = *(const bf16x8*)&flds[FW1 + fchunk(128u, (uint32_t)ks, (uint32_t)(16 * c + j), (uint32_t)g)];
+          const f32x4 b0 = ks == 0 ? bias4(bias_l, c) : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+          for (int u = 0; u < FT; ++u)
+            acc[c][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, cur[u], ks == 0 ? b0 : acc[c][u], 0, 0, 0);
+        }
+        if (ks + 1 < MLP_KPAD / 32) features(ks + 1, nxt);
+        // keep each step's work in its step: scheduling the unrolled steps
+        // together would raise the register pressure past the budget
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
     bf16x8 hb[4][FT];
 #pragma unroll
     for (int s = 0; s < 4; ++s)
 #pragma unroll
-      for (int u = 0; u < FT; ++u) hb[s][u] = next_b(acc[2 * s][u], acc[2 * s + 1][u], A.b[0], s, g);
-    // layers 2..4
-#pragma unroll 1
+      for (int u = 0; u < FT; ++u) hb[s][u] = next_b(acc[2 * s][u], acc[2 * s + 1][u]);
+    // layers 2..4, one pair of m-tiles at a time: pair p completes the next
+    // layer's K step p, whose ReLU + packing then overlaps the MFMAs of pair
+    // p + 1 (and only two m-tiles of accumulators are live)
+#pragma unroll
     for (int l = 1; l < MLP_HIDDEN_LAYERS; ++l) {
       const uint32_t base = l == 1 ? FW2 : (l == 2 ? FW3 : FW4);
+      const float* bl = bias_l + 128 * l;
+      bf16x8 nb[4][FT];
 #pragma unroll
-      for (int c = 0; c < 8; ++c)
+      for (int p = 0; p < 4; ++p) {
+        f32x4 d[2][FT];
 #pragma unroll
-        for (int u = 0; u < FT; ++u) acc[c][u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int cc = 0; cc < 2; ++cc) {
+          const f32x4 b0 = bias4(bl, 2 * p + cc);
 #pragma unroll
-      for (int s = 0; s < 4; ++s)
+          for (int s = 0; s < 4; ++s) {
+            const bf16x8 a =
+                *(const bf16x8*)&flds[base + fchunk(128u, (uint32_t)s, (uint32_t)(16 * (2 * p + cc) + j), (uint32_t)g)];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          const bf16x8 a = *(const bf16x8*)&flds[base + fchunk(128u, (uint32_t)s, (uint32_t)(16 * c + j), (uint32_t)g)];
-#pragma unroll
-          for (int u = 0; u < FT; ++u) acc[c][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, hb[s][u], acc[c][u], 0, 0, 0);
-        }
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
-#pragma unroll
-        for (int u = 0; u < FT; ++u) hb[s][u] = next_b(acc[2 * s][u], acc[2 * s + 1][u], A.b[l], s, g);
-    }
-    // layer 5: 16 output rows (3 used), read by lane group 0
-    f32x4 y[FT];
-#pragma unroll
-    for (int u = 0; u < FT; ++u) y[u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const bf16x8 a = *(const bf16x8*)&flds[FW5 + fchunk(16u, (uint32_t)s, (uint32_t)j, (uint32_t)g)];
-#pragma unroll
-      for (int u = 0; u < FT; ++u) y[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, hb[s][u], y[u], 0, 0, 0);
-    }
-    if (g == 0) {
-      const float* b5 = A.b[MLP_HIDDEN_LAYERS];
-#pragma unroll
-      for (int u = 0; u < FT; ++u) {
-        const uint32_t row = row0 + 16u * (uint32_t)u + (uint32_t)j;
-        if (row < A.rows) {
-#pragma unroll
-          for (int e = 0; e < MLP_OUT; ++e) {
-            const float v = exp_(y[u][e] + b5[e]) - 1.0f;
-            A.rgb[(size_t)row * MLP_OUT + e] = v > 0.0f ? v : 0.0f;
+            for (int u = 0; u < FT; ++u)
+              d[cc][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, hb[s][u], s == 0 ? b0 : d[cc][u], 0, 0, 0);
           }
         }
+#pragma unroll
+        for (int u = 0; u < FT; ++u) nb[p][u] = next_b(d[0][u], d[1][u]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+#pragma unroll
+      for (int p = 0; p < 4; ++p)
+#pragma unroll
+        for (int u = 0; u < FT; ++u) hb[p][u] = nb[p][u];
+    }
+    // layer 5 (3 outputs): tile u's outputs are placed in lane group u by
+    // reading W5's rows shifted by 4u (rows 3..15 of W5 are zero), so every lane
+    // ends with the 3 outputs of one row and evaluates 3 exponentials
+    const float* b5 = A.b[MLP_HIDDEN_LAYERS];
+    const f32x4 y0 = f32x4{b5[0], b5[1], b5[2], 0.0f};
+    f32x4 y[FT];
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int u = 0; u < FT; ++u) {
+        const bf16x8 a =
+            *(const bf16x8*)&flds[FW5 + fchunk(16u, (uint32_t)s, (uint32_t)((j - 4 * u) & 15), (uint32_t)g)];
+        y[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, hb[s][u], s == 0 ? y0 : y[u], 0, 0, 0);
+      }
+    const f32x4 yg = g == 0 ? y[0] : (g == 1 ? y[1] : (g == 2 ? y[2] : y[3]));
+    const uint32_t row = row0 + 16u * (uint32_t)g + (uint32_t)j;
+    if (row < A.rows) {
+#pragma unroll
+      for (int e = 0; e < MLP_OUT; ++e) {
+        const float v = exp_(yg[e]) - 1.0f;
+        A.rgb[(size_t)row * MLP_OUT + e] = v > 0.0f ? v : 0.0f;
       }
     }
   }
