@@ -21,7 +21,9 @@ ms/frame, speed-up over the plain FP32 frame, and its error against the FP64
 oracle render of stratified rows (also given for the plain FP32 frame as
 `plain_fp32_vs_fp64_oracle`).  `batch` reports the batched workload (64 random
 scenes per frame through one hipGraph launch) against the same scenes
-rendered one at a time.
+rendered one at a time.  `voxel_grid` reports the reference's own 32^3 voxel-grid
+estimate (the compatibility mode) on the same photon maps and frame, with the
+CPU oracle's rate on rows of it.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -402,6 +404,68 @@ def run_cpu(cfg, budget_s, ref_rows=None):
                 t_render=t_render, t_emit=t_emit, t_build=t_build, cpu=cpu_model(), cfg1=cfg1, ref=ref, t_ref=t_ref)
 
 
+def run_voxel(cfg, steps, cpu_budget_s):
+    """The voxel-grid compatibility mode (PM_OPT_VOXEL_GRID, the reference's
+    own 32^3-grid estimate) on the 1080p workload's photon maps: grids and
+    summed-volume tables built once per emission, then the frame with the
+    window estimates in place of the kNN gather, without and with the bilinear
+    blend.  With cpu_budget_s > 0 the CPU oracle renders stratified rows of
+    the same frame from the same records (the checker and the CPU figure)."""
+    lib = pmamd.load()
+    g = pmamd.PhotonMapper(cfg)
+    g.emit()
+    npx = cfg.width * cfg.height
+    rgb = pmamd.DeviceBuffer(npx * 12)
+    res = {"workload": "voxel-grid estimate on the 1080p workload's photon maps: " + workload_desc(cfg),
+           "kernel": "k_render_voxel (eye ray + march, window sums from summed-volume tables)"}
+    imgs = {}
+    for mode in (1, 2):
+        g.set_option(pmamd.PM_OPT_VOXEL_GRID, mode)
+        t = time.perf_counter()
+        pmamd.check(lib.pm_render(g.h, rgb.ptr, None), "pm_render")
+        first = (time.perf_counter() - t) * 1e3
+        ms = []
+        for _ in range(steps):
+            pmamd.check(lib.pm_render(g.h, rgb.ptr, None), "pm_render")
+            ms.append(g.stats().ms_render)
+        s = g.stats()
+        q = int(s.surface_queries + s.volume_queries)
+        tk = float(np.median(ms))
+        key = "window" if mode == 1 else "bilinear"
+        res[key] = {"ms_per_frame_render": round(tk, 3), "radiance_queries": q,
+                    "radiance_queries_per_s": round(q / (tk * 1e-3), 1)}
+        if mode == 1:
+            res["first_render_ms_with_grid_build"] = round(first, 3)
+        imgs[mode] = rgb.download(np.float32, npx * 3).reshape(cfg.height, cfg.width, 3)
+        log("voxel mode %d: %.2f ms render, %d queries" % (mode, tk, q))
+    if cpu_budget_s > 0:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O
+
+        cores = min(CPU_SHARE_PER_GPU, len(os.sched_getaffinity(0)))
+        o = O.Oracle(cfg, cores)
+        for which, n_emit in ((pmamd.PM_MAP_SURFACE, s.n_emit_surface), (pmamd.PM_MAP_VOLUME, s.n_emit_volume)):
+            pos, pw, d = g.records(which)
+            o.set_records(which, pos, pw, d, n_emit)
+        rows = queries = exact = 0
+        t_render = 0.0
+        for y in range(cfg.height // 32, cfg.height, cfg.height // 16):
+            t = time.perf_counter()
+            row = o.render_voxel(0, y, cfg.width, 1)
+            t_render += time.perf_counter() - t
+            queries += sum(o.last_queries)
+            exact += int(np.array_equal(row[0], imgs[1][y].astype(np.float64)))
+            rows += 1
+            if t_render > cpu_budget_s:
+                break
+        res["cpu_oracle"] = {"radiance_queries_per_s": round(queries / t_render, 1), "cores": cores,
+                             "kind": "port", "rows": rows, "radiance_queries": queries,
+                             "seconds": round(t_render, 2), "rows_bit_exact_vs_gpu": exact}
+        log("voxel CPU oracle: %d rows, %.1f s, %d of them bit-exact against the GPU" % (rows, t_render, exact))
+    g.close()
+    return res
+
+
 def read_profile(name):
     """A committed counter summary under profiles/ (None if absent)."""
     p = os.path.join(REPO, "profiles", name)
@@ -443,6 +507,9 @@ def main():
                     help="stratified frame rows rendered by the FP64 oracle for the error of the BF16 + cache frame")
     ap.add_argument("--batch-steps", type=int, default=3, help="timed batch frames (0: skip the batch leg)")
     ap.add_argument("--batch-tune-stride", type=int, default=4)
+    ap.add_argument("--voxel-steps", type=int, default=3,
+                    help="timed frames of the voxel-grid compatibility mode (0: skip the leg)")
+    ap.add_argument("--voxel-cpu-seconds", type=float, default=8.0)
     ap.add_argument("--copy-calibrate", type=int, default=0,
                     help="run only the library's copy kernel over this many bytes (memory-counter calibration)")
     args = ap.parse_args()
@@ -525,6 +592,9 @@ def main():
                 if k.startswith("rmse")}
     if args.batch_steps > 0:
         out["batch"] = run_batch(args.batch_steps, args.batch_tune_stride)
+    if args.voxel_steps > 0:
+        out["voxel_grid"] = run_voxel(cfg, args.voxel_steps, 0.0 if args.no_cpu else args.voxel_cpu_seconds)
+        out["voxel_grid"]["knn_render_ms_same_frame"] = out["breakdown_ms"]["render"]
     if not args.no_cpu:
         cb = run_cpu(cfg, args.cpu_seconds, err_rows if ns is not None else None)
         out["cpu_baseline"] = {

@@ -181,6 +181,7 @@ int do_emit(pm_context* c) {
   }
   c->emitted = true;
   c->built = false;
+  c->vox_ready = false;
   return PM_OK;
 }
 
@@ -365,7 +366,8 @@ int pm_destroy(pm_context* c) {
   c->ms.release();
   c->mv.release();
   for (DevBuf* b : {&c->spheres_d, &c->density_d, &c->cnt_s, &c->cnt_v, &c->off_s, &c->off_v, &c->scratch,
-                    &c->tally_part, &c->tally9, &c->small, &c->counters})
+                    &c->tally_part, &c->tally9, &c->small, &c->counters, &c->vox_grid_s, &c->vox_grid_v,
+                    &c->vox_sat_s, &c->vox_sat_v})
     b->release();
   if (c->e0) (void)hipEventDestroy(c->e0);
   if (c->e1) (void)hipEventDestroy(c->e1);
@@ -417,8 +419,11 @@ namespace {
 int render_crop_impl(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev, pm_uchar4* out_dev,
                      unsigned long long* counters) {
   if (!c) return PM_ERR_INVALID;
-  if (!c->built) return PM_ERR_STATE;
+  // the voxel-grid estimate needs the records only, the k-NN gather their trees
+  if (c->voxel_grid ? !c->emitted : !c->built) return PM_ERR_STATE;
   if (w <= 0 || h <= 0 || x0 < 0 || y0 < 0 || x0 + w > c->cfg.width || y0 + h > c->cfg.height) return PM_ERR_INVALID;
+  // the caller that allocates nothing needs grids built by an earlier render
+  if (c->voxel_grid && counters && !c->vox_ready) return PM_ERR_STATE;
   RenderLaunch r;
   memset(&r, 0, sizeof(r));
   r.sc = c->scene;
@@ -443,6 +448,29 @@ int render_crop_impl(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev
     r.counters = c->counters.as<unsigned long long>();
     HIPCHK(hipMemsetAsync(c->counters.p, 0, 16, c->stream));
   }
+  if (c->voxel_grid) {
+    const size_t grid_bytes = 8ull * 3 * kVox * kVox * kVox, sat_bytes = 8ull * 3 * kSat * kSat * kSat;
+    if (!c->vox_ready) {
+      HIPCHK(c->vox_grid_s.ensure(grid_bytes));
+      HIPCHK(c->vox_grid_v.ensure(grid_bytes));
+      HIPCHK(c->vox_sat_s.ensure(sat_bytes));
+      HIPCHK(c->vox_sat_v.ensure(sat_bytes));
+      HIPCHK(launch_voxel_grids(c->ms.m, c->mv.m, c->vox_grid_s.as<unsigned long long>(),
+                                c->vox_grid_v.as<unsigned long long>(), c->vox_sat_s.as<unsigned long long>(),
+                                c->vox_sat_v.as<unsigned long long>(), c->stream));
+      c->vox_ready = true;
+    }
+    VoxelGrids G;
+    G.sat_s = c->vox_sat_s.as<uint64_t>();
+    G.sat_v = c->vox_sat_v.as<uint64_t>();
+    for (int j = 0; j < 3; ++j) {
+      G.scale_s[j] = c->cfg.light_power[j] / (float)(c->ms.m.n_emit ? c->ms.m.n_emit : 1u);
+      G.scale_v[j] = c->cfg.light_power[j] / (float)(c->mv.m.n_emit ? c->mv.m.n_emit : 1u);
+    }
+    G.interp = c->voxel_grid == 2 ? 1 : 0;
+    HIPCHK(hipEventRecord(c->e0, c->stream));
+    HIPCHK(launch_render_voxel(r, G, c->stream));
+  } else {
   HIPCHK(hipEventRecord(c->e0, c->stream));
   if (r.gather_mode == 5) {
     // auto: the two-pass gather (march passes merged, first passes counted)
@@ -463,6 +491,7 @@ int render_crop_impl(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev
     case 6: HIPCHK(launch_render_lane6(r, c->stream)); break;
     case 7: HIPCHK(launch_render_lane7(r, c->stream)); break;
     default: HIPCHK(launch_render_lane2(r, c->stream)); break;
+  }
   }
   HIPCHK(hipEventRecord(c->e1, c->stream));
   unsigned long long q[2] = {0, 0};
@@ -561,6 +590,10 @@ int pm_set_option(pm_context* c, int option, int value) {
       if (value != 0 && value != 1) return PM_ERR_INVALID;
       c->render_media = value;
       return PM_OK;
+    case PM_OPT_VOXEL_GRID:
+      if (value < 0 || value > 2) return PM_ERR_INVALID;
+      c->voxel_grid = value;
+      return PM_OK;
     default:
       return PM_ERR_INVALID;
   }
@@ -628,6 +661,17 @@ int pm_set_records(pm_context* c, int map, const float* pos, const float* power,
   }
   c->emitted = true;
   c->built = false;
+  c->vox_ready = false;
+  return PM_OK;
+}
+
+int pm_get_voxel_table(pm_context* c, int map, uint64_t* table, size_t n) {
+  if (!c || !table) return PM_ERR_INVALID;
+  if (n != 3ull * kSat * kSat * kSat) return PM_ERR_INVALID;
+  if (!c->vox_ready) return PM_ERR_STATE;
+  const DevBuf& t = map == PM_MAP_VOLUME ? c->vox_sat_v : c->vox_sat_s;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(table, t.p, 8ull * n, hipMemcpyDeviceToHost));
   return PM_OK;
 }
 
@@ -701,6 +745,11 @@ void launch_init_random_numbers_kernel(void) {
   if (r != PM_OK) legacy_fail("launch_init_random_numbers_kernel", r);
 }
 
+pm_context* pm_legacy_context(void) {
+  if (!g_legacy) launch_init_random_numbers_kernel();
+  return g_legacy;
+}
+
 static void legacy_configure(float animTime, bool interpolateFlag, bool participatingMediaFlag) {
   if (!g_legacy) launch_init_random_numbers_kernel();
   pm_config& c = g_legacy->cfg;
@@ -708,11 +757,13 @@ static void legacy_configure(float animTime, bool interpolateFlag, bool particip
   c.anim_time = animTime;
   c.media_on = participatingMediaFlag ? 1 : 0;
   c.estimator = interpolateFlag ? PM_EST_CONE : PM_EST_BOX;
+  if (g_legacy->voxel_grid) g_legacy->voxel_grid = interpolateFlag ? 2 : 1;
   if (scene_changed) {
     int r = setup_scene(g_legacy);
     if (r != PM_OK) legacy_fail("scene setup", r);
     g_legacy->emitted = false;
     g_legacy->built = false;
+    g_legacy->vox_ready = false;
   }
 }
 

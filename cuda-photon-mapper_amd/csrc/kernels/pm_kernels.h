@@ -8,6 +8,7 @@
 
 #include "../core/pm_kdtree.h"
 #include "../core/pm_transport.h"
+#include "../core/pm_voxel.h"
 
 namespace pm {
 
@@ -171,6 +172,13 @@ hipError_t launch_render_lane3(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane4(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane6(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane7(const RenderLaunch& r, hipStream_t st);
+
+// voxel-grid compatibility mode (pm_voxel.hip): both maps splatted into
+// [32^3][3] fixed-point grids, then their [33^3][3] summed-volume tables
+hipError_t launch_voxel_grids(const DevMap& ms, const DevMap& mv, unsigned long long* grid_s,
+                              unsigned long long* grid_v, unsigned long long* sat_s, unsigned long long* sat_v,
+                              hipStream_t st);
+hipError_t launch_render_voxel(const RenderLaunch& r, const VoxelGrids& G, hipStream_t st);
 
 // batched scenes (pm_trace.hip, pm_gather_lane.hip)
 // Emission: scene s owns the trace blocks [block0, block0 + nblocks), whose

@@ -36,6 +36,8 @@ PM_OPT_GATHER_MODE, PM_OPT_BLOCKS_PER_CU, PM_OPT_CACHE_FUSED_INFER, PM_OPT_CACHE
 PM_OPT_TILE_REGIONS = 5
 PM_OPT_SAMPLE_LANES = 6
 PM_OPT_RENDER_MEDIA = 7
+PM_OPT_VOXEL_GRID = 8
+VOXEL_TABLE_ENTRIES = 3 * 33 ** 3  # summed-volume table of one map's 32^3 voxel grid
 
 
 class PmSphere(ctypes.Structure):
@@ -187,6 +189,8 @@ _SIGS = {
                                       ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     "pm_get_tree": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                    ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
+    "pm_get_voxel_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t]),
+    "pm_legacy_context": (ctypes.c_void_p, []),
     "pm_knn": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int,
                               ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "pm_device_alloc": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]),
@@ -378,6 +382,12 @@ class PhotonMapper:
         check(load().pm_get_tree(self.h, which, perm.ctypes.data, split.ctypes.data, axis.ctypes.data, n, ni),
               "pm_get_tree")
         return perm, split, axis
+
+    def voxel_table(self, which):
+        """Summed-volume table [33, 33, 33, 3] (z, y, x, channel) of a map's voxel grid, uint64."""
+        t = np.empty(VOXEL_TABLE_ENTRIES, np.uint64)
+        check(load().pm_get_voxel_table(self.h, which, t.ctypes.data, t.size), "pm_get_voxel_table")
+        return t.reshape(33, 33, 33, 3)
 
     # ---- learned radiance cache
     def cache_reset(self, seed):

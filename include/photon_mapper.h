@@ -58,6 +58,15 @@ void launch_emit_photons_kernel(pm_uchar4* pos, unsigned int image_width, unsign
 void launch_photon_mapping_kernel(pm_uchar4* pos, unsigned int image_width, unsigned int image_height,
                                   float animTime, bool interpolateFlag, bool participatingMediaFlag);
 
+/* The context behind the three calls above (created by the first of them
+ * when needed), so that a legacy host can set options on it, e.g.
+ * pm_set_option(pm_legacy_context(), PM_OPT_VOXEL_GRID, 1) for the
+ * reference's own 32^3 voxel-grid estimate; with that option interpolateFlag
+ * selects the bilinear window blend (photonMappingKernel.cu:591) instead of
+ * the cone filter.  No reference counterpart: the reference chooses its
+ * estimate at compile time. */
+struct pm_context* pm_legacy_context(void);
+
 /* ------------------------------------------------------------------ pm_* API
  * Additive API (no reference counterpart; the reference configures itself
  * through compile-time macros at photonMappingKernel.cu:7-46 and __device__
@@ -168,7 +177,8 @@ int pm_stats_get(pm_context* ctx, pm_stats* out);
 int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface, int k_volume,
                   float rcap_surface, float rcap_volume);
 
-/* Execution options (no effect on results, only on how the GPU runs them):
+/* Options.  PM_OPT_RENDER_MEDIA and PM_OPT_VOXEL_GRID change what is
+ * estimated; the others change only how the GPU runs it, never the results:
  * PM_OPT_GATHER_MODE   0 = one wave64 per radiance query (cross-lane sorted list),
  *                      1 = one lane per query, 8x8 pixel tile per wave,
  *                          8-byte (d^2, index) heap keys in LDS,
@@ -210,7 +220,20 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
  * PM_OPT_RENDER_MEDIA  1 = pm_render / pm_render_crop march the medium (default,
  *                      when the scene has one), 0 = they gather at surface hits
  *                      only: the surface-gather measurement on the same photon
- *                      maps (the volume map is built but not read). */
+ *                      maps (the volume map is built but not read);
+ * PM_OPT_VOXEL_GRID    0 = the k-NN estimate (default); 1 = the reference's
+ *                      voxel-grid estimate (photonMappingKernel.cu:314-389,
+ *                      :831-870): both maps splatted into 32^3 grids over the
+ *                      room, a wall point gets the photon flux of the 6x6
+ *                      window of its wall's boundary layer around its voxel
+ *                      over the window's area, times albedo / pi (spheres get
+ *                      none), a medium point the flux of the 6x6x6 interior
+ *                      window over its volume, times 1 / (4 pi); 2 = as 1 with
+ *                      the windows of the 4 nearest voxel centres blended
+ *                      bilinearly (:442-583).  Needs pm_emit (or
+ *                      pm_set_records) only; the grids are built by the first
+ *                      pm_render / pm_render_crop after new records, which
+ *                      pm_render_crop_noalloc needs (PM_ERR_STATE before). */
 enum {
   PM_OPT_GATHER_MODE = 1,
   PM_OPT_BLOCKS_PER_CU = 2,
@@ -218,7 +241,8 @@ enum {
   PM_OPT_CACHE_INFER_CHUNK = 4,
   PM_OPT_TILE_REGIONS = 5,
   PM_OPT_SAMPLE_LANES = 6,
-  PM_OPT_RENDER_MEDIA = 7
+  PM_OPT_RENDER_MEDIA = 7,
+  PM_OPT_VOXEL_GRID = 8
 };
 int pm_set_option(pm_context* ctx, int option, int value);
 
@@ -351,6 +375,11 @@ int pm_set_records(pm_context* ctx, int map, const float* pos_host, const float*
                    uint32_t n, uint32_t n_emit);
 int pm_get_tree(pm_context* ctx, int map, uint32_t* perm_host, float* split_host, uint32_t* axis_host,
                 uint32_t n, uint32_t n_internal);
+/* The voxel-grid estimate's summed-volume table of map `map` (PM_OPT_VOXEL_GRID):
+ * n = 3 * 33^3 entries, entry 3 * ((z * 33 + y) * 33 + x) + channel = the
+ * record power (in units of 2^-24) of the voxels [0, x) x [0, y) x [0, z),
+ * modulo 2^64.  PM_ERR_STATE until a voxel-grid render has built it. */
+int pm_get_voxel_table(pm_context* ctx, int map, uint64_t* table_host, size_t n);
 /* kNN over a map's kd-tree for nq device query points (xyz, FP32): idx_dev and
  * d2_dev receive nq*k entries sorted by (d2, photon index), padded with
  * 0xffffffff / +inf; count_dev (may be NULL) receives the number found. */

@@ -58,6 +58,11 @@ _SIGS = {
     "orc_render_const": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P, _P]),
     "orc_camera_ray": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _P, _P, _P]),
     "orc_validate": (ctypes.c_int, [ctypes.POINTER(PmConfig)]),
+    "orc_voxel_tables": (ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, _P]),
+    "orc_voxel_query": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P, ctypes.c_uint32,
+                                       _P]),
+    "orc_render_voxel": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, _P, _P]),
 }
 
 
@@ -271,6 +276,37 @@ class Oracle:
                               _ptr(tout), _ptr(nq))
         assert st == 0, st
         return L, tout, int(nq[0])
+
+    def voxel_tables(self, bf16=False):
+        """(grid_s, grid_v, table_s, table_v): uint64 [32,32,32,3] grids and [33,33,33,3] tables (z, y, x, ch)."""
+        gs = np.empty((32, 32, 32, 3), np.uint64)
+        gv = np.empty_like(gs)
+        ts = np.empty((33, 33, 33, 3), np.uint64)
+        tv = np.empty_like(ts)
+        assert load().orc_voxel_tables(self.h, 1 if bf16 else 0, _ptr(gs), _ptr(gv), _ptr(ts), _ptr(tv)) == 0
+        return gs, gv, ts, tv
+
+    def voxel_query(self, kind, x, n=None, albedo=None, interp=False, bf16=False):
+        """Voxel-grid estimate at points x (kind 0: surface with normals n and albedo, 1: volume), FP64 [n, 3]."""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+        m = x.shape[0]
+        n = np.ascontiguousarray(np.zeros((m, 3)) if n is None else n, np.float32).reshape(-1, 3)
+        albedo = np.ascontiguousarray(np.ones((m, 3)) if albedo is None else albedo, np.float32).reshape(-1, 3)
+        out = np.empty((m, 3), np.float64)
+        assert load().orc_voxel_query(self.h, kind, 1 if interp else 0, 1 if bf16 else 0, _ptr(x), _ptr(n),
+                                      _ptr(albedo), m, _ptr(out)) == 0
+        return out
+
+    def render_voxel(self, x0=0, y0=0, w=0, h=0, interp=False, bf16=False):
+        """FP32 render with the voxel-grid estimate (values as doubles), [h, w, 3]."""
+        if w <= 0:
+            w, h, x0, y0 = self.cfg.width, self.cfg.height, 0, 0
+        out = np.empty((h, w, 3), np.float64)
+        nq = np.zeros(2, np.uint64)
+        assert load().orc_render_voxel(self.h, x0, y0, w, h, 1 if interp else 0, 1 if bf16 else 0, _ptr(out),
+                                       _ptr(nq)) == 0
+        self.last_queries = (int(nq[0]), int(nq[1]))
+        return out
 
     def render_const(self, px, py, s, Sv, Ss):
         """One eye sample with a constant gather: (radiance, (surface, volume) queries)."""

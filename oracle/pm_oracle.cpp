@@ -482,6 +482,154 @@ void render_crop(Ctx& c, int x0, int y0, int w, int h, bool bf16, double* rgb, u
 
 Map& map_of(Ctx& c, int map) { return map == PM_MAP_VOLUME ? c.vol : c.surf; }
 
+// ---------------------------------------------------------------- voxel grids
+// The reference's voxel-grid estimate (photonMappingKernel.cu:260-267 voxel
+// of a point, :1147-1183 photon into its voxel, :314-389 the surface window,
+// :442-583 the bilinear blend, :831-870 the volume window), restated with the
+// library's units: record power in 2^-24 fixed point, 64-bit sums, the room
+// is the unit cube.  Window sums are taken cell by cell from the grid, not
+// from summed-volume tables.
+constexpr int OV = 32;
+constexpr double OFIX = 16777216.0;
+
+int ovox(float p) {
+  const int v = (int)(p * 32.0f);
+  return std::min(std::max(v, 0), OV - 1);
+}
+
+struct VoxGrid {
+  std::vector<uint64_t> g;  // [z][y][x][channel]
+  uint64_t at(int x, int y, int z, int ch) const { return g[(((size_t)z * OV + y) * OV + x) * 3 + ch]; }
+};
+
+VoxGrid voxel_grid(const Map& m, bool bf16) {
+  VoxGrid G;
+  G.g.assign((size_t)OV * OV * OV * 3, 0);
+  auto rd = [bf16](float v) { return bf16 ? bf16_to_f32(f32_to_bf16(v)) : v; };
+  for (size_t i = 0; i < m.size(); ++i) {
+    const size_t cell = (((size_t)ovox(m.z[i]) * OV + ovox(m.y[i])) * OV + ovox(m.x[i])) * 3;
+    const float p[3] = {rd(m.pr[i]), rd(m.pg[i]), rd(m.pb[i])};
+    for (int k = 0; k < 3; ++k) G.g[cell + k] += (uint64_t)((double)p[k] * OFIX + 0.5);
+  }
+  return G;
+}
+
+// Summed-volume table by the 3-D recurrence (inclusion-exclusion of the 7
+// neighbours), modulo 2^64.
+std::vector<uint64_t> voxel_table(const VoxGrid& G) {
+  const int S = OV + 1;
+  std::vector<uint64_t> t((size_t)S * S * S * 3, 0);
+  auto T = [&](int x, int y, int z, int ch) -> uint64_t& { return t[(((size_t)z * S + y) * S + x) * 3 + ch]; };
+  for (int z = 1; z < S; ++z)
+    for (int y = 1; y < S; ++y)
+      for (int x = 1; x < S; ++x)
+        for (int ch = 0; ch < 3; ++ch)
+          T(x, y, z, ch) = G.at(x - 1, y - 1, z - 1, ch) + T(x - 1, y, z, ch) + T(x, y - 1, z, ch) +
+                           T(x, y, z - 1, ch) - T(x - 1, y - 1, z, ch) - T(x - 1, y, z - 1, ch) -
+                           T(x, y - 1, z - 1, ch) + T(x - 1, y - 1, z - 1, ch);
+  return t;
+}
+
+template <class T>
+struct VoxelOracleGather {
+  const VoxGrid* gs;
+  const VoxGrid* gv;
+  double ss[3], sv[3];  // light power / photons emitted (FP32 quotient, as the library forms it)
+  bool interp = false;
+  uint64_t nsurf = 0, nvol = 0;
+
+  // Flux density of the window around in-plane cell (cb, cc) on the wall
+  // layer `layer` across axis a: the reference's [v - 3, v + 3) clamped to
+  // [0, 32] in each in-plane axis.
+  void window(int a, int layer, int cb, int cc, double e[3]) const {
+    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+    const int blo = std::max(cb - 3, 0), bhi = std::min(cb + 3, OV);
+    const int clo = std::max(cc - 3, 0), chi = std::min(cc + 3, OV);
+    uint64_t q[3] = {0, 0, 0};
+    for (int u = blo; u < bhi; ++u)
+      for (int v = clo; v < chi; ++v) {
+        int ix[3];
+        ix[a] = layer; ix[b] = u; ix[c] = v;
+        for (int ch = 0; ch < 3; ++ch) q[ch] += gs->at(ix[0], ix[1], ix[2], ch);
+      }
+    const double area = (double)((bhi - blo) * (chi - clo)) / (double)(OV * OV);
+    for (int ch = 0; ch < 3; ++ch) e[ch] = (double)q[ch] / OFIX * ss[ch] / area;
+  }
+
+  V3<T> surface(V3<T> x, V3<T> n, V3<T> albedo, V3<T> = V3<T>(), V3<T> = V3<T>()) {
+    ++nsurf;
+    // walls only: a normal along one axis
+    const double nn[3] = {std::fabs((double)n.x), std::fabs((double)n.y), std::fabs((double)n.z)};
+    int a = -1;
+    for (int k = 0; k < 3; ++k)
+      if (nn[k] == 1.0 && nn[(k + 1) % 3] == 0.0 && nn[(k + 2) % 3] == 0.0) a = k;
+    if (a < 0) return v3<T>(T(0), T(0), T(0));  // spheres: no estimate
+    const int b = a == 0 ? 1 : 0, c = a == 2 ? 1 : 2;
+    const double xs[3] = {(double)x.x, (double)x.y, (double)x.z};
+    const int layer = xs[a] > 0.5 ? OV - 1 : 0;
+    double e[3];
+    if (!interp) {
+      window(a, layer, ovox((float)xs[b]), ovox((float)xs[c]), e);
+    } else {
+      // blend the windows at the 4 voxel centres (i + 1/2) / 32 around x
+      const double ub = xs[b] * OV - 0.5, uc = xs[c] * OV - 0.5;
+      const double fb0 = ub < 0.0 ? -1.0 : std::trunc(ub), fc0 = uc < 0.0 ? -1.0 : std::trunc(uc);
+      const double fb = std::min(std::max(ub - fb0, 0.0), 1.0), fc = std::min(std::max(uc - fc0, 0.0), 1.0);
+      const int ib = (int)fb0, ic = (int)fc0;
+      auto cl = [](int v) { return std::min(std::max(v, 0), OV - 1); };
+      double w00[3], w10[3], w01[3], w11[3];
+      window(a, layer, cl(ib), cl(ic), w00);
+      window(a, layer, cl(ib + 1), cl(ic), w10);
+      window(a, layer, cl(ib), cl(ic + 1), w01);
+      window(a, layer, cl(ib + 1), cl(ic + 1), w11);
+      for (int ch = 0; ch < 3; ++ch) {
+        const double lo = w00[ch] + fb * (w10[ch] - w00[ch]);
+        const double hi = w01[ch] + fb * (w11[ch] - w01[ch]);
+        e[ch] = lo + fc * (hi - lo);
+      }
+    }
+    const double ip = 0.318309886183790671537767526745;  // 1 / pi
+    return v3<T>(T((double)albedo.x * ip * e[0]), T((double)albedo.y * ip * e[1]), T((double)albedo.z * ip * e[2]));
+  }
+
+  // The reference's interior window [v - 3, v + 3) clamped to [1, 31) per axis.
+  V3<T> volume(V3<T> x, V3<T>, V3<T> = V3<T>()) {
+    ++nvol;
+    int lo[3], hi[3];
+    const float xs[3] = {(float)x.x, (float)x.y, (float)x.z};
+    for (int a = 0; a < 3; ++a) {
+      const int v = ovox(xs[a]);
+      lo[a] = std::min(std::max(v - 3, 1), OV - 1);
+      hi[a] = std::min(std::max(v + 3, 1), OV - 1);
+    }
+    const int cells = (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]);
+    if (cells <= 0) return v3<T>(T(0), T(0), T(0));
+    uint64_t q[3] = {0, 0, 0};
+    for (int zz = lo[2]; zz < hi[2]; ++zz)
+      for (int yy = lo[1]; yy < hi[1]; ++yy)
+        for (int xx = lo[0]; xx < hi[0]; ++xx)
+          for (int ch = 0; ch < 3; ++ch) q[ch] += gv->at(xx, yy, zz, ch);
+    const double vol = (double)cells / (double)(OV * OV * OV);
+    const double i4p = 0.0795774715459476678844418816863;  // 1 / (4 pi)
+    double s[3];
+    for (int ch = 0; ch < 3; ++ch) s[ch] = (double)q[ch] / OFIX * sv[ch] / vol * i4p;
+    return v3<T>(T(s[0]), T(s[1]), T(s[2]));
+  }
+};
+
+template <class T>
+VoxelOracleGather<T> voxel_gather(const Ctx& c, const VoxGrid& gs, const VoxGrid& gv, bool interp) {
+  VoxelOracleGather<T> g;
+  g.gs = &gs;
+  g.gv = &gv;
+  g.interp = interp;
+  for (int k = 0; k < 3; ++k) {
+    g.ss[k] = (double)(c.cfg.light_power[k] / (float)(c.surf.n_emit ? c.surf.n_emit : 1u));
+    g.sv[k] = (double)(c.cfg.light_power[k] / (float)(c.vol.n_emit ? c.vol.n_emit : 1u));
+  }
+  return g;
+}
+
 }  // namespace
 
 extern "C" {
@@ -621,6 +769,83 @@ int orc_render(void* h, int x0, int y0, int w, int hh, int precision, int bf16, 
   if (!c->surf.built || !c->vol.built) return PM_ERR_STATE;
   if (precision == 0) render_crop<float>(*c, x0, y0, w, hh, bf16 != 0, rgb, nq);
   else render_crop<double>(*c, x0, y0, w, hh, bf16 != 0, rgb, nq);
+  return 0;
+}
+
+// The voxel-grid estimate's grids ([32^3][3]) and summed-volume tables
+// ([33^3][3]) of both maps, power through BF16 rounding when bf16 (any output
+// may be NULL).
+int orc_voxel_tables(void* h, int bf16, uint64_t* grid_s, uint64_t* grid_v, uint64_t* sat_s, uint64_t* sat_v) {
+  Ctx* c = (Ctx*)h;
+  const VoxGrid gs = voxel_grid(c->surf, bf16 != 0), gv = voxel_grid(c->vol, bf16 != 0);
+  if (grid_s) memcpy(grid_s, gs.g.data(), gs.g.size() * 8);
+  if (grid_v) memcpy(grid_v, gv.g.data(), gv.g.size() * 8);
+  if (sat_s) {
+    const std::vector<uint64_t> t = voxel_table(gs);
+    memcpy(sat_s, t.data(), t.size() * 8);
+  }
+  if (sat_v) {
+    const std::vector<uint64_t> t = voxel_table(gv);
+    memcpy(sat_v, t.data(), t.size() * 8);
+  }
+  return 0;
+}
+
+// Voxel-grid estimates at n points in FP64: kind 0 = surface (normal n,
+// albedo alb, 3 floats each), 1 = volume.  out: 3 doubles per point.
+int orc_voxel_query(void* h, int kind, int interp, int bf16, const float* x, const float* nrm, const float* alb,
+                    uint32_t n, double* out) {
+  Ctx* c = (Ctx*)h;
+  const VoxGrid gs = voxel_grid(c->surf, bf16 != 0), gv = voxel_grid(c->vol, bf16 != 0);
+  VoxelOracleGather<double> g = voxel_gather<double>(*c, gs, gv, interp != 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    const V3<double> p = v3<double>(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+    V3<double> r;
+    if (kind == 0)
+      r = g.surface(p, v3<double>(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]),
+                    v3<double>(alb[3 * i], alb[3 * i + 1], alb[3 * i + 2]));
+    else
+      r = g.volume(p, v3<double>(0.0, 0.0, 1.0));
+    out[3 * i] = r.x; out[3 * i + 1] = r.y; out[3 * i + 2] = r.z;
+  }
+  return 0;
+}
+
+// The render of the voxel-grid estimate (FP32 frame as the library's), the
+// pixel loop of orc_render.  Needs the records only.
+int orc_render_voxel(void* h, int x0, int y0, int w, int hh, int interp, int bf16, double* rgb, uint64_t* nq) {
+  Ctx* c = (Ctx*)h;
+  const VoxGrid gs = voxel_grid(c->surf, bf16 != 0), gv = voxel_grid(c->vol, bf16 != 0);
+  std::atomic<int> next(0);
+  std::vector<std::thread> th;
+  std::vector<uint64_t> qs((size_t)c->nthreads * 2, 0);
+  for (int t = 0; t < c->nthreads; ++t) {
+    th.emplace_back([&, t]() {
+      VoxelOracleGather<float> g = voxel_gather<float>(*c, gs, gv, interp != 0);
+      int pi;
+      while ((pi = next.fetch_add(1)) < w * hh) {
+        const int row = pi / w, i = pi % w;
+        NSum3<float> acc;
+        nsum3_init(acc);
+        for (int s = 0; s < c->rp.spp; ++s)
+          nsum3_add(acc, render_sample<float>(c->scene, c->rp, (uint32_t)(x0 + i), (uint32_t)(y0 + row),
+                                              (uint32_t)s, g));
+        const V3f v = nsum3_get(acc) * (1.0f / (float)c->rp.spp);
+        double* o = rgb + ((size_t)row * w + i) * 3;
+        o[0] = v.x; o[1] = v.y; o[2] = v.z;
+      }
+      qs[(size_t)t * 2] = g.nsurf;
+      qs[(size_t)t * 2 + 1] = g.nvol;
+    });
+  }
+  for (auto& x : th) x.join();
+  if (nq) {
+    nq[0] = nq[1] = 0;
+    for (int t = 0; t < c->nthreads; ++t) {
+      nq[0] += qs[(size_t)t * 2];
+      nq[1] += qs[(size_t)t * 2 + 1];
+    }
+  }
   return 0;
 }
 
