@@ -759,11 +759,12 @@ static void legacy_configure(float animTime, bool interpolateFlag, bool particip
   c.estimator = interpolateFlag ? PM_EST_CONE : PM_EST_BOX;
   if (g_legacy->voxel_grid) g_legacy->voxel_grid = interpolateFlag ? 2 : 1;
   if (scene_changed) {
+    // the spheres move (or the medium switches) for the eye rays at once; the
+    // photon maps stay those of the last emission until the next emit call,
+    // as the reference's grid does while its emitFlag is off
+    // (callbacksPBO.cpp:60-61, photonMappingKernel.cu:1523-1566)
     int r = setup_scene(g_legacy);
     if (r != PM_OK) legacy_fail("scene setup", r);
-    g_legacy->emitted = false;
-    g_legacy->built = false;
-    g_legacy->vox_ready = false;
   }
 }
 
@@ -782,8 +783,8 @@ void launch_photon_mapping_kernel(pm_uchar4* pos, unsigned int image_width, unsi
   legacy_configure(animTime, interpolateFlag, participatingMediaFlag);
   if (!pos) legacy_fail("launch_photon_mapping_kernel", PM_ERR_INVALID);
   pm_context* c = g_legacy;
-  if (!c->built) {
-    int r = pm_emit(c);
+  if (!c->emitted || !c->built) {  // no emission yet (or its trees are missing): emit now
+    int r = c->emitted ? PM_OK : pm_emit(c);
     if (r == PM_OK) r = pm_build(c);
     if (r != PM_OK) legacy_fail("launch_photon_mapping_kernel (emit)", r);
   }

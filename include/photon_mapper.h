@@ -54,7 +54,11 @@ void launch_emit_photons_kernel(pm_uchar4* pos, unsigned int image_width, unsign
 
 /* Replaces photonMappingKernel.cu:1549-1566 (one thread per pixel into the
  * GL PBO).  Here: renders image_width x image_height into the caller's
- * device buffer `pos` (W*H uchar4), 1 sample per pixel. */
+ * device buffer `pos` (W*H uchar4), 1 sample per pixel, from the photon maps
+ * of the last launch_emit_photons_kernel call (the first render emits when
+ * none was made).  A new animTime or participatingMediaFlag changes the scene
+ * the eye rays see at once; the photon maps change at the next emission, as
+ * the reference's grid does while its emitFlag is off (callbacksPBO.cpp:60-61). */
 void launch_photon_mapping_kernel(pm_uchar4* pos, unsigned int image_width, unsigned int image_height,
                                   float animTime, bool interpolateFlag, bool participatingMediaFlag);
 
