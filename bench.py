@@ -70,11 +70,12 @@ def gather_bytes_per_query(k, bf16=False):
     return k * per + 36
 
 
-def run_gpu(cfg, steps, warmup, gather_mode, tile_regions=0):
+def run_gpu(cfg, steps, warmup, gather_mode, tile_regions=0, top_levels=0):
     lib = pmamd.load()
     g = pmamd.PhotonMapper(cfg)
     g.set_option(pmamd.PM_OPT_GATHER_MODE, gather_mode)
     g.set_option(pmamd.PM_OPT_TILE_REGIONS, tile_regions)
+    g.set_option(pmamd.PM_OPT_TOP_LEVELS, top_levels)
     npx = cfg.width * cfg.height
     rgb = pmamd.DeviceBuffer(npx * 12)
     u8 = pmamd.DeviceBuffer(npx * 4)
@@ -281,7 +282,8 @@ def run_batch(steps, tune_stride, n_scenes=64, seed=1):
     best, tune_ms = b.autotune(tune_stride)
     t_tune = time.perf_counter() - t
     log("batch: %d scenes, setup %.1f s, autotune %.1f s -> leaf %d, block %d, heap slots %d, key bytes %d, "
-        "gather mode %d" % (n_scenes, t_setup, t_tune, best[0], best[1], best[2], best[3], best[4]))
+        "gather mode %d, top levels %d" % (n_scenes, t_setup, t_tune, best[0], best[1], best[2], best[3], best[4],
+                                           best[5]))
     b.submit()
     per = []
     for i in range(steps):
@@ -315,7 +317,8 @@ def run_batch(steps, tune_stride, n_scenes=64, seed=1):
         "volume_records": int(st.volume_records), "radiance_queries": int(q),
         "ms_per_frame": round(ms, 3), "radiance_queries_per_s": round(q / (ms * 1e-3), 1),
         "tuning": {"leaf_size": best[0], "block_threads": best[1], "heap_slots": best[2], "heap_key_bytes": best[3],
-                   "gather_mode": best[4], "autotune_sample_stride": tune_stride, "autotune_s": round(t_tune, 2)},
+                   "gather_mode": best[4], "top_levels": best[5], "autotune_sample_stride": tune_stride,
+                   "autotune_s": round(t_tune, 2)},
         "graph_nodes": int(st.graph_nodes), "work_items": int(st.work_items),
         "scenes_one_at_a_time_ms": round(alone, 3), "speedup_vs_one_at_a_time": round(alone / ms, 3),
     }
@@ -507,6 +510,8 @@ def main():
                     help="stratified frame rows rendered by the FP64 oracle for the error of the BF16 + cache frame")
     ap.add_argument("--batch-steps", type=int, default=3, help="timed batch frames (0: skip the batch leg)")
     ap.add_argument("--batch-tune-stride", type=int, default=4)
+    ap.add_argument("--top-levels", type=int, default=0,
+                    help="kd-tree levels the mode-7 render stages in LDS per wave (PM_OPT_TOP_LEVELS)")
     ap.add_argument("--voxel-steps", type=int, default=3,
                     help="timed frames of the voxel-grid compatibility mode (0: skip the leg)")
     ap.add_argument("--voxel-cpu-seconds", type=float, default=8.0)
@@ -528,7 +533,8 @@ def main():
     if args.frame:
         w, h, spp = (int(v) for v in args.frame.lower().split("x"))
         cfg.width, cfg.height, cfg.spp = w, h, spp
-    sec_per_frame, per, stats, img, g = run_gpu(cfg, args.steps, args.warmup, args.gather_mode, args.tile_regions)
+    sec_per_frame, per, stats, img, g = run_gpu(cfg, args.steps, args.warmup, args.gather_mode, args.tile_regions,
+                                                args.top_levels)
     q_frame = stats.surface_queries + stats.volume_queries
     value = q_frame / sec_per_frame
     kern_ms = float(np.mean([p[3] for p in per]))

@@ -440,6 +440,7 @@ int render_crop_impl(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev
   r.gather_mode = c->gather_mode;
   r.tile_regions = c->tile_regions;
   r.sample_lanes = c->sample_lanes;
+  r.top_levels = c->top_levels;
   if (counters) {
     r.counters = counters;
     HIPCHK(launch_fill_u32(reinterpret_cast<uint32_t*>(counters), 0u, 4, c->stream));
@@ -589,6 +590,10 @@ int pm_set_option(pm_context* c, int option, int value) {
     case PM_OPT_RENDER_MEDIA:
       if (value != 0 && value != 1) return PM_ERR_INVALID;
       c->render_media = value;
+      return PM_OK;
+    case PM_OPT_TOP_LEVELS:
+      if (value < 0 || value > kMaxTopLevels) return PM_ERR_INVALID;
+      c->top_levels = value;
       return PM_OK;
     case PM_OPT_VOXEL_GRID:
       if (value < 0 || value > 2) return PM_ERR_INVALID;

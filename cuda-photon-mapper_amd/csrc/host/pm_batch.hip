@@ -390,7 +390,7 @@ int setup_render(pm_batch* b) {
 
 int set_grid(pm_batch* b) {
   const int occ = render_batch_occupancy(b->fp64, b->cone, b->tune.block_threads, b->tune.heap_slots,
-                                         b->tune.heap_key_bytes, b->tune.gather_mode == 7);
+                                         b->tune.heap_key_bytes, b->tune.gather_mode == 7, b->tune.top_levels);
   if (occ < 1) return PM_ERR_INVALID;
   b->grid = b->cus * occ;
   return PM_OK;
@@ -434,6 +434,7 @@ hipError_t enqueue_frame(pm_batch* b, const uint2* items, uint32_t nitems, hipEv
   R.heap_slots = b->tune.heap_slots;
   R.key_bytes = b->tune.heap_key_bytes;
   R.counting = b->tune.gather_mode == 7;
+  R.top_levels = b->tune.top_levels;
   hipError_t e;
   if ((e = launch_trace_count_batch(b->tsc_d.as<BatchTraceScene>(), b->block_scene_d.as<uint32_t>(), b->tblocks,
                                     b->cnt_s.as<uint32_t>(), b->cnt_v.as<uint32_t>(), st)) != hipSuccess)
@@ -521,6 +522,8 @@ int apply_tuning(pm_batch* b, const pm_batch_tuning& t, bool capture_graph) {
   if (t.heap_key_bytes != 8 && t.heap_key_bytes != 4 && t.heap_key_bytes != 2) return PM_ERR_INVALID;
   if (t.gather_mode != 0 && t.gather_mode != 7) return PM_ERR_INVALID;
   if (t.gather_mode == 7 && (t.heap_key_bytes != 2 || t.heap_slots != 64)) return PM_ERR_INVALID;
+  if (t.top_levels < 0 || t.top_levels > kMaxTopLevels || (t.top_levels > 0 && t.gather_mode != 7))
+    return PM_ERR_INVALID;
   int r;
   // the grid is checked first, then the tuning is committed; a later failure
   // puts the previous tuning (and its grid) back
@@ -637,6 +640,7 @@ int pm_batch_create(const pm_config* cfgs, int n, pm_batch** out) {
   t.heap_slots = b->kmax <= 32 ? 32 : 64;
   t.heap_key_bytes = 2;
   t.gather_mode = 0;
+  t.top_levels = 0;
   int r = setup_scenes(b);
   if (r == PM_OK) r = size_records(b);
   if (r == PM_OK) r = setup_render(b);
@@ -714,6 +718,7 @@ int pm_batch_autotune(pm_batch* b, int sample_stride, pm_batch_tuning* best_out,
     return PM_OK;
   };
   const int keyb[3] = {8, 4, 2}, slots[2] = {32, 64}, leaves[4] = {4, 8, 16, 32}, blocks[3] = {64, 128, 256};
+  const int tops[4] = {0, 3, 5, 7};
   const pm_batch_tuning start = b->tune;
   for (int kb : keyb)
     for (int sl : slots) {
@@ -721,6 +726,7 @@ int pm_batch_autotune(pm_batch* b, int sample_stride, pm_batch_tuning* best_out,
       t.heap_key_bytes = kb;
       t.heap_slots = sl;
       t.gather_mode = 0;
+      t.top_levels = 0;
       int r = consider(t);
       if (r != PM_OK) return r;
     }
@@ -729,6 +735,7 @@ int pm_batch_autotune(pm_batch* b, int sample_stride, pm_batch_tuning* best_out,
     t.heap_key_bytes = 2;
     t.heap_slots = 64;
     t.gather_mode = 7;
+    t.top_levels = 0;
     int r = consider(t);
     if (r != PM_OK) return r;
   }
@@ -747,6 +754,16 @@ int pm_batch_autotune(pm_batch* b, int sample_stride, pm_batch_tuning* best_out,
     t.block_threads = bt;
     int r = consider(t);
     if (r != PM_OK) return r;
+  }
+  if (best.gather_mode == 7) {  // LDS tile: top-of-tree levels staged per wave (mode 7 reads them)
+    const pm_batch_tuning s3 = best;
+    for (int lv : tops) {
+      if (lv == s3.top_levels) continue;
+      pm_batch_tuning t = s3;
+      t.top_levels = lv;
+      int r = consider(t);
+      if (r != PM_OK) return r;
+    }
   }
   int r = apply_tuning(b, best, true);
   if (r != PM_OK) return r;

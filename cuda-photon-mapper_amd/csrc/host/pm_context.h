@@ -113,6 +113,7 @@ struct pm_context {
   int tile_regions = 0;  // per-XCD tile regions of the lane render (0: raster order)
   int sample_lanes = 1;  // mode 7: a pixel's samples in adjacent lanes
   int render_media = 1;  // 0: the plain render gathers at surface hits only
+  int top_levels = 0;    // mode 7: kd-tree levels staged in LDS per wave
   int voxel_grid = 0;    // 1: the plain render uses the voxel-grid estimate, 2: with the bilinear blend
   bool vox_ready = false;  // voxel grids and tables hold the current records
   pm::DevBuf vox_grid_s, vox_grid_v, vox_sat_s, vox_sat_v;

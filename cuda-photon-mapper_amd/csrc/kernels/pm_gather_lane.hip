@@ -329,17 +329,35 @@ hipError_t launch_render_lane6(const RenderLaunch& r, hipStream_t st) {
   return hipGetLastError();
 }
 
+// LDS of a mode-7 wave: the 8 KB of 2-byte heap keys / bucket counters, then
+// with TOP the top-of-tree tiles of the surface and volume trees.
+constexpr size_t kLane7HeapBytes = 64 * 64 * sizeof(uint16_t);
+
+template <bool TOP>
+__device__ __forceinline__ void stage_lane7_tiles(GatherMap& gs, GatherMap& gv, uint16_t* heap, int levels,
+                                                  int lane) {
+  if constexpr (TOP) {
+    unsigned char* t = (unsigned char*)heap + kLane7HeapBytes;
+    stage_top_tile(gs, t, levels, lane);
+    stage_top_tile(gv, t + top_tile_bytes(levels), levels, lane);
+  }
+}
+
 // Gather mode 7: mode 6 with the march's first passes done by counting in a
-// bucket window (pm_lane_knn5.h), 5 waves per SIMD.
-template <bool FP64T, bool CONE>
+// bucket window (pm_lane_knn5.h), 5 waves per SIMD.  TOP: the traversals read
+// the top r.top_levels levels of both trees from LDS.
+template <bool FP64T, bool CONE, bool TOP>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_render_lane7(RenderLaunch r) {
-  extern __shared__ uint16_t heap16_lds[];  // [64 slots][64 lanes]
+  extern __shared__ uint16_t heap16_lds[];  // [64 slots][64 lanes], then the top tiles
   const int lane = lane_id();
+  GatherMap gs = r.gs, gv = r.gv;
+  stage_lane7_tiles<TOP>(gs, gv, heap16_lds, r.top_levels, lane);  // by every lane, before any leaves
   int lx, ly;
   if (!lane_tile(r, lane, lx, ly)) return;
   const uint32_t px = (uint32_t)(r.x0 + lx), py = (uint32_t)(r.y0 + ly);
   uint32_t ns, nv;
-  V3f v = lane_pixel4<FP64T, CONE, LaneGather5<CONE>>(r.sc, r.rp, r.gs, r.gv, heap16_lds, lane, px, py, ns, nv);
+  V3f v = lane_pixel4<FP64T, CONE, LaneGather5<CONE, false, TOP>>(r.sc, r.rp, gs, gv, heap16_lds, lane, px, py, ns,
+                                                                   nv);
   store_pixel(r.rgb, r.u8, (size_t)ly * (size_t)r.w + (size_t)lx, v, r.exposure);
   if (r.counters) {
     atomicAdd(&r.counters[0], (unsigned long long)ns);
@@ -358,10 +376,14 @@ __device__ __forceinline__ int sample_tile_w(int spp) {
   return P == 32 ? 8 : (P == 16 ? 4 : (P == 8 ? 4 : 2));
 }
 
-template <bool FP64T, bool CONE>
+template <bool FP64T, bool CONE, bool TOP>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_render_lane7s(RenderLaunch r) {
-  extern __shared__ uint16_t heap16_lds[];  // [64 slots][64 lanes]
+  extern __shared__ uint16_t heap16_lds[];  // [64 slots][64 lanes], then the top tiles
   const int lane = lane_id();
+  LaneGather5<CONE, false, TOP> g;
+  g.gs = r.gs;
+  g.gv = r.gv;
+  stage_lane7_tiles<TOP>(g.gs, g.gv, heap16_lds, r.top_levels, lane);  // by every lane, before any leaves
   const int spp = r.rp.spp, lg = __builtin_ctz((unsigned)spp);
   const int pw = sample_tile_w(spp), ph = (64 / spp) / pw;
   const int p = lane >> lg, s = lane & (spp - 1);
@@ -370,9 +392,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
   const int lx = tx * pw + p % pw, ly = ty * ph + p / pw;
   if (lx >= r.w || ly >= r.h) return;  // all lanes of a pixel leave together
   const uint32_t px = (uint32_t)(r.x0 + lx), py = (uint32_t)(r.y0 + ly);
-  LaneGather5<CONE> g;
-  g.gs = r.gs;
-  g.gv = r.gv;
   g.hp.H = heap16_lds;
   g.hp.lane = lane;
   g.hp.k = 0;
@@ -398,28 +417,32 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
   }
 }
 
-hipError_t launch_render_lane7(const RenderLaunch& r, hipStream_t st) {
-  if (r.w <= 0 || r.h <= 0) return hipSuccess;
+template <bool TOP>
+static hipError_t launch_render_lane7_t(const RenderLaunch& r, hipStream_t st) {
+  const size_t lds = kLane7HeapBytes + (TOP ? 2 * top_tile_bytes(r.top_levels) : 0);
+  const bool fp64 = r.rp.fp64_transmittance != 0, cone = r.estimator == 1;
   const int spp = r.rp.spp;
   if (r.sample_lanes && (spp == 2 || spp == 4 || spp == 8 || spp == 16)) {
     const int pw = spp == 4 || spp == 8 ? 4 : (spp == 2 ? 8 : 2), ph = (64 / spp) / pw;
     const unsigned tiles = (unsigned)(((r.w + pw - 1) / pw) * ((r.h + ph - 1) / ph));
-    size_t lds = 64 * 64 * sizeof(uint16_t);
-    const bool fp64 = r.rp.fp64_transmittance != 0, cone = r.estimator == 1;
-    if (fp64 && cone) hipLaunchKernelGGL((k_render_lane7s<true, true>), dim3(tiles), dim3(64), lds, st, r);
-    else if (fp64) hipLaunchKernelGGL((k_render_lane7s<true, false>), dim3(tiles), dim3(64), lds, st, r);
-    else if (cone) hipLaunchKernelGGL((k_render_lane7s<false, true>), dim3(tiles), dim3(64), lds, st, r);
-    else hipLaunchKernelGGL((k_render_lane7s<false, false>), dim3(tiles), dim3(64), lds, st, r);
+    if (fp64 && cone) hipLaunchKernelGGL((k_render_lane7s<true, true, TOP>), dim3(tiles), dim3(64), lds, st, r);
+    else if (fp64) hipLaunchKernelGGL((k_render_lane7s<true, false, TOP>), dim3(tiles), dim3(64), lds, st, r);
+    else if (cone) hipLaunchKernelGGL((k_render_lane7s<false, true, TOP>), dim3(tiles), dim3(64), lds, st, r);
+    else hipLaunchKernelGGL((k_render_lane7s<false, false, TOP>), dim3(tiles), dim3(64), lds, st, r);
     return hipGetLastError();
   }
   const unsigned tiles = lane_grid(r);
-  size_t lds = 64 * 64 * sizeof(uint16_t);
-  const bool fp64 = r.rp.fp64_transmittance != 0, cone = r.estimator == 1;
-  if (fp64 && cone) hipLaunchKernelGGL((k_render_lane7<true, true>), dim3(tiles), dim3(64), lds, st, r);
-  else if (fp64) hipLaunchKernelGGL((k_render_lane7<true, false>), dim3(tiles), dim3(64), lds, st, r);
-  else if (cone) hipLaunchKernelGGL((k_render_lane7<false, true>), dim3(tiles), dim3(64), lds, st, r);
-  else hipLaunchKernelGGL((k_render_lane7<false, false>), dim3(tiles), dim3(64), lds, st, r);
+  if (fp64 && cone) hipLaunchKernelGGL((k_render_lane7<true, true, TOP>), dim3(tiles), dim3(64), lds, st, r);
+  else if (fp64) hipLaunchKernelGGL((k_render_lane7<true, false, TOP>), dim3(tiles), dim3(64), lds, st, r);
+  else if (cone) hipLaunchKernelGGL((k_render_lane7<false, true, TOP>), dim3(tiles), dim3(64), lds, st, r);
+  else hipLaunchKernelGGL((k_render_lane7<false, false, TOP>), dim3(tiles), dim3(64), lds, st, r);
   return hipGetLastError();
+}
+
+hipError_t launch_render_lane7(const RenderLaunch& r, hipStream_t st) {
+  if (r.w <= 0 || r.h <= 0) return hipSuccess;
+  if (r.top_levels < 0 || r.top_levels > kMaxTopLevels) return hipErrorInvalidValue;
+  return r.top_levels > 0 ? launch_render_lane7_t<true>(r, st) : launch_render_lane7_t<false>(r, st);
 }
 
 hipError_t launch_render_lane2(const RenderLaunch& r, hipStream_t st) {
@@ -455,14 +478,24 @@ hipError_t launch_render_lane(const RenderLaunch& r, hipStream_t st) {
 // with 2-byte keys), so every scene's image is bit-identical to its own render
 // in that mode.  heap_slots (32 or 64 keys per lane) and the key width size
 // each wave's LDS heap; mode 7 counts in the same 8 KB (64 slots).
-template <bool FP64T, bool CONE, int G>
+// TOP (gather mode 7 only): each wave stages the top r.top_levels levels of
+// its current scene's two trees in its own LDS region after the heaps, and
+// stages them again when its next work item belongs to another scene.
+template <bool FP64T, bool CONE, int G, bool TOP>
 __global__ void __launch_bounds__(256) k_render_batch(BatchRenderLaunch r) {
   constexpr int KB = G == 7 ? 2 : G;
+  static_assert(!TOP || G == 7, "top-of-tree tiles are read by the mode-7 gather");
   extern __shared__ uint32_t lds[];
   const int lane = lane_id();
-  const int wave = (int)(threadIdx.x >> 6);
+  const int wave = (int)uniform(threadIdx.x >> 6);  // wave-uniform: the LDS regions' addresses stay scalar
   const size_t heap_words = (size_t)r.heap_slots * 64u * KB / 4u;
   uint32_t* heap = lds + (size_t)wave * heap_words;
+  unsigned char* tiles = nullptr;
+  GatherMap tgs, tgv;
+  uint32_t staged = 0xffffffffu;
+  if constexpr (TOP)
+    tiles = (unsigned char*)(lds + (size_t)(blockDim.x >> 6) * heap_words) +
+            (size_t)wave * 2u * top_tile_bytes(r.top_levels);
   unsigned long long ns_tot = 0, nv_tot = 0;
   for (;;) {
     uint32_t it = 0;
@@ -471,7 +504,17 @@ __global__ void __launch_bounds__(256) k_render_batch(BatchRenderLaunch r) {
     if (it >= r.nitems) break;
     const uint2 item = r.items[it];
     // wave-uniform scene and tile: scalar loads of the scene descriptor
-    const BatchRenderScene& S = r.scenes[uniform(item.x)];
+    const uint32_t scene = uniform(item.x);
+    const BatchRenderScene& S = r.scenes[scene];
+    if constexpr (TOP) {
+      if (scene != staged) {  // every lane of the wave is here
+        tgs = S.gs;
+        tgv = S.gv;
+        stage_top_tile(tgs, tiles, r.top_levels, lane);
+        stage_top_tile(tgv, tiles + top_tile_bytes(r.top_levels), r.top_levels, lane);
+        staged = scene;
+      }
+    }
     const uint32_t tile = uniform(item.y);
     const int tiles_x = (S.w + 7) / 8;
     const int tx = (int)tile % tiles_x, ty = (int)tile / tiles_x;
@@ -484,6 +527,9 @@ __global__ void __launch_bounds__(256) k_render_batch(BatchRenderLaunch r) {
                                     nv);
       else if (KB == 4)
         v = lane_pixel2<FP64T, CONE>(S.sc, S.rp, S.gs, S.gv, heap, lane, (uint32_t)lx, (uint32_t)ly, ns, nv);
+      else if (G == 7 && TOP)
+        v = lane_pixel4<FP64T, CONE, LaneGather5<CONE, false, true>>(S.sc, S.rp, tgs, tgv, (uint16_t*)heap, lane,
+                                                                     (uint32_t)lx, (uint32_t)ly, ns, nv);
       else if (G == 7)
         v = lane_pixel4<FP64T, CONE, LaneGather5<CONE>>(S.sc, S.rp, S.gs, S.gv, (uint16_t*)heap, lane, (uint32_t)lx,
                                                         (uint32_t)ly, ns, nv);
@@ -501,21 +547,25 @@ __global__ void __launch_bounds__(256) k_render_batch(BatchRenderLaunch r) {
   }
 }
 
-size_t render_batch_lds_bytes(int block_threads, int heap_slots, int key_bytes) {
-  return (size_t)(block_threads / 64) * (size_t)heap_slots * 64u * (size_t)key_bytes;
+size_t render_batch_lds_bytes(int block_threads, int heap_slots, int key_bytes, int top_levels) {
+  return (size_t)(block_threads / 64) * ((size_t)heap_slots * 64u * (size_t)key_bytes + 2u * top_tile_bytes(top_levels));
 }
 
 // Gather kind of a launch: the heap key width, or 7 for mode 7.
 static int batch_gather(const BatchRenderLaunch& r) { return r.counting ? 7 : r.key_bytes; }
 
-template <int KB>
+template <int KB, bool TOP>
 static hipError_t launch_render_batch_kb(const BatchRenderLaunch& r, int fp64, int cone, int block_threads, int grid,
                                          hipStream_t st) {
-  const size_t lds = render_batch_lds_bytes(block_threads, r.heap_slots, KB == 7 ? 2 : KB);
-  if (fp64 && cone) hipLaunchKernelGGL((k_render_batch<true, true, KB>), dim3(grid), dim3(block_threads), lds, st, r);
-  else if (fp64) hipLaunchKernelGGL((k_render_batch<true, false, KB>), dim3(grid), dim3(block_threads), lds, st, r);
-  else if (cone) hipLaunchKernelGGL((k_render_batch<false, true, KB>), dim3(grid), dim3(block_threads), lds, st, r);
-  else hipLaunchKernelGGL((k_render_batch<false, false, KB>), dim3(grid), dim3(block_threads), lds, st, r);
+  const size_t lds = render_batch_lds_bytes(block_threads, r.heap_slots, KB == 7 ? 2 : KB, TOP ? r.top_levels : 0);
+  if (fp64 && cone)
+    hipLaunchKernelGGL((k_render_batch<true, true, KB, TOP>), dim3(grid), dim3(block_threads), lds, st, r);
+  else if (fp64)
+    hipLaunchKernelGGL((k_render_batch<true, false, KB, TOP>), dim3(grid), dim3(block_threads), lds, st, r);
+  else if (cone)
+    hipLaunchKernelGGL((k_render_batch<false, true, KB, TOP>), dim3(grid), dim3(block_threads), lds, st, r);
+  else
+    hipLaunchKernelGGL((k_render_batch<false, false, KB, TOP>), dim3(grid), dim3(block_threads), lds, st, r);
   return hipGetLastError();
 }
 
@@ -524,33 +574,37 @@ hipError_t launch_render_batch(const BatchRenderLaunch& r, int fp64, int cone, i
   if (r.nitems == 0) return hipSuccess;
   const int G = batch_gather(r);
   if (G == 7 && (r.key_bytes != 2 || r.heap_slots != 64)) return hipErrorInvalidValue;
-  if (G == 7) return launch_render_batch_kb<7>(r, fp64, cone, block_threads, grid, st);
-  if (G == 8) return launch_render_batch_kb<8>(r, fp64, cone, block_threads, grid, st);
-  if (G == 4) return launch_render_batch_kb<4>(r, fp64, cone, block_threads, grid, st);
-  return launch_render_batch_kb<2>(r, fp64, cone, block_threads, grid, st);
+  if (r.top_levels < 0 || r.top_levels > kMaxTopLevels || (r.top_levels > 0 && G != 7)) return hipErrorInvalidValue;
+  if (G == 7 && r.top_levels > 0) return launch_render_batch_kb<7, true>(r, fp64, cone, block_threads, grid, st);
+  if (G == 7) return launch_render_batch_kb<7, false>(r, fp64, cone, block_threads, grid, st);
+  if (G == 8) return launch_render_batch_kb<8, false>(r, fp64, cone, block_threads, grid, st);
+  if (G == 4) return launch_render_batch_kb<4, false>(r, fp64, cone, block_threads, grid, st);
+  return launch_render_batch_kb<2, false>(r, fp64, cone, block_threads, grid, st);
 }
 
-template <int KB>
+template <int KB, bool TOP>
 static int occupancy_kb(int fp64, int cone, int block_threads, size_t lds) {
   int nb = 0;
   hipError_t e;
   if (fp64 && cone)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<true, true, KB>, block_threads, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<true, true, KB, TOP>, block_threads, lds);
   else if (fp64)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<true, false, KB>, block_threads, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<true, false, KB, TOP>, block_threads, lds);
   else if (cone)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<false, true, KB>, block_threads, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<false, true, KB, TOP>, block_threads, lds);
   else
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<false, false, KB>, block_threads, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_batch<false, false, KB, TOP>, block_threads, lds);
   return e == hipSuccess ? nb : 0;
 }
 
-int render_batch_occupancy(int fp64, int cone, int block_threads, int heap_slots, int key_bytes, int counting) {
-  const size_t lds = render_batch_lds_bytes(block_threads, heap_slots, key_bytes);
-  if (counting) return occupancy_kb<7>(fp64, cone, block_threads, lds);
-  if (key_bytes == 8) return occupancy_kb<8>(fp64, cone, block_threads, lds);
-  if (key_bytes == 4) return occupancy_kb<4>(fp64, cone, block_threads, lds);
-  return occupancy_kb<2>(fp64, cone, block_threads, lds);
+int render_batch_occupancy(int fp64, int cone, int block_threads, int heap_slots, int key_bytes, int counting,
+                           int top_levels) {
+  const size_t lds = render_batch_lds_bytes(block_threads, heap_slots, key_bytes, counting ? top_levels : 0);
+  if (counting && top_levels > 0) return occupancy_kb<7, true>(fp64, cone, block_threads, lds);
+  if (counting) return occupancy_kb<7, false>(fp64, cone, block_threads, lds);
+  if (key_bytes == 8) return occupancy_kb<8, false>(fp64, cone, block_threads, lds);
+  if (key_bytes == 4) return occupancy_kb<4, false>(fp64, cone, block_threads, lds);
+  return occupancy_kb<2, false>(fp64, cone, block_threads, lds);
 }
 
 __global__ void k_tonemap(const float* __restrict__ rgb, size_t npix, float exposure, uchar4* __restrict__ out) {

@@ -57,7 +57,18 @@ struct GatherMap {
   int k;
   float rcap2;
   float scale[3];  // light power / n_emit
+  // top-of-tree tile: the first top_count internal nodes' records, staged in
+  // LDS by the gather kernels that take one (0: none, every read from HBM)
+  const KdNode* top_nodes = nullptr;
+  const uint2* top_nrange = nullptr;
+  const float4* top_bmin = nullptr;
+  const float4* top_bmax = nullptr;
+  uint32_t top_count = 0;
 };
+// LDS bytes of a top-of-tree tile of `levels` levels (2^levels - 1 nodes):
+// boxes (2 x 16 B), node record (8 B) and point range (8 B) per node.
+PM_HD size_t top_tile_bytes(int levels) { return levels > 0 ? 48u * (((size_t)1 << levels) - 1u) : 0u; }
+constexpr int kMaxTopLevels = 7;
 
 // scan / sort (pm_scan_sort.hip)
 size_t scan_temp_words(size_t n);
@@ -164,6 +175,7 @@ struct RenderLaunch {
   int gather_mode;               // 0: wave per query; lane per query: 1 8-byte, 2 4-byte, 3 2-byte heap
   int tile_regions;              // lane modes: 0 raster tile order; rx = 1, 2, 4, 8: rx x (8 / rx) per-XCD regions
   int sample_lanes;              // mode 7: a pixel's spp samples in adjacent lanes (spp 2, 4, 8 or 16)
+  int top_levels;                // mode 7: levels of each tree staged in LDS per wave (0: none)
 };
 hipError_t launch_render(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane(const RenderLaunch& r, hipStream_t st);
@@ -218,10 +230,11 @@ struct BatchRenderLaunch {
   int heap_slots;                  // LDS heap keys per lane (>= every scene's k)
   int key_bytes;                   // heap key width: 8, 4 or 2 (gather modes 1, 2, 4)
   int counting;                    // 1: gather mode 7 (key_bytes 2, heap_slots 64; maps with boxes and tpayload)
+  int top_levels;                  // gather mode 7: levels of each scene's trees staged in LDS per wave
 };
-size_t render_batch_lds_bytes(int block_threads, int heap_slots, int key_bytes);
-int render_batch_occupancy(int fp64, int cone, int block_threads, int heap_slots, int key_bytes,
-                           int counting);  // per CU
+size_t render_batch_lds_bytes(int block_threads, int heap_slots, int key_bytes, int top_levels);
+int render_batch_occupancy(int fp64, int cone, int block_threads, int heap_slots, int key_bytes, int counting,
+                           int top_levels);  // per CU
 hipError_t launch_tonemap(const float* rgb, size_t npix, float exposure, uchar4* out, hipStream_t st);
 hipError_t launch_gather_u32(const uint32_t* src, const uint32_t* idx, uint32_t n, uint32_t* dst, hipStream_t st);
 hipError_t launch_render_batch(const BatchRenderLaunch& r, int fp64, int cone, int block_threads, int grid,

@@ -198,7 +198,66 @@ struct Sum16CVisitor : Sum16Visitor<Add> {
 // held.  When the map has node boxes, a far child and a leaf are entered only
 // if the visitor's box test (V::near_box) passes: the points skipped are all
 // outside every search bound.
-template <int NB, class V>
+// Node reads of the traversal: from the map's top-of-tree tile in LDS when
+// TOP and the node is in it, else from HBM.  The values are the same.
+template <bool TOP>
+__device__ __forceinline__ KdNode tree_node(const GatherMap& g, uint32_t h) {
+  if constexpr (TOP) {
+    if (h < g.top_count) return g.top_nodes[h];
+  }
+  return g.nodes[h];
+}
+template <bool TOP>
+__device__ __forceinline__ uint2 tree_range(const GatherMap& g, uint32_t h) {
+  if constexpr (TOP) {
+    if (h < g.top_count) return g.top_nrange[h];
+  }
+  return g.nrange[h];
+}
+// The visitor's box test on node h (true when the map has no boxes).
+template <bool TOP, class V>
+__device__ __forceinline__ bool tree_box_near(const GatherMap& g, uint32_t h, const V& v) {
+  if (!g.bmin) return true;
+  if constexpr (TOP) {
+    if (h < g.top_count) return v.near_box(g.top_bmin[h], g.top_bmax[h]);
+  }
+  return v.near_box(g.bmin[h], g.bmax[h]);
+}
+
+// Stages the first `levels` levels of g's internal nodes (at most nl - 1
+// nodes) into `lds` (top_tile_bytes(levels) bytes, 16-byte aligned) with the
+// 64 lanes of the wave, and points g's top tile at them.  Every lane of the
+// wave must call it; the copies are visible to the wave when it returns.
+__device__ __forceinline__ void stage_top_tile(GatherMap& g, unsigned char* lds, int levels, int lane) {
+  const uint32_t cap = ((uint32_t)1 << levels) - 1u;
+  const uint32_t ni = g.nl > 0u ? g.nl - 1u : 0u;
+  const uint32_t c = ni < cap ? ni : cap;
+  float4* bmn = (float4*)lds;
+  float4* bmx = bmn + cap;
+  KdNode* nd = (KdNode*)(bmx + cap);
+  uint2* nr = (uint2*)(nd + cap);
+  // the wave's earlier reads of this region (a previous tile) come first
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  for (uint32_t i = (uint32_t)lane; i < c; i += 64u) {
+    nd[i] = g.nodes[i];
+    nr[i] = g.nrange[i];
+    if (g.bmin) {
+      bmn[i] = g.bmin[i];
+      bmx[i] = g.bmax[i];
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  g.top_nodes = nd;
+  g.top_nrange = nr;
+  g.top_bmin = bmn;
+  g.top_bmax = bmx;
+  g.top_count = c;
+}
+
+template <int NB, bool TOP = false, class V>
 __device__ __forceinline__ void lane_visit_dual_spec(const GatherMap& g, float qx, float qy, float qz, V& v) {
   const uint32_t n = g.n, nl = g.nl;
   if (n == 0) return;
@@ -209,7 +268,7 @@ __device__ __forceinline__ void lane_visit_dual_spec(const GatherMap& g, float q
     for (;;) {
       if (!done && !kept && down && h + 1u >= nl) {
         // keep this leaf (unless its box is out of reach) and walk on
-        if (!g.bmin || v.near_box(g.bmin[h], g.bmax[h])) {
+        if (tree_box_near<TOP>(g, h, v)) {
           plo = lo;
           phi = hi;
           kept = true;
@@ -221,7 +280,7 @@ __device__ __forceinline__ void lane_visit_dual_spec(const GatherMap& g, float q
       const bool step = !done && !(down && h + 1u >= nl);
       if (!step) continue;
       if (down) {
-        KdNode nd = g.nodes[h];
+        KdNode nd = tree_node<TOP>(g, h);
         int a = kd_axis(nd.meta);
         float qa = a == 0 ? qx : (a == 1 ? qy : qz);
         uint32_t mid = lo + kd_left_count(nd.meta);
@@ -239,13 +298,13 @@ __device__ __forceinline__ void lane_visit_dual_spec(const GatherMap& g, float q
         continue;
       }
       uint32_t p = (from - 1u) >> 1;
-      KdNode nd = g.nodes[p];
+      KdNode nd = tree_node<TOP>(g, p);
       int a = kd_axis(nd.meta);
       float qa = a == 0 ? qx : (a == 1 ? qy : qz);
       uint32_t nearc = qa - nd.split < 0.0f ? 2u * p + 1u : 2u * p + 2u;
       const uint32_t farc = nearc == 2u * p + 1u ? 2u * p + 2u : 2u * p + 1u;
-      if (from == nearc && v.far(a, nd.split) && (!g.bmin || v.near_box(g.bmin[farc], g.bmax[farc]))) {
-        uint2 r = g.nrange[p];
+      if (from == nearc && v.far(a, nd.split) && tree_box_near<TOP>(g, farc, v)) {
+        uint2 r = tree_range<TOP>(g, p);
         uint32_t mid = r.x + kd_left_count(nd.meta);
         if (nearc == 2u * p + 1u) {
           h = 2u * p + 2u;
@@ -351,7 +410,8 @@ struct HeapSpecVisitor {
 // SPEC_HEAP: the heap pass on the speculative traversal (HeapSpecVisitor), for
 // points that are each gathered on their own; the march's heap passes are rare
 // and scattered, and run faster on the plain traversal.
-template <bool CONE, bool SPEC_HEAP = false>
+// TOP: the traversals read the top-of-tree tiles the kernel staged in LDS.
+template <bool CONE, bool SPEC_HEAP = false, bool TOP = false>
 struct LaneGather5 : LaneGather4<CONE, 4> {
   using Base = LaneGather4<CONE, 4>;
   using VolAdd = VolAddT;
@@ -407,7 +467,7 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
     nsum3_init(add.s);
     Sum16CVisitor<VolAdd> sv = pending_sums(add);
     SumOnlyVisitor<VolAdd> w{this->pend_x, &sv};
-    lane_visit_dual_spec<4>(this->gv, this->pend_x.x, this->pend_x.y, this->pend_x.z, w);
+    lane_visit_dual_spec<4, TOP>(this->gv, this->pend_x.x, this->pend_x.y, this->pend_x.z, w);
     c = this->finish_pending(sv, add);
     return true;
   }
@@ -421,7 +481,7 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
       kv.bound_h = t == 0 ? bh : caph;
       if constexpr (SPEC_HEAP) {
         HeapSpecVisitor hs{x, &kv};
-        lane_visit_dual_spec<4>(this->gv, x.x, x.y, x.z, hs);
+        lane_visit_dual_spec<4, TOP>(this->gv, x.x, x.y, x.z, hs);
       } else {
         lane_visit<4>(this->gv, x.x, x.y, x.z, kv);
       }
@@ -452,7 +512,7 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
     this->hp.size = 0;
     Knn16Visitor kv{&this->hp, caph, 0u, false};
     HeapSpecVisitor hs{x, &kv};
-    lane_visit_dual_spec<4>(gs, x.x, x.y, x.z, hs);
+    lane_visit_dual_spec<4, TOP>(gs, x.x, x.y, x.z, hs);
     if (this->hp.size == 0) return v3(0.0f, 0.0f, 0.0f);
     SurfAddT add{&gs, n, {}};
     nsum3_init(add.s);
@@ -464,7 +524,7 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
     sv.d0 = sv.i0 = sv.d1 = sv.i1 = LKEY_INF;
     sv.add = &add;
     SumOnlyVisitor<SurfAddT> w{x, &sv};
-    lane_visit_dual_spec<4>(gs, x.x, x.y, x.z, w);
+    lane_visit_dual_spec<4, TOP>(gs, x.x, x.y, x.z, w);
     float r2 = u2f(caph);
     if (kv.full) {
       const uint32_t need = (uint32_t)gs.k - sv.n_less;
@@ -522,11 +582,11 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
           HistVisitor hv = window(this->pend_Q, caph);
           hv.clear();
           DualHistVisitor<VolAdd> dv{x, this->pend_x, &hv, &sv};
-          lane_visit_dual_spec<4>(gv, x.x, x.y, x.z, dv);
+          lane_visit_dual_spec<4, TOP>(gv, x.x, x.y, x.z, dv);
           heap = !resolve(hv, caph, kv);
         } else {
           SumOnlyVisitor<VolAdd> w{this->pend_x, &sv};  // fewer than k within the cap
-          lane_visit_dual_spec<4>(gv, this->pend_x.x, this->pend_x.y, this->pend_x.z, w);
+          lane_visit_dual_spec<4, TOP>(gv, this->pend_x.x, this->pend_x.y, this->pend_x.z, w);
         }
         c = this->finish_pending(sv, add);
       }

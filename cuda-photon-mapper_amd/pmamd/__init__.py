@@ -37,6 +37,7 @@ PM_OPT_TILE_REGIONS = 5
 PM_OPT_SAMPLE_LANES = 6
 PM_OPT_RENDER_MEDIA = 7
 PM_OPT_VOXEL_GRID = 8
+PM_OPT_TOP_LEVELS = 9
 VOXEL_TABLE_ENTRIES = 3 * 33 ** 3  # summed-volume table of one map's 32^3 voxel grid
 
 
@@ -134,6 +135,7 @@ class PmBatchTuning(ctypes.Structure):
         ("heap_slots", ctypes.c_int),
         ("heap_key_bytes", ctypes.c_int),
         ("gather_mode", ctypes.c_int),
+        ("top_levels", ctypes.c_int),
     ]
 
 
@@ -497,15 +499,15 @@ class Batch:
         except Exception:
             pass
 
-    def set_tuning(self, leaf_size, block_threads, heap_slots, heap_key_bytes=2, gather_mode=0):
-        t = PmBatchTuning(leaf_size, block_threads, heap_slots, heap_key_bytes, gather_mode)
+    def set_tuning(self, leaf_size, block_threads, heap_slots, heap_key_bytes=2, gather_mode=0, top_levels=0):
+        t = PmBatchTuning(leaf_size, block_threads, heap_slots, heap_key_bytes, gather_mode, top_levels)
         check(load().pm_batch_set_tuning(self.h, ctypes.byref(t)), "pm_batch_set_tuning")
 
     def autotune(self, sample_stride=16):
         t = PmBatchTuning()
         ms = ctypes.c_double()
         check(load().pm_batch_autotune(self.h, sample_stride, ctypes.byref(t), ctypes.byref(ms)), "pm_batch_autotune")
-        return (t.leaf_size, t.block_threads, t.heap_slots, t.heap_key_bytes, t.gather_mode), ms.value
+        return (t.leaf_size, t.block_threads, t.heap_slots, t.heap_key_bytes, t.gather_mode, t.top_levels), ms.value
 
     def submit(self):
         check(load().pm_batch_submit(self.h), "pm_batch_submit")

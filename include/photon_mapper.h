@@ -225,6 +225,10 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
  *                      when the scene has one), 0 = they gather at surface hits
  *                      only: the surface-gather measurement on the same photon
  *                      maps (the volume map is built but not read);
+ * PM_OPT_TOP_LEVELS    gather mode 7 (also when auto picks it): the top 0..7
+ *                      levels of both kd-trees (2^levels - 1 nodes, 48 B each)
+ *                      staged in each wave's LDS for the traversals to read,
+ *                      0 = none (default), every node read from HBM;
  * PM_OPT_VOXEL_GRID    0 = the k-NN estimate (default); 1 = the reference's
  *                      voxel-grid estimate (photonMappingKernel.cu:314-389,
  *                      :831-870): both maps splatted into 32^3 grids over the
@@ -246,7 +250,8 @@ enum {
   PM_OPT_TILE_REGIONS = 5,
   PM_OPT_SAMPLE_LANES = 6,
   PM_OPT_RENDER_MEDIA = 7,
-  PM_OPT_VOXEL_GRID = 8
+  PM_OPT_VOXEL_GRID = 8,
+  PM_OPT_TOP_LEVELS = 9
 };
 int pm_set_option(pm_context* ctx, int option, int value);
 
@@ -324,6 +329,8 @@ typedef struct pm_batch_tuning {
   int heap_key_bytes; /* heap key width: 8 (d^2, index), 4 (d^2) or 2 (d^2 bucket); see PM_OPT_GATHER_MODE */
   int gather_mode;    /* 0: the heap gather of heap_key_bytes (modes 1, 2, 4); 7: gather mode 7, which needs
                          heap_key_bytes 2 and heap_slots 64 (its bucket counters take that LDS) */
+  int top_levels;     /* LDS tile, gather mode 7 only: the top 0..7 levels of the current scene's two kd-trees
+                         (2^levels - 1 nodes each, 48 B per node) staged in each wave's LDS, 0 = none */
 } pm_batch_tuning;
 
 typedef struct pm_batch_stats {
@@ -352,7 +359,8 @@ int pm_batch_set_tuning(pm_batch* b, const pm_batch_tuning* t);
 /* Times the kd-build and a 1-in-`sample_stride` subset of the render work
  * items for a staged sweep: heap key width {8, 4, 2} x heap slots {32 (when
  * every k <= 32), 64} and gather mode 7, then leaf size {4, 8, 16, 32}, then
- * workgroup size {64, 128, 256}, each stage from the best so far; keeps the
+ * workgroup size {64, 128, 256}, then (when gather mode 7 won) the LDS tile
+ * top_levels {0, 3, 5, 7}, each stage from the best so far; keeps the
  * fastest and returns it (and its time). */
 int pm_batch_autotune(pm_batch* b, int sample_stride, pm_batch_tuning* best_out, double* best_ms_out);
 /* One frame of every scene (one graph launch); blocking. */

@@ -100,8 +100,9 @@ def assert_same_radiance(ref, img, bitwise, what):
         np.testing.assert_allclose(img, ref, rtol=1e-5, atol=1e-7, err_msg=what)
 
 
-@pytest.mark.parametrize("tuning", [(32, 256, 32, 2, 0), (32, 128, 64, 8, 0), (4, 256, 32, 4, 0),
-                                    (16, 192, 64, 2, 0), (32, 64, 64, 2, 7), (8, 256, 64, 2, 7)])
+@pytest.mark.parametrize("tuning", [(32, 256, 32, 2, 0, 0), (32, 128, 64, 8, 0, 0), (4, 256, 32, 4, 0, 0),
+                                    (16, 192, 64, 2, 0, 0), (32, 64, 64, 2, 7, 0), (8, 256, 64, 2, 7, 0),
+                                    (32, 128, 64, 2, 7, 3), (16, 256, 64, 2, 7, 5), (32, 64, 64, 2, 7, 7)])
 def test_batch_tuning_invariance(batch, tuning):
     cfgs, b = batch
     b.set_tuning(32, 64, 32, 2)
@@ -125,11 +126,12 @@ def test_batch_autotune(batch):
     b.submit()
     ref = [b.image(i) for i in range(len(cfgs))]
     best, ms = b.autotune(sample_stride=3)
-    say("autotune: leaf %d, block %d, heap slots %d, key bytes %d, gather mode %d (%.2f ms on the sampled frame)"
-        % (best + (ms,)))
+    say("autotune: leaf %d, block %d, heap slots %d, key bytes %d, gather mode %d, top levels %d "
+        "(%.2f ms on the sampled frame)" % (best + (ms,)))
     assert best[0] in (4, 8, 16, 32) and best[1] in (64, 128, 256) and best[2] in (32, 64) and ms > 0
-    assert best[3] in (8, 4, 2) and best[4] in (0, 7)
+    assert best[3] in (8, 4, 2) and best[4] in (0, 7) and best[5] in (0, 3, 5, 7)
     assert best[4] == 0 or (best[3], best[2]) == (2, 64)
+    assert best[5] == 0 or best[4] == 7
     b.submit()
     for i in range(len(cfgs)):
         assert_same_radiance(ref[i], b.image(i), best[0] == 32, "scene %d after autotune" % i)
