@@ -192,7 +192,8 @@ def northstar_cfg():
     return c
 
 
-def run_northstar(steps, train_frames, stride, thresholds, batch, rows):
+def run_northstar(steps, train_frames, stride, thresholds, batch, rows, near_groups=2, extra_error_frames=0,
+                  ema=950):
     """Cache frames of the 1080p workload in the BF16 form: emit + build + cache
     render (record queries, train on 2^18 exact kNN targets, MLP inference,
     exact fallback), after `train_frames` frames of online training from a
@@ -206,6 +207,8 @@ def run_northstar(steps, train_frames, stride, thresholds, batch, rows):
     rgb = pmamd.DeviceBuffer(npx * 12)
     g.cache_reset(1)
     g.cache_configure(threshold=thresholds[0], batch=batch, march_stride=stride)
+    g.set_option(pmamd.PM_OPT_CACHE_NEAR_GROUPS, near_groups)
+    g.set_option(pmamd.PM_OPT_CACHE_EMA, ema)
 
     def frame():
         g.emit()
@@ -230,6 +233,12 @@ def run_northstar(steps, train_frames, stride, thresholds, batch, rows):
                 % (i, th, cs.ms_total, cs.ms_infer, cs.fallbacks, cs.records))
         t1 = time.perf_counter()
         img = rgb.download(np.float32, npx * 3).reshape(cfg.height, cfg.width, 3)
+        # the error rows of more frames (untimed; training continues), for the
+        # spread of the error from frame to frame
+        more_rows = []
+        for i in range(extra_error_frames):
+            frame()
+            more_rows.append(rgb.download(np.float32, npx * 3).reshape(cfg.height, cfg.width, 3)[rows].copy())
         m = np.mean(np.array(per), axis=0)
         infer_flop = 154368.0 * cs.records
         out.append({
@@ -245,13 +254,16 @@ def run_northstar(steps, train_frames, stride, thresholds, batch, rows):
                                    "frac": round(infer_flop / (m[2] * 1e-3) / 1e12 / 2500.0, 4),
                                    "flop_per_query": 154368},
             "_rows": img[rows].copy(),
+            "_more_rows": more_rows,
         })
     g.close()
     return {"workload": "the same frame with photon power and direction stored BF16 (positions and splits FP32), "
                         "FP64 transmittance on segments of >= 32 march steps through the density field, and the "
                         "learned radiance cache (MLP 216 -> 4 x 128 -> 3, BF16 MFMA, Adam lr 1e-3, 2^18 training "
                         "samples per frame, exact k=64 fallback above the threshold)",
-            "march_stride": stride, "adam_batch": batch, "train_samples_per_frame": 1 << 18,
+            "march_stride": stride, "near_light_groups": near_groups, "weight_average_decay": ema / 1000.0,
+            "adam_batch": batch,
+            "train_samples_per_frame": 1 << 18,
             "train_frames_before_timing": train_frames, "settings": out}
 
 
@@ -504,6 +516,12 @@ def main():
     ap.add_argument("--cache-train-frames", type=int, default=48)
     ap.add_argument("--cache-batch", type=int, default=8192, help="Adam batch (2^18 training samples per frame)")
     ap.add_argument("--cache-stride", type=int, default=4)
+    ap.add_argument("--cache-error-frames", type=int, default=4,
+                    help="further untimed cache frames whose rows are also compared with the FP64 oracle")
+    ap.add_argument("--cache-ema", type=int, default=950,
+                    help="inference on the moving average of the weights, decay per mille per Adam step (0: off)")
+    ap.add_argument("--cache-near", type=int, default=2,
+                    help="near-light radius of the cache frame in merged march groups (PM_OPT_CACHE_NEAR_GROUPS)")
     ap.add_argument("--cache-thresholds", type=str, default="0.12,0.05",
                     help="fallback thresholds timed in turn (comma-separated)")
     ap.add_argument("--error-rows", type=int, default=16,
@@ -590,7 +608,7 @@ def main():
     if args.cache_steps > 0:
         ths = [float(v) for v in args.cache_thresholds.split(",") if v]
         ns = run_northstar(args.cache_steps, args.cache_train_frames, args.cache_stride, ths, args.cache_batch,
-                           err_rows)
+                           err_rows, args.cache_near, args.cache_error_frames, args.cache_ema)
         for st in ns["settings"]:
             st["speedup_vs_plain_fp32_frame"] = round(out["ms_per_step"] / st["ms_per_frame"], 3)
             st["rmse_vs_plain_fp32_same_samples"] = {
@@ -621,12 +639,19 @@ def main():
             out["plain_fp32_vs_fp64_oracle"] = row_errors(plain_rows, ref, cfg.exposure)
             for st in ns["settings"]:
                 st["error_vs_fp64_oracle"] = row_errors(st["_rows"], ref, cfg.exposure)
+                if st["_more_rows"]:
+                    d = [st["error_vs_fp64_oracle"]["rmse_display_rel"]] + [
+                        row_errors(r, ref, cfg.exposure)["rmse_display_rel"] for r in st["_more_rows"]]
+                    st["display_rmse_vs_fp64_over_frames"] = {"frames": len(d), "mean": round(float(np.mean(d)), 6),
+                                                              "max": round(float(np.max(d)), 6),
+                                                              "each": [round(float(x), 6) for x in d]}
             ns["fp64_reference"] = ("oracle FP64 render (FP32 photon records, FP64 eye paths, march and estimates) "
                                     "of %d stratified rows x 1920 px x 4 spp (%.1f s on the host)"
                                     % (len(err_rows), cb["t_ref"]))
     if ns is not None:
         for st in ns["settings"]:
             del st["_rows"]
+            del st["_more_rows"]
         out["cache_bf16"] = ns
     print(json.dumps(out), flush=True)
 

@@ -591,6 +591,15 @@ int pm_set_option(pm_context* c, int option, int value) {
       if (value != 0 && value != 1) return PM_ERR_INVALID;
       c->render_media = value;
       return PM_OK;
+    case PM_OPT_CACHE_EMA:
+      if (value < 0 || value > 999) return PM_ERR_INVALID;
+      c->cache.ema_decay = (float)value / 1000.0f;
+      c->cache.ema_stale = true;
+      return PM_OK;
+    case PM_OPT_CACHE_NEAR_GROUPS:
+      if (value < 0 || value > 16) return PM_ERR_INVALID;
+      c->cache.near_groups = value;
+      return PM_OK;
     case PM_OPT_TOP_LEVELS:
       if (value < 0 || value > kMaxTopLevels) return PM_ERR_INVALID;
       c->top_levels = value;

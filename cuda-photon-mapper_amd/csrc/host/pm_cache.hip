@@ -59,7 +59,9 @@ int ensure_cache(pm_context* c) {
   HIPCHK(k.grads.ensure(4 * np));
   HIPCHK(k.adam_m.ensure(4 * np));
   HIPCHK(k.adam_v.ensure(4 * np));
-  HIPCHK(k.params_bf16.ensure(2 * nwb));
+  HIPCHK(k.params_bf16.ensure(2 * 2 * nwb));  // W, then W^T
+  HIPCHK(k.params_ema.ensure(4 * np));
+  HIPCHK(k.ema_bf16.ensure(2 * nwb));
   for (int l = 0; l < MLP_LAYERS; ++l) {
     k.w.w[l] = k.params.as<float>() + w_offset(l);
     k.w.b[l] = k.w.w[l] + mlp_w_count(l);
@@ -68,6 +70,10 @@ int ensure_cache(pm_context* c) {
     k.w.vw[l] = k.adam_v.as<float>() + w_offset(l);
     k.w.vb[l] = k.w.vw[l] + mlp_w_count(l);
     k.w.wb[l] = k.params_bf16.as<uint16_t>() + wb_offset(l);
+    k.w.wbt[l] = k.params_bf16.as<uint16_t>() + nwb + wb_offset(l);
+    k.w.we[l] = k.params_ema.as<float>() + w_offset(l);
+    k.w.be[l] = k.w.we[l] + mlp_w_count(l);
+    k.w.wbe[l] = k.ema_bf16.as<uint16_t>() + wb_offset(l);
     k.tb.gw[l] = k.grads.as<float>() + w_offset(l);
     k.tb.gb[l] = k.tb.gw[l] + mlp_w_count(l);
   }
@@ -79,12 +85,10 @@ int ensure_cache(pm_context* c) {
   HIPCHK(k.d0.ensure(2ull * R * MLP_HID));
   HIPCHK(k.d1.ensure(2ull * R * MLP_HID));
   HIPCHK(k.row_err.ensure(4ull * R));
-  HIPCHK(k.partial.ensure(4ull * 64 * MLP_HID * MLP_KPAD));
   k.tb.dy = k.dy.as<uint16_t>();
   k.tb.d0 = k.d0.as<uint16_t>();
   k.tb.d1 = k.d1.as<uint16_t>();
   k.tb.row_err = k.row_err.as<float>();
-  k.tb.partial = k.partial.as<float>();
   k.rows_cap = R;
   k.ready = true;
   return PM_OK;
@@ -94,7 +98,36 @@ int upload_params(pm_context* c, const float* host) {
   CacheState& k = c->cache;
   HIPCHK(hipMemcpy(k.params.p, host, 4 * mlp_param_count(), hipMemcpyHostToDevice));
   HIPCHK(mlp_refresh_bf16(k.w, c->stream));
+  HIPCHK(mlp_reset_ema(k.w, c->stream));
+  k.ema_stale = false;
   HIPCHK(hipStreamSynchronize(c->stream));
+  return PM_OK;
+}
+
+// The training pass's transposed activations and split partials for `rows`.
+int ensure_train_t(CacheState& k, uint32_t rows, MlpTrainT& tt) {
+  const uint32_t ld = mlp_train_ld(rows);
+  HIPCHK(k.XT.ensure(2ull * MLP_KPAD * ld));
+  for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) HIPCHK(k.HT[l].ensure(2ull * MLP_HID * ld));
+  for (int j = 0; j < 2; ++j) HIPCHK(k.dT[j].ensure(2ull * MLP_HID * ld));
+  HIPCHK(k.dyT.ensure(2ull * MLP_OUTPAD * ld));
+  HIPCHK(k.tpart.ensure(4ull * mlp_train_partial_floats(rows)));
+  tt.XT = k.XT.as<uint16_t>();
+  for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) tt.HT[l] = k.HT[l].as<uint16_t>();
+  for (int j = 0; j < 2; ++j) tt.dT[j] = k.dT[j].as<uint16_t>();
+  tt.dyT = k.dyT.as<uint16_t>();
+  tt.ld = ld;
+  tt.part = k.tpart.as<float>();
+  return PM_OK;
+}
+
+// The weights' moving average as the next use needs it: restarted at the
+// current weights after the option changed.
+int sync_ema(pm_context* c) {
+  CacheState& k = c->cache;
+  k.w.ema_decay = k.ema_decay;
+  if (k.ema_stale && k.ema_decay > 0.0f) HIPCHK(mlp_reset_ema(k.w, c->stream));
+  k.ema_stale = false;
   return PM_OK;
 }
 
@@ -107,11 +140,15 @@ uint16_t* const* hidden_ptrs(CacheState& k, uint16_t** tmp) {
 
 namespace pm {
 void cache_release(CacheState& k) {
-  for (DevBuf* b : {&k.params, &k.grads, &k.adam_m, &k.adam_v, &k.params_bf16, &k.X, &k.y, &k.dy, &k.d0, &k.d1,
-                    &k.row_err, &k.partial, &k.rgb, &k.err_sum, &k.err_cnt, &k.rec_cnt, &k.rec_off, &k.rec_in,
-                    &k.rec_wt, &k.rec_pix, &k.rec_val, &k.sel_in, &k.sel_t, &k.flags, &k.flags2, &k.flag_off, &k.list})
+  for (DevBuf* b : {&k.params, &k.grads, &k.adam_m, &k.adam_v, &k.params_bf16, &k.params_ema, &k.ema_bf16, &k.X, &k.y, &k.dy, &k.d0, &k.d1,
+                    &k.row_err, &k.rgb, &k.err_sum, &k.err_cnt, &k.rec_cnt, &k.rec_off, &k.rec_in,
+                    &k.rec_wt, &k.rec_pix, &k.rec_val, &k.sel_in, &k.sel_t, &k.flags, &k.flags2, &k.flag_off, &k.list,
+                    &k.XT, &k.dT[0], &k.dT[1], &k.dyT, &k.tpart})
     b->release();
-  for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) k.H[l].release();
+  for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) {
+    k.H[l].release();
+    k.HT[l].release();
+  }
   k.ready = false;
 }
 
@@ -120,15 +157,17 @@ int cache_predict_dev(pm_context* c, const float* in, uint32_t n, float* rgb) {
   int r = ensure_cache(c);
   if (r != PM_OK) return r;
   CacheState& k = c->cache;
+  if ((r = sync_ema(c)) != PM_OK) return r;
+  const MlpDev wi = mlp_inference_net(k.w);
   if (k.fused_infer) {
     int maxlds = 0;
     HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
     if (k.fused_infer == 3 && (size_t)maxlds >= mlp_fused_lds_bytes()) {
-      HIPCHK(mlp_infer_fused(k.w, in, n, rgb, c->stream));
+      HIPCHK(mlp_infer_fused(wi, in, n, rgb, c->stream));
       return PM_OK;
     }
     if ((size_t)maxlds >= mlp_infer_lds_bytes()) {
-      HIPCHK(mlp_infer(k.w, in, n, k.H[0].as<uint16_t>(), rgb, c->stream, k.fused_infer == 2 ? 1 : 0,
+      HIPCHK(mlp_infer(wi, in, n, k.H[0].as<uint16_t>(), rgb, c->stream, k.fused_infer == 2 ? 1 : 0,
                        1u << k.infer_chunk_log2));
       return PM_OK;
     }
@@ -141,7 +180,7 @@ int cache_predict_dev(pm_context* c, const float* in, uint32_t n, float* rgb) {
   for (uint32_t o = 0; o < n; o += kChunk) {
     uint32_t rows = std::min(kChunk, n - o);
     HIPCHK(mlp_encode(in + (size_t)o * MLP_IN, rows, k.X.as<uint16_t>(), c->stream));
-    HIPCHK(mlp_forward(k.w, k.X.as<uint16_t>(), rows, Hp, k.y.as<float>(), c->stream));
+    HIPCHK(mlp_forward(wi, k.X.as<uint16_t>(), rows, Hp, k.y.as<float>(), c->stream));
     HIPCHK(mlp_decode(k.y.as<float>(), rows, rgb + (size_t)o * MLP_OUT, c->stream));
   }
   return PM_OK;
@@ -156,21 +195,31 @@ int cache_train_dev(pm_context* c, const float* in, const float* target, uint32_
   if (batch == 0 || batch > kChunk) return PM_ERR_INVALID;
   uint16_t* tmp[MLP_HIDDEN_LAYERS];
   uint16_t* const* H = hidden_ptrs(k, tmp);
+  MlpTrainT tt;
+  if ((r = ensure_train_t(k, batch, tt)) != PM_OK) return r;
+  if ((r = sync_ema(c)) != PM_OK) return r;
   uint64_t cnt = 0;
+  uint32_t ro = 0;  // rows of squared errors in k.row_err not yet summed
   HIPCHK(c->small.ensure(64));
   double* dloss = reinterpret_cast<double*>(c->small.as<uint32_t>() + 12);
   HIPCHK(hipMemsetAsync(dloss, 0, sizeof(double), c->stream));
   for (uint32_t o = 0; o + batch <= n; o += batch) {
-    HIPCHK(mlp_encode(in + (size_t)o * MLP_IN, batch, k.X.as<uint16_t>(), c->stream));
-    HIPCHK(mlp_forward(k.w, k.X.as<uint16_t>(), batch, H, k.y.as<float>(), c->stream));
-    HIPCHK(mlp_backward(k.w, k.X.as<uint16_t>(), batch, H, k.y.as<float>(), target + (size_t)o * MLP_OUT, k.tb,
-                        c->stream));
-    if (esum) HIPCHK(launch_err_accum(in + (size_t)o * MLP_IN, k.tb.row_err, batch, esum, ecnt, c->stream));
+    // each step's squared errors go to the next rows of k.row_err, summed in one pass when it is full
+    if (ro + batch > k.rows_cap) {
+      HIPCHK(launch_sum_rows(k.row_err.as<float>(), ro, dloss, c->stream));
+      ro = 0;
+    }
+    MlpTrainBufs tb = k.tb;
+    tb.row_err = k.row_err.as<float>() + ro;
+    HIPCHK(mlp_train_grads(k.w, in + (size_t)o * MLP_IN, batch, k.X.as<uint16_t>(), H, k.y.as<float>(),
+                           target + (size_t)o * MLP_OUT, tb, tt, false, c->stream));
+    if (esum) HIPCHK(launch_err_accum(in + (size_t)o * MLP_IN, tb.row_err, batch, esum, ecnt, c->stream));
     k.step += 1;
-    HIPCHK(mlp_adam(k.w, k.tb, lr, k.step, c->stream));
-    HIPCHK(launch_sum_rows(k.row_err.as<float>(), batch, dloss, c->stream));
+    HIPCHK(mlp_adam(k.w, tb, lr, k.step, c->stream, &tt));
+    ro += batch;
     cnt += (uint64_t)batch * MLP_OUT;
   }
+  if (ro) HIPCHK(launch_sum_rows(k.row_err.as<float>(), ro, dloss, c->stream));
   double total = 0.0;
   HIPCHK(hipMemcpyAsync(&total, dloss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -213,9 +262,10 @@ int pm_cache_grad(pm_context* c, const float* in_dev, const float* target_dev, u
   CacheState& k = c->cache;
   uint16_t* tmp[MLP_HIDDEN_LAYERS];
   uint16_t* const* H = hidden_ptrs(k, tmp);
-  HIPCHK(mlp_encode(in_dev, n, k.X.as<uint16_t>(), c->stream));
-  HIPCHK(mlp_forward(k.w, k.X.as<uint16_t>(), n, H, k.y.as<float>(), c->stream));
-  HIPCHK(mlp_backward(k.w, k.X.as<uint16_t>(), n, H, k.y.as<float>(), target_dev, k.tb, c->stream));
+  MlpTrainT tt;
+  if ((r = ensure_train_t(k, n, tt)) != PM_OK) return r;
+  HIPCHK(mlp_train_grads(k.w, in_dev, n, k.X.as<uint16_t>(), H, k.y.as<float>(), target_dev, k.tb, tt, true,
+                         c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy(grad_host, k.grads.p, 4ull * count, hipMemcpyDeviceToHost));
   if (loss) {
@@ -307,8 +357,8 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   rl.counts = k.rec_cnt.as<uint32_t>();
   rl.write = 0;
   rl.stride = k.stride;
-  // near-light radius: 4 merged groups (none without merging)
-  const float near = k.stride > 1 ? 4.0f * (float)k.stride * c->rp.march_step : 0.0f;
+  // near-light radius: near_groups merged groups (none without merging)
+  const float near = k.stride > 1 ? (float)k.near_groups * (float)k.stride * c->rp.march_step : 0.0f;
   rl.near2 = near * near;
   HIPCHK(launch_cache_record(rl, st));
   HIPCHK(exclusive_scan(k.rec_cnt.as<uint32_t>(), k.rec_off.as<uint32_t>(), npix, c->scratch.as<uint32_t>(), nullptr,

@@ -50,8 +50,9 @@ struct MapBufs {
 
 // Learned radiance cache state: parameters, Adam moments, training scratch.
 struct CacheState {
-  DevBuf params, grads, adam_m, adam_v, params_bf16;
-  DevBuf X, H[MLP_HIDDEN_LAYERS], y, dy, d0, d1, row_err, partial, rgb;
+  DevBuf params, grads, adam_m, adam_v, params_bf16, params_ema, ema_bf16;
+  DevBuf X, H[MLP_HIDDEN_LAYERS], y, dy, d0, d1, row_err, rgb;
+  DevBuf XT, HT[MLP_HIDDEN_LAYERS], dT[2], dyT, tpart;  // training: transposes and split partials
   MlpDev w;
   MlpTrainBufs tb;
   uint32_t rows_cap = 0;
@@ -70,6 +71,9 @@ struct CacheState {
   int infer_chunk_log2 = 20;  // rows per fused-inference launch pair (log2)
   int stride = 1;          // march points per volume record
   int cache_surface = 0;   // 0: surface queries are always gathered exactly
+  int near_groups = 2;     // near-light radius in merged march groups
+  float ema_decay = 0.95f;  // inference on the moving average of the weights (decay per Adam step), 0: off
+  bool ema_stale = true;    // the average must restart at the current weights before its next use
   DevBuf rec_cnt, rec_off, rec_in, rec_wt, rec_pix, rec_val, sel_in, sel_t, flags, flags2, flag_off, list;
   pm_cache_stats st;
 };

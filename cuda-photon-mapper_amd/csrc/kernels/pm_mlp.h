@@ -40,10 +40,17 @@ struct MlpDev {
   float* w[MLP_LAYERS];       // FP32 master weights
   float* b[MLP_LAYERS];
   uint16_t* wb[MLP_LAYERS];   // BF16 copies used by the MFMA GEMMs
+  uint16_t* wbt[MLP_LAYERS];  // BF16 transposes [kin x nout] (the training's delta products)
   float* mw[MLP_LAYERS];      // Adam moments
   float* vw[MLP_LAYERS];
   float* mb[MLP_LAYERS];
   float* vb[MLP_LAYERS];
+  // exponential moving average of the weights (the inference network when
+  // ema_decay > 0): FP32 weights and biases, BF16 weights
+  float* we[MLP_LAYERS];
+  float* be[MLP_LAYERS];
+  uint16_t* wbe[MLP_LAYERS];
+  float ema_decay;            // per Adam step; 0: no average
 };
 
 struct MlpTrainBufs {
@@ -51,19 +58,50 @@ struct MlpTrainBufs {
   uint16_t* d0;       // [rows x HID] deltas (ping-pong)
   uint16_t* d1;
   float* row_err;     // [rows]
-  float* partial;     // split-K partials
   float* gw[MLP_LAYERS];
   float* gb[MLP_LAYERS];
 };
+
+// Transposed copies kept by the training pass (pm_mlp_train.hip): every
+// activation and delta also as [features x ld] (ld = rows rounded up to 64),
+// the operands of the weight-gradient products, and the split partials.
+struct MlpTrainT {
+  uint16_t* XT;                     // [KPAD x ld]
+  uint16_t* HT[MLP_HIDDEN_LAYERS];  // [HID x ld]
+  uint16_t* dT[2];                  // [HID x ld], ping-pong
+  uint16_t* dyT;                    // [OUTPAD x ld]
+  uint32_t ld;
+  float* part;                      // mlp_train_partial_floats(rows)
+  // filled by mlp_train_grads: layer l's weight and bias partials, slice z at
+  // part_w[l] + z * (its weight count) and part_b[l] + z * (its bias count)
+  const float* part_w[MLP_LAYERS];
+  const float* part_b[MLP_LAYERS];
+  int slices;
+};
+int mlp_train_slices(uint32_t rows);
+uint32_t mlp_train_ld(uint32_t rows);
+size_t mlp_train_partial_floats(uint32_t rows);
+// Encode, forward (keeping H and its transposes), loss, backward for `rows`
+// inputs / targets: y the outputs, tb.row_err the per-row squared errors, and
+// the gradients as split partials (tt.part_w / part_b); with `reduce` they are
+// also summed into tb.gw / tb.gb (mlp_adam can instead sum them itself).
+hipError_t mlp_train_grads(const MlpDev& w, const float* in, uint32_t rows, uint16_t* X, uint16_t* const* H,
+                           float* y, const float* target, MlpTrainBufs& tb, MlpTrainT& tt, bool reduce,
+                           hipStream_t st);
 
 hipError_t mlp_encode(const float* in, uint32_t n, uint16_t* out, hipStream_t st);
 hipError_t mlp_forward(const MlpDev& w, const uint16_t* X, uint32_t rows, uint16_t* const* H, float* y,
                        hipStream_t st);
 hipError_t mlp_decode(const float* y, uint32_t rows, float* rgb, hipStream_t st);
-hipError_t mlp_backward(const MlpDev& w, const uint16_t* X, uint32_t rows, uint16_t* const* H, const float* y,
-                        const float* target, MlpTrainBufs& tb, hipStream_t st);
-hipError_t mlp_adam(MlpDev& w, const MlpTrainBufs& tb, float lr, int step, hipStream_t st);
+// One Adam step on the gradients in tb.gw / tb.gb, or, with parts, on the sum
+// over slices (in slice order) of the partials mlp_train_grads left there.
+hipError_t mlp_adam(MlpDev& w, const MlpTrainBufs& tb, float lr, int step, hipStream_t st,
+                    const MlpTrainT* parts = nullptr);
 hipError_t mlp_refresh_bf16(MlpDev& w, hipStream_t st);
+// The moving average restarted at the current weights (FP32 and BF16 copies).
+hipError_t mlp_reset_ema(MlpDev& w, hipStream_t st);
+// The network inference reads: the moving average when ema_decay > 0, else w.
+MlpDev mlp_inference_net(const MlpDev& w);
 // Fused inference (pm_mlp_infer.hip): h1 holds mlp_infer_h1_rows() x 128 BF16.
 size_t mlp_infer_h1_rows();
 size_t mlp_infer_lds_bytes();

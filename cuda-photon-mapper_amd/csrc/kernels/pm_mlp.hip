@@ -1,5 +1,7 @@
-// pm_mlp.hip -- learned radiance cache: frequency encoding, BF16 MFMA GEMMs
-// with FP32 accumulation, backward pass and Adam, for gfx950.
+// pm_mlp.hip -- learned radiance cache: frequency encoding, the general BF16
+// MFMA GEMM (FP32 accumulation) of the unfused inference path, and Adam, for
+// gfx950.  The training pass is in pm_mlp_train.hip, the fused inference in
+// pm_mlp_infer.hip.
 //
 // No reference counterpart: the reference has no learned component (its
 // radiance comes only from the voxel-grid sums, photonMappingKernel.cu:
@@ -12,10 +14,8 @@
 // GEMM: C[M x N] = A[M x K] * B[K x N], A/B in BF16 (either may be read
 // transposed), FP32 accumulation on v_mfma_f32_16x16x16_bf16.  A 256-thread
 // workgroup computes a 64x64 tile of C; each wave a 32x32 quarter (2x2 MFMA
-// tiles); K advances 32 at a time through LDS.  The weight-gradient GEMM
-// splits its long K (the batch) over grid.z slices that write FP32 partials;
-// a second kernel adds the slices in a fixed order, so results are
-// deterministic run to run.
+// tiles); K advances 32 at a time through LDS.  With grid.z > 1 it splits K
+// over slices that write FP32 partials (slice z at Cf + z * slice_stride).
 #include "pm_kernels.h"
 #include "pm_mlp.h"
 #include "pm_wave.h"
@@ -179,58 +179,6 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
       }
 }
 
-// Both reductions of one layer's gradients in one launch.  Blocks below
-// nwb = ceil(n / 256) sum the split-K weight partials over the slices, in
-// slice order; the next `cols` blocks take the column sums of the BF16 delta
-// [rows x cols] (bias gradients), one block per column with a fixed-order
-// tree reduction.
-__global__ void __launch_bounds__(256) k_grad_reduce(const float* __restrict__ part, int slices, size_t stride,
-                                                     size_t n, unsigned nwb, float* __restrict__ gw,
-                                                     const uint16_t* __restrict__ x, int rows, int cols, int ld,
-                                                     float* __restrict__ gb) {
-  __shared__ float red[256];
-  if (blockIdx.x < nwb) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float s = 0.0f;
-    for (int z = 0; z < slices; ++z) s += part[(size_t)z * stride + i];
-    gw[i] = s;
-    return;
-  }
-  const int c = (int)(blockIdx.x - nwb);
-  float s = 0.0f;
-  for (int r = threadIdx.x; r < rows; r += 256) s += bf2f(x[(size_t)r * ld + c]);
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) gb[c] = red[0];
-}
-
-// Loss L = mean over (rows, 3 channels) of (y - t)^2 on y = net output,
-// t = log(1 + target).  Writes dL/dy (BF16, padded to MLP_OUTPAD columns)
-// and per-row squared errors.
-__global__ void k_loss_grad(const float* __restrict__ y, const float* __restrict__ target, int rows,
-                            uint16_t* __restrict__ dy, float* __restrict__ row_err) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= rows) return;
-  float e2 = 0.0f;
-  const float inv = 2.0f / (float)(rows * MLP_OUT);
-  for (int c = 0; c < MLP_OUTPAD; ++c) {
-    float g = 0.0f;
-    if (c < MLP_OUT) {
-      float t = log_(1.0f + fmax_(target[(size_t)r * MLP_OUT + c], 0.0f));
-      float d = y[(size_t)r * MLP_OUTPAD + c] - t;
-      e2 += d * d;
-      g = d * inv;
-    }
-    dy[(size_t)r * MLP_OUTPAD + c] = f2bf(g);
-  }
-  row_err[r] = e2;
-}
-
 // Adam on FP32 master parameters, refreshing the BF16 copy.
 // Adam over every weight and bias tensor in one launch: tensor s takes the
 // blocks [first[s], first[s + 1]).
@@ -240,7 +188,14 @@ struct AdamSegs {
   float* v[2 * MLP_LAYERS];
   const float* g[2 * MLP_LAYERS];
   uint16_t* pb[2 * MLP_LAYERS];
+  uint16_t* pbt[2 * MLP_LAYERS];  // transposed BF16 copy [kin x nout] of a weight tensor, or null
+  uint32_t kin[2 * MLP_LAYERS], nout[2 * MLP_LAYERS];
   uint32_t n[2 * MLP_LAYERS];
+  const float* gp[2 * MLP_LAYERS];  // split partials (slice z at gp + z * n), when slices > 0
+  int slices;
+  float* pe[2 * MLP_LAYERS];        // moving average of the tensor (null: none)
+  uint16_t* pbe[2 * MLP_LAYERS];    // its BF16 copy (weights only)
+  float ema_decay;
   uint32_t first[2 * MLP_LAYERS + 1];
 };
 
@@ -252,7 +207,13 @@ __global__ void k_adam_all(AdamSegs S, float lr, float b1, float b2, float eps, 
   float* p = S.p[s];
   float* m = S.m[s];
   float* v = S.v[s];
-  float gr = S.g[s][i];
+  float gr;
+  if (S.slices > 0) {  // the same fixed-order sum over slices as k_grad_sum
+    gr = 0.0f;
+    for (int z = 0; z < S.slices; ++z) gr += S.gp[s][(size_t)z * S.n[s] + i];
+  } else {
+    gr = S.g[s][i];
+  }
   float mi = b1 * m[i] + (1.0f - b1) * gr;
   float vi = b2 * v[i] + (1.0f - b2) * gr * gr;
   m[i] = mi;
@@ -261,11 +222,22 @@ __global__ void k_adam_all(AdamSegs S, float lr, float b1, float b2, float eps, 
   float pi = p[i] - lr * mh / (sqrt_(vh) + eps);
   p[i] = pi;
   if (S.pb[s]) S.pb[s][i] = f2bf(pi);
+  if (S.pbt[s]) S.pbt[s][(i % S.kin[s]) * S.nout[s] + i / S.kin[s]] = f2bf(pi);
+  if (S.pe[s]) {
+    const float e = S.ema_decay * S.pe[s][i] + (1.0f - S.ema_decay) * pi;
+    S.pe[s][i] = e;
+    if (S.pbe[s]) S.pbe[s][i] = f2bf(e);
+  }
 }
 
-__global__ void k_to_bf16(const float* __restrict__ x, size_t n, uint16_t* __restrict__ y) {
+// BF16 copy of an [nout x kin] weight matrix, and its transpose when yt is set.
+__global__ void k_to_bf16(const float* __restrict__ x, size_t n, uint16_t* __restrict__ y, uint16_t* __restrict__ yt,
+                          uint32_t kin, uint32_t nout) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] = f2bf(x[i]);
+  if (i >= n) return;
+  const uint16_t h = f2bf(x[i]);
+  y[i] = h;
+  if (yt) yt[(i % kin) * nout + i / kin] = h;
 }
 
 // radiance = max(0, exp(y) - 1) for the first 3 output columns.
@@ -334,51 +306,7 @@ hipError_t mlp_decode(const float* y, uint32_t rows, float* rgb, hipStream_t st)
   return hipGetLastError();
 }
 
-// Gradients of the loss for one batch (the forward must have filled H and y).
-// grads: FP32 buffers laid out like the parameters; scratch per MlpTrainBufs.
-hipError_t mlp_backward(const MlpDev& w, const uint16_t* X, uint32_t rows, uint16_t* const* H, const float* y,
-                        const float* target, MlpTrainBufs& tb, hipStream_t st) {
-  hipError_t e;
-  hipLaunchKernelGGL(k_loss_grad, dim3((rows + 255) / 256), dim3(256), 0, st, y, target, (int)rows, tb.dy,
-                     tb.row_err);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  const int slices = (int)std::max<uint32_t>(1u, std::min<uint32_t>(64u, rows / 256u));
-  const int kps = (int)(((rows + slices - 1) / slices + GK - 1) / GK * GK);
-  // delta of the current layer's pre-activation (BF16): starts as dL/dy.
-  const uint16_t* delta = tb.dy;
-  int dcols = MLP_OUTPAD;
-  for (int l = MLP_HIDDEN_LAYERS; l >= 0; --l) {
-    const uint16_t* act = l == 0 ? X : H[l - 1];
-    const int kin = l == 0 ? MLP_KPAD : MLP_HID;
-    const int nout = l == MLP_HIDDEN_LAYERS ? MLP_OUTPAD : MLP_HID;
-    // dW_l [nout x kin] = delta^T [nout x rows] * act [rows x kin], split over the batch
-    GemmArgs a = gemm_args(nout, kin, (int)rows);
-    a.A = delta; a.lda = dcols; a.transA = 1;
-    a.B = act; a.ldb = kin; a.transB = 0;
-    a.Cf = tb.partial; a.ldc = kin; a.slice_stride = (size_t)nout * kin;
-    a.k_per_slice = kps;
-    if ((e = gemm(a, slices, st)) != hipSuccess) return e;
-    size_t nw = (size_t)nout * kin;
-    const unsigned nwb = (unsigned)((nw + 255) / 256);
-    hipLaunchKernelGGL(k_grad_reduce, dim3(nwb + (unsigned)nout), dim3(256), 0, st, tb.partial, slices,
-                       (size_t)nout * kin, nw, nwb, tb.gw[l], delta, (int)rows, nout, dcols, tb.gb[l]);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (l == 0) break;
-    // delta_{l-1} [rows x 128] = (delta [rows x nout] * W_l [nout x 128]) * (H[l-1] > 0)
-    uint16_t* nd = (l & 1) ? tb.d1 : tb.d0;
-    GemmArgs b = gemm_args((int)rows, MLP_HID, nout);
-    b.A = delta; b.lda = dcols; b.transA = 0;
-    b.B = w.wb[l]; b.ldb = kin; b.transB = 0;  // W [nout x kin] row-major: B(k=out, n=in)
-    b.Cb = nd; b.ldc = MLP_HID;
-    b.mask = H[l - 1];
-    if ((e = gemm(b, 1, st)) != hipSuccess) return e;
-    delta = nd;
-    dcols = MLP_HID;
-  }
-  return hipSuccess;
-}
-
-hipError_t mlp_adam(MlpDev& w, const MlpTrainBufs& tb, float lr, int step, hipStream_t st) {
+hipError_t mlp_adam(MlpDev& w, const MlpTrainBufs& tb, float lr, int step, hipStream_t st, const MlpTrainT* parts) {
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   float c1 = 1.0f - (float)std::pow(0.9, (double)step), c2 = 1.0f - (float)std::pow(0.999, (double)step);
   AdamSegs S;
@@ -391,20 +319,51 @@ hipError_t mlp_adam(MlpDev& w, const MlpTrainBufs& tb, float lr, int step, hipSt
       S.v[s] = j ? w.vb[l] : w.vw[l];
       S.g[s] = j ? tb.gb[l] : tb.gw[l];
       S.pb[s] = j ? nullptr : w.wb[l];
+      S.pbt[s] = j ? nullptr : w.wbt[l];
+      S.kin[s] = (uint32_t)mlp_kin(l);
+      S.nout[s] = (uint32_t)mlp_nout(l);
       S.n[s] = (uint32_t)(j ? mlp_b_count(l) : mlp_w_count(l));
+      S.gp[s] = parts ? (j ? parts->part_b[l] : parts->part_w[l]) : nullptr;
+      S.pe[s] = w.ema_decay > 0.0f ? (j ? w.be[l] : w.we[l]) : nullptr;
+      S.pbe[s] = w.ema_decay > 0.0f && !j ? w.wbe[l] : nullptr;
       S.first[s] = blocks;
       blocks += (S.n[s] + 255u) / 256u;
     }
   }
   S.first[2 * MLP_LAYERS] = blocks;
+  S.slices = parts ? parts->slices : 0;
+  S.ema_decay = w.ema_decay;
   hipLaunchKernelGGL(k_adam_all, dim3(blocks), dim3(256), 0, st, S, lr, b1, b2, eps, c1, c2);
   return hipGetLastError();
+}
+
+MlpDev mlp_inference_net(const MlpDev& w) {
+  MlpDev n = w;
+  if (w.ema_decay > 0.0f)
+    for (int l = 0; l < MLP_LAYERS; ++l) {
+      n.w[l] = w.we[l];
+      n.b[l] = w.be[l];
+      n.wb[l] = w.wbe[l];
+    }
+  return n;
+}
+
+hipError_t mlp_reset_ema(MlpDev& w, hipStream_t st) {
+  for (int l = 0; l < MLP_LAYERS; ++l) {
+    hipError_t e = hipMemcpyAsync(w.we[l], w.w[l], 4 * mlp_w_count(l), hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(w.be[l], w.b[l], 4 * mlp_b_count(l), hipMemcpyDeviceToDevice, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(w.wbe[l], w.wb[l], 2 * mlp_w_count(l), hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t mlp_refresh_bf16(MlpDev& w, hipStream_t st) {
   for (int l = 0; l <= MLP_HIDDEN_LAYERS; ++l) {
     size_t nw = mlp_w_count(l);
-    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, w.w[l], nw, w.wb[l]);
+    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, st, w.w[l], nw, w.wb[l],
+                       w.wbt[l], (uint32_t)mlp_kin(l), (uint32_t)mlp_nout(l));
   }
   return hipGetLastError();
 }

@@ -1,0 +1,408 @@
+// pm_mlp_train.hip -- the learned cache's training pass (forward with stored
+// activations, backward, weight and bias gradients) on gfx950, built from one
+// "NT" GEMM kernel whose operands are both contiguous along K.
+//
+// No reference counterpart (the reference has no learned component).
+//
+// Every product of a training step has the form C[m][n] = sum_k A[m][k] B[n][k]
+// once each operand is kept in the layout that makes K its contiguous axis:
+//   forward      H_l [rows x out] = X_{l} [rows x in] . W_l [out x in]^T
+//   delta        D_{l-1} [rows x in] = D_l [rows x out] . (W_l^T) [in x out]^T
+//   weight grad  dW_l [out x in] = (D_l^T) [out x rows] . (X_l^T) [in x rows]^T
+// so the forward and delta products also write their output transposed (the
+// weight gradients' operands), the encoder writes X and X^T, and the BF16
+// weights are kept in both orders (W and W^T, refreshed by Adam).  A
+// 256-thread workgroup computes a 64 x 64 tile of C; each wave a 32 x 32
+// quarter on v_mfma_f32_16x16x32_bf16.  K is staged through LDS 128 at a time
+// with 16-byte loads and 16-byte LDS fragment reads.  The weight gradients
+// split the batch (K) over grid.z slices that write FP32 partial tiles, and
+// the bias gradients (row sums of D^T) likewise; the slices of every layer are
+// added in slice order (deterministic), by one kernel or by Adam itself.
+#include "pm_kernels.h"
+#include "pm_mlp.h"
+#include "pm_wave.h"
+
+namespace pm {
+
+namespace {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int NT_TILE = 64;  // tile M and N
+constexpr int NT_KC = 128;   // K per LDS stage
+constexpr int NT_LDS = NT_KC + 8;  // LDS row length (bf16), padded
+
+__device__ __forceinline__ float bf_to_f(uint16_t h) { return bf16_to_f32(h); }
+
+struct NtArgs {
+  int M, N, K;
+  const uint16_t* A;  // A[m * lda + k]
+  int lda;
+  const uint16_t* B;  // B[n * ldb + k]
+  int ldb;
+  // epilogue of an unsplit product
+  const float* bias;       // + bias[n]
+  int relu;                // max(0, .)
+  const uint16_t* mask;    // * (mask[m * ldm + n] > 0)
+  int ldm;
+  uint16_t* Cb;            // BF16 C[m * ldc + n]
+  int ldc;
+  uint16_t* Ct;            // BF16 C^T[n * ldt + m] (ldt >= M rounded up to 64)
+  int ldt;
+  float* Cf;               // FP32 C[m * ldf + n]
+  int ldf;
+  // split product (gridDim.z slices of kps): partial tiles part[z][m][n],
+  // and when bpart is set the row sums of A over the slice, bpart[z][m]
+  int kps;
+  float* part;
+  float* bpart;
+};
+
+// Stages A rows [m0, m0+64) and B rows [n0, n0+64), k in [k0, k0 + kc) (kc a
+// multiple of 32), into LDS; elements outside M, N or k_end are zero.
+__device__ __forceinline__ void nt_stage(const NtArgs& g, uint16_t (*As)[NT_LDS], uint16_t (*Bs)[NT_LDS], int m0,
+                                         int n0, int k0, int kc, int k_end) {
+  const int vec_per_row = kc / 8;
+  for (int v = (int)threadIdx.x; v < 2 * NT_TILE * vec_per_row; v += 256) {
+    const bool isB = v >= NT_TILE * vec_per_row;
+    const int w = isB ? v - NT_TILE * vec_per_row : v;
+    const int r = w / vec_per_row, kk = (w % vec_per_row) * 8;
+    const int gr = (isB ? n0 : m0) + r, gk = k0 + kk;
+    const int rlim = isB ? g.N : g.M;
+    const uint16_t* src = isB ? g.B + (size_t)gr * g.ldb + gk : g.A + (size_t)gr * g.lda + gk;
+    uint4 q = make_uint4(0u, 0u, 0u, 0u);
+    if (gr < rlim) {
+      if (gk + 8 <= k_end) {
+        q = *(const uint4*)src;
+      } else if (gk < k_end) {
+        uint16_t e[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) e[j] = gk + j < k_end ? src[j] : (uint16_t)0;
+        q.x = (uint32_t)e[0] | ((uint32_t)e[1] << 16);
+        q.y = (uint32_t)e[2] | ((uint32_t)e[3] << 16);
+        q.z = (uint32_t)e[4] | ((uint32_t)e[5] << 16);
+        q.w = (uint32_t)e[6] | ((uint32_t)e[7] << 16);
+      }
+    }
+    uint16_t* dst = isB ? &Bs[r][kk] : &As[r][kk];
+    *(uint4*)dst = q;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_gemm_nt(NtArgs g) {
+  __shared__ __align__(16) uint16_t As[NT_TILE][NT_LDS];
+  __shared__ __align__(16) uint16_t Bs[NT_TILE][NT_LDS];
+  const int lane = lane_id();
+  const int wave = (int)(threadIdx.x >> 6);
+  const int m0 = (int)blockIdx.y * NT_TILE, n0 = (int)blockIdx.x * NT_TILE;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const bool split = gridDim.z > 1 || g.part;
+  const int k_begin = split ? (int)blockIdx.z * g.kps : 0;
+  const int k_end = split ? min(g.K, k_begin + g.kps) : g.K;
+  const int r16 = lane & 15, q = lane >> 4;
+  f32x4_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+  float rowsum = 0.0f;  // bias gradient: row threadIdx.x of A over this slice (n-tile 0)
+  const bool want_rowsum = g.bpart && blockIdx.x == 0 && threadIdx.x < NT_TILE;
+  // the epilogue's bias and mask values, loaded before the K loop so that
+  // their latency overlaps the operand loads
+  float bias_v[2] = {0.0f, 0.0f};
+  uint16_t mask_v[2][2][4];
+  if (!split) {
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int n = n0 + wn + 16 * b + r16;
+      if (g.bias && n < g.N) bias_v[b] = g.bias[n];
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int m = m0 + wm + 16 * a + 4 * q + e, n = n0 + wn + 16 * b + r16;
+          // 0x3f80 is 1.0 in BF16: kept when there is no mask
+          mask_v[a][b][e] = g.mask && m < g.M && n < g.N ? g.mask[(size_t)m * g.ldm + n] : (uint16_t)0x3f80;
+        }
+  }
+  for (int k0 = k_begin; k0 < k_end; k0 += NT_KC) {
+    const int kc = min(NT_KC, (k_end - k0 + 31) / 32 * 32);
+    nt_stage(g, As, Bs, m0, n0, k0, kc, k_end);
+    __syncthreads();
+    if (want_rowsum)
+      for (int k = 0; k < kc; ++k) rowsum += bf_to_f(As[threadIdx.x][k]);
+    for (int ks = 0; ks < kc; ks += 32) {
+      bf16x8_t af[2], bfr[2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const uint4 va = *(const uint4*)&As[wm + t * 16 + r16][ks + 8 * q];
+        af[t] = *(const bf16x8_t*)&va;
+        const uint4 vb = *(const uint4*)&Bs[wn + t * 16 + r16][ks + 8 * q];
+        bfr[t] = *(const bf16x8_t*)&vb;
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // lane holds C[wm + 16a + 4q + e][wn + 16b + r16]
+  if (split) {
+    float* P = g.part + (size_t)blockIdx.z * g.M * g.N;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int m = m0 + wm + 16 * a + 4 * q + e, n = n0 + wn + 16 * b + r16;
+          if (m < g.M && n < g.N) P[(size_t)m * g.N + n] = acc[a][b][e];
+        }
+    if (want_rowsum && m0 + (int)threadIdx.x < g.M)
+      g.bpart[(size_t)blockIdx.z * g.M + m0 + threadIdx.x] = rowsum;
+    return;
+  }
+  // unsplit: bias, ReLU, mask, then the BF16 / FP32 outputs; the transposed
+  // BF16 output goes through LDS so its rows are written 16 bytes at a time
+  uint16_t (*T)[NT_TILE + 8] = (uint16_t (*)[NT_TILE + 8])&As[0][0];  // [n][m], 9.2 KB of As
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int ml = wm + 16 * a + 4 * q + e, nl = wn + 16 * b + r16;
+        const int m = m0 + ml, n = n0 + nl;
+        float v = acc[a][b][e];
+        if (n < g.N) {
+          if (g.bias) v += bias_v[b];
+          if (g.relu) v = v > 0.0f ? v : 0.0f;
+          if (!(bf_to_f(mask_v[a][b][e]) > 0.0f)) v = 0.0f;
+        }
+        const uint16_t h = f32_to_bf16(v);
+        if (m < g.M && n < g.N) {
+          if (g.Cb) g.Cb[(size_t)m * g.ldc + n] = h;
+          if (g.Cf) g.Cf[(size_t)m * g.ldf + n] = v;
+        }
+        if (g.Ct) T[nl][ml] = h;
+      }
+  if (g.Ct) {
+    __syncthreads();
+    // 64 rows (n) x 8 vectors of 8 m; columns m >= M land in the padding of C^T
+    for (int v = (int)threadIdx.x; v < NT_TILE * 8; v += 256) {
+      const int nl = v >> 3, j = (v & 7) * 8;
+      if (n0 + nl < g.N) *(uint4*)&g.Ct[(size_t)(n0 + nl) * g.ldt + m0 + j] = *(const uint4*)&T[nl][j];
+    }
+  }
+}
+
+hipError_t gemm_nt(const NtArgs& a, int slices, hipStream_t st) {
+  dim3 grid((unsigned)((a.N + NT_TILE - 1) / NT_TILE), (unsigned)((a.M + NT_TILE - 1) / NT_TILE), (unsigned)slices);
+  hipLaunchKernelGGL(k_gemm_nt, grid, dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+NtArgs nt_args(int M, int N, int K) {
+  NtArgs a;
+  memset(&a, 0, sizeof(a));
+  a.M = M;
+  a.N = N;
+  a.K = K;
+  return a;
+}
+
+// Encoding of the inputs as rows (X [rows x KPAD]) and as columns
+// (X^T [KPAD x ld]); one lane per (input, row), rows fastest, so the X^T
+// stores of a wave are contiguous.  Features follow k_encode: out[i*24 + 2b +
+// {0,1}] = {sin, cos}(2 pi 2^(b-1) v).
+__global__ void k_encode_t(const float* __restrict__ in, uint32_t n, uint16_t* __restrict__ X,
+                           uint16_t* __restrict__ XT, uint32_t ld) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t i = t / n, r = t % n;
+  if (i >= (uint32_t)MLP_IN) return;
+  const float v = in[(size_t)r * MLP_IN + i];
+  uint16_t* o = X + (size_t)r * MLP_KPAD + i * 2 * MLP_BANDS;
+  float scale = 0.5f;
+  for (int b = 0; b < MLP_BANDS; ++b) {
+    float s, c;
+    sincos_turns(v * scale, &s, &c);
+    const uint16_t hs = f32_to_bf16(s), hc = f32_to_bf16(c);
+    o[2 * b] = hs;
+    o[2 * b + 1] = hc;
+    XT[(size_t)(i * 2 * MLP_BANDS + 2 * b) * ld + r] = hs;
+    XT[(size_t)(i * 2 * MLP_BANDS + 2 * b + 1) * ld + r] = hc;
+    scale *= 2.0f;
+  }
+  if (i == MLP_IN - 1)
+    for (int k = MLP_FEAT; k < MLP_KPAD; ++k) {
+      X[(size_t)r * MLP_KPAD + k] = 0;
+      XT[(size_t)k * ld + r] = 0;
+    }
+}
+
+// Loss gradient of k_loss_grad, also written transposed (dy^T [OUTPAD x ld]).
+__global__ void k_loss_grad_t(const float* __restrict__ y, const float* __restrict__ target, int rows,
+                              uint16_t* __restrict__ dy, uint16_t* __restrict__ dyT, uint32_t ld,
+                              float* __restrict__ row_err) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  float e2 = 0.0f;
+  const float inv = 2.0f / (float)(rows * MLP_OUT);
+  for (int c = 0; c < MLP_OUTPAD; ++c) {
+    float gr = 0.0f;
+    if (c < MLP_OUT) {
+      const float t = log_(1.0f + fmax_(target[(size_t)r * MLP_OUT + c], 0.0f));
+      const float d = y[(size_t)r * MLP_OUTPAD + c] - t;
+      e2 += d * d;
+      gr = d * inv;
+    }
+    const uint16_t h = f32_to_bf16(gr);
+    dy[(size_t)r * MLP_OUTPAD + c] = h;
+    dyT[(size_t)c * ld + r] = h;
+  }
+  row_err[r] = e2;
+}
+
+// The gradients of every layer from the split partials: gw[l][i] = sum over
+// slices z (in order) of part_l[z][i], gb[l][m] = sum over z of bpart_l[z][m].
+struct GradSum {
+  const float* part[MLP_LAYERS];
+  const float* bpart[MLP_LAYERS];
+  float* gw[MLP_LAYERS];
+  float* gb[MLP_LAYERS];
+  uint32_t nw[MLP_LAYERS], nb[MLP_LAYERS];
+  uint32_t first[MLP_LAYERS + 1];  // first thread of layer l (weights then biases)
+  int slices;
+};
+
+__global__ void k_grad_sum(GradSum S) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= S.first[MLP_LAYERS]) return;
+  int l = 0;
+  while (l + 1 < MLP_LAYERS && t >= S.first[l + 1]) ++l;
+  const uint32_t i = t - S.first[l];
+  float s = 0.0f;
+  if (i < S.nw[l]) {
+    for (int z = 0; z < S.slices; ++z) s += S.part[l][(size_t)z * S.nw[l] + i];
+    S.gw[l][i] = s;
+  } else {
+    const uint32_t m = i - S.nw[l];
+    for (int z = 0; z < S.slices; ++z) s += S.bpart[l][(size_t)z * S.nb[l] + m];
+    S.gb[l][m] = s;
+  }
+}
+
+}  // namespace
+
+size_t mlp_train_partial_floats(uint32_t rows) {
+  const int slices = mlp_train_slices(rows);
+  size_t n = 0;
+  for (int l = 0; l < MLP_LAYERS; ++l) n += (mlp_w_count(l) + mlp_b_count(l)) * (size_t)slices;
+  return n;
+}
+
+int mlp_train_slices(uint32_t rows) { return (int)std::max<uint32_t>(1u, std::min<uint32_t>(32u, (rows + 255u) / 256u)); }
+
+uint32_t mlp_train_ld(uint32_t rows) { return (rows + 63u) / 64u * 64u; }
+
+hipError_t mlp_train_grads(const MlpDev& w, const float* in, uint32_t rows, uint16_t* X, uint16_t* const* H,
+                           float* y, const float* target, MlpTrainBufs& tb, MlpTrainT& tt, bool reduce,
+                           hipStream_t st) {
+  if (rows == 0) return hipSuccess;
+  const uint32_t ld = tt.ld;
+  if (ld < mlp_train_ld(rows)) return hipErrorInvalidValue;
+  hipError_t e;
+  {
+    const uint64_t th = (uint64_t)rows * MLP_IN;
+    hipLaunchKernelGGL(k_encode_t, dim3((unsigned)((th + 255) / 256)), dim3(256), 0, st, in, rows, X, tt.XT, ld);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  // forward: H_l and H_l^T, then y
+  const uint16_t* act = X;
+  int kin = MLP_KPAD;
+  for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) {
+    NtArgs a = nt_args((int)rows, MLP_HID, kin);
+    a.A = act; a.lda = kin;
+    a.B = w.wb[l]; a.ldb = kin;
+    a.bias = w.b[l];
+    a.relu = 1;
+    a.Cb = H[l]; a.ldc = MLP_HID;
+    a.Ct = tt.HT[l]; a.ldt = (int)ld;
+    if ((e = gemm_nt(a, 1, st)) != hipSuccess) return e;
+    act = H[l];
+    kin = MLP_HID;
+  }
+  {
+    NtArgs a = nt_args((int)rows, MLP_OUTPAD, MLP_HID);
+    a.A = act; a.lda = MLP_HID;
+    a.B = w.wb[MLP_HIDDEN_LAYERS]; a.ldb = MLP_HID;
+    a.bias = w.b[MLP_HIDDEN_LAYERS];
+    a.Cf = y; a.ldf = MLP_OUTPAD;
+    if ((e = gemm_nt(a, 1, st)) != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_loss_grad_t, dim3((rows + 255) / 256), dim3(256), 0, st, y, target, (int)rows, tb.dy, tt.dyT,
+                     ld, tb.row_err);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  // backward: per layer the split weight-gradient product, then the delta
+  // of the layer below (with its transpose)
+  const int slices = mlp_train_slices(rows);
+  const int kps = (int)(((rows + (uint32_t)slices - 1u) / (uint32_t)slices + 31u) / 32u * 32u);
+  GradSum S;
+  memset(&S, 0, sizeof(S));
+  S.slices = slices;
+  float* part = tt.part;
+  const uint16_t* delta = tb.dy;
+  const uint16_t* deltaT = tt.dyT;
+  int dcols = MLP_OUTPAD;
+  for (int l = MLP_HIDDEN_LAYERS; l >= 0; --l) {
+    const int nout = l == MLP_HIDDEN_LAYERS ? MLP_OUTPAD : MLP_HID;
+    const int kl = l == 0 ? MLP_KPAD : MLP_HID;
+    const uint16_t* actT = l == 0 ? tt.XT : tt.HT[l - 1];
+    NtArgs a = nt_args(nout, kl, (int)rows);
+    a.A = deltaT; a.lda = (int)ld;
+    a.B = actT; a.ldb = (int)ld;
+    a.kps = kps;
+    a.part = part;
+    a.bpart = part + (size_t)slices * nout * kl;
+    if ((e = gemm_nt(a, slices, st)) != hipSuccess) return e;
+    S.part[l] = a.part;
+    S.bpart[l] = a.bpart;
+    tt.part_w[l] = a.part;
+    tt.part_b[l] = a.bpart;
+    S.gw[l] = tb.gw[l];
+    S.gb[l] = tb.gb[l];
+    S.nw[l] = (uint32_t)(nout * kl);
+    S.nb[l] = (uint32_t)nout;
+    part += (size_t)slices * (nout * kl + nout);
+    if (l == 0) break;
+    uint16_t* nd = (l & 1) ? tb.d1 : tb.d0;
+    uint16_t* ndT = (l & 1) ? tt.dT[1] : tt.dT[0];
+    NtArgs b = nt_args((int)rows, MLP_HID, nout);
+    b.A = delta; b.lda = dcols;
+    b.B = w.wbt[l]; b.ldb = nout;  // W_l^T [in x out]
+    b.mask = H[l - 1]; b.ldm = MLP_HID;
+    b.Cb = nd; b.ldc = MLP_HID;
+    b.Ct = ndT; b.ldt = (int)ld;
+    if ((e = gemm_nt(b, 1, st)) != hipSuccess) return e;
+    delta = nd;
+    deltaT = ndT;
+    dcols = MLP_HID;
+  }
+  tt.slices = slices;
+  if (!reduce) return hipSuccess;
+  uint32_t tot = 0;
+  for (int l = 0; l < MLP_LAYERS; ++l) {
+    S.first[l] = tot;
+    tot += S.nw[l] + S.nb[l];
+  }
+  S.first[MLP_LAYERS] = tot;
+  hipLaunchKernelGGL(k_grad_sum, dim3((tot + 255) / 256), dim3(256), 0, st, S);
+  return hipGetLastError();
+}
+
+}  // namespace pm

@@ -38,6 +38,8 @@ PM_OPT_SAMPLE_LANES = 6
 PM_OPT_RENDER_MEDIA = 7
 PM_OPT_VOXEL_GRID = 8
 PM_OPT_TOP_LEVELS = 9
+PM_OPT_CACHE_NEAR_GROUPS = 10
+PM_OPT_CACHE_EMA = 11
 VOXEL_TABLE_ENTRIES = 3 * 33 ** 3  # summed-volume table of one map's 32^3 voxel grid
 
 

@@ -225,6 +225,15 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
  *                      when the scene has one), 0 = they gather at surface hits
  *                      only: the surface-gather measurement on the same photon
  *                      maps (the volume map is built but not read);
+ * PM_OPT_CACHE_NEAR_GROUPS the cache render's near-light radius, in merged
+ *                      march groups (0..16, default 2; a group is the march
+ *                      stride's worth of steps): inside it every march point
+ *                      is its own query and is gathered exactly;
+ * PM_OPT_CACHE_EMA     the cache's inference network: 0 = the trained weights,
+ *                      d = 1..999 = their exponential moving average with
+ *                      decay d / 1000 per Adam step (default 950; training
+ *                      itself is unchanged; setting the parameters or a reset
+ *                      restarts the average);
  * PM_OPT_TOP_LEVELS    gather mode 7 (also when auto picks it): the top 0..7
  *                      levels of both kd-trees (2^levels - 1 nodes, 48 B each)
  *                      staged in each wave's LDS for the traversals to read,
@@ -251,7 +260,9 @@ enum {
   PM_OPT_SAMPLE_LANES = 6,
   PM_OPT_RENDER_MEDIA = 7,
   PM_OPT_VOXEL_GRID = 8,
-  PM_OPT_TOP_LEVELS = 9
+  PM_OPT_TOP_LEVELS = 9,
+  PM_OPT_CACHE_NEAR_GROUPS = 10,
+  PM_OPT_CACHE_EMA = 11
 };
 int pm_set_option(pm_context* ctx, int option, int value);
 
