@@ -487,9 +487,7 @@ int render_crop_impl(pm_context* c, int x0, int y0, int w, int h, float* rgb_dev
   switch (r.gather_mode) {
     case 0: HIPCHK(launch_render(r, c->stream)); break;
     case 1: HIPCHK(launch_render_lane(r, c->stream)); break;
-    case 3: HIPCHK(launch_render_lane3(r, c->stream)); break;
     case 4: HIPCHK(launch_render_lane4(r, c->stream)); break;
-    case 6: HIPCHK(launch_render_lane6(r, c->stream)); break;
     case 7: HIPCHK(launch_render_lane7(r, c->stream)); break;
     default: HIPCHK(launch_render_lane2(r, c->stream)); break;
   }
@@ -564,7 +562,8 @@ int pm_set_option(pm_context* c, int option, int value) {
   if (!c) return PM_ERR_INVALID;
   switch (option) {
     case PM_OPT_GATHER_MODE:
-      if (value < 0 || value > 7) return PM_ERR_INVALID;
+      // 3 and 6 were the 8-point-batch and uncounted forms of modes 4 and 7
+      if (value < 0 || value > 7 || value == 3 || value == 6) return PM_ERR_INVALID;
       c->gather_mode = value;
       return PM_OK;
     case PM_OPT_BLOCKS_PER_CU:

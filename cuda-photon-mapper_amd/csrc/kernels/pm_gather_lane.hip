@@ -238,25 +238,8 @@ __global__ void __launch_bounds__(64) k_render_lane2(RenderLaunch r) {
   }
 }
 
-// Gather modes 3 and 4: the 2-byte heap (8 KB of LDS per wave), leaf points
-// loaded in batches of NB = 8 (mode 3) or 4 (mode 4).
-template <bool FP64T, bool CONE, int NB>
-__global__ void __launch_bounds__(64) k_render_lane3(RenderLaunch r) {
-  extern __shared__ uint16_t heap16_lds[];  // [64 slots][64 lanes]
-  const int lane = lane_id();
-  int lx, ly;
-  if (!lane_tile(r, lane, lx, ly)) return;
-  const uint32_t px = (uint32_t)(r.x0 + lx), py = (uint32_t)(r.y0 + ly);
-  uint32_t ns, nv;
-  V3f v = lane_pixel3<FP64T, CONE, NB>(r.sc, r.rp, r.gs, r.gv, heap16_lds, lane, px, py, ns, nv);
-  store_pixel(r.rgb, r.u8, (size_t)ly * (size_t)r.w + (size_t)lx, v, r.exposure);
-  if (r.counters) {
-    atomicAdd(&r.counters[0], (unsigned long long)ns);
-    atomicAdd(&r.counters[1], (unsigned long long)nv);
-  }
-}
-
-// Mode 4: batches of 4 with the register budget of 5 waves per SIMD.
+// Gather mode 4: the 2-byte heap (8 KB of LDS per wave), leaf points loaded 4
+// at a time, with the register budget of 5 waves per SIMD.
 template <bool FP64T, bool CONE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_render_lane4(RenderLaunch r) {
   extern __shared__ uint16_t heap16_lds[];  // [64 slots][64 lanes]
@@ -273,20 +256,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
   }
 }
 
-template <int NB>
-hipError_t launch_render_lane3_nb(const RenderLaunch& r, hipStream_t st) {
-  if (r.w <= 0 || r.h <= 0) return hipSuccess;
-  const unsigned tiles = lane_grid(r);
-  size_t lds = 64 * 64 * sizeof(uint16_t);
-  const bool fp64 = r.rp.fp64_transmittance != 0, cone = r.estimator == 1;
-  if (fp64 && cone) hipLaunchKernelGGL((k_render_lane3<true, true, NB>), dim3(tiles), dim3(64), lds, st, r);
-  else if (fp64) hipLaunchKernelGGL((k_render_lane3<true, false, NB>), dim3(tiles), dim3(64), lds, st, r);
-  else if (cone) hipLaunchKernelGGL((k_render_lane3<false, true, NB>), dim3(tiles), dim3(64), lds, st, r);
-  else hipLaunchKernelGGL((k_render_lane3<false, false, NB>), dim3(tiles), dim3(64), lds, st, r);
-  return hipGetLastError();
-}
-
-hipError_t launch_render_lane3(const RenderLaunch& r, hipStream_t st) { return launch_render_lane3_nb<8>(r, st); }
 hipError_t launch_render_lane4(const RenderLaunch& r, hipStream_t st) {
   if (r.w <= 0 || r.h <= 0) return hipSuccess;
   const unsigned tiles = lane_grid(r);
@@ -296,36 +265,6 @@ hipError_t launch_render_lane4(const RenderLaunch& r, hipStream_t st) {
   else if (fp64) hipLaunchKernelGGL((k_render_lane4<true, false>), dim3(tiles), dim3(64), lds, st, r);
   else if (cone) hipLaunchKernelGGL((k_render_lane4<false, true>), dim3(tiles), dim3(64), lds, st, r);
   else hipLaunchKernelGGL((k_render_lane4<false, false>), dim3(tiles), dim3(64), lds, st, r);
-  return hipGetLastError();
-}
-
-// Gather mode 6: the deferred-volume gather (2-byte heap, one shared
-// traversal per march step), 5 waves per SIMD.
-template <bool FP64T, bool CONE>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_render_lane6(RenderLaunch r) {
-  extern __shared__ uint16_t heap16_lds[];  // [64 slots][64 lanes]
-  const int lane = lane_id();
-  int lx, ly;
-  if (!lane_tile(r, lane, lx, ly)) return;
-  const uint32_t px = (uint32_t)(r.x0 + lx), py = (uint32_t)(r.y0 + ly);
-  uint32_t ns, nv;
-  V3f v = lane_pixel4<FP64T, CONE>(r.sc, r.rp, r.gs, r.gv, heap16_lds, lane, px, py, ns, nv);
-  store_pixel(r.rgb, r.u8, (size_t)ly * (size_t)r.w + (size_t)lx, v, r.exposure);
-  if (r.counters) {
-    atomicAdd(&r.counters[0], (unsigned long long)ns);
-    atomicAdd(&r.counters[1], (unsigned long long)nv);
-  }
-}
-
-hipError_t launch_render_lane6(const RenderLaunch& r, hipStream_t st) {
-  if (r.w <= 0 || r.h <= 0) return hipSuccess;
-  const unsigned tiles = lane_grid(r);
-  size_t lds = 64 * 64 * sizeof(uint16_t);
-  const bool fp64 = r.rp.fp64_transmittance != 0, cone = r.estimator == 1;
-  if (fp64 && cone) hipLaunchKernelGGL((k_render_lane6<true, true>), dim3(tiles), dim3(64), lds, st, r);
-  else if (fp64) hipLaunchKernelGGL((k_render_lane6<true, false>), dim3(tiles), dim3(64), lds, st, r);
-  else if (cone) hipLaunchKernelGGL((k_render_lane6<false, true>), dim3(tiles), dim3(64), lds, st, r);
-  else hipLaunchKernelGGL((k_render_lane6<false, false>), dim3(tiles), dim3(64), lds, st, r);
   return hipGetLastError();
 }
 
@@ -343,7 +282,9 @@ __device__ __forceinline__ void stage_lane7_tiles(GatherMap& gs, GatherMap& gv, 
   }
 }
 
-// Gather mode 7: mode 6 with the march's first passes done by counting in a
+// Gather mode 7: the 2-byte-key gather of mode 4 with each march step's
+// second pass merged into the next step's first-pass traversal (the deferred
+// volume query of pm_lane_knn4.h) and the first passes done by counting in a
 // bucket window (pm_lane_knn5.h), 5 waves per SIMD.  TOP: the traversals read
 // the top r.top_levels levels of both trees from LDS.
 template <bool FP64T, bool CONE, bool TOP>

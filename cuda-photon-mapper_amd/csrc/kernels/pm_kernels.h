@@ -180,9 +180,7 @@ struct RenderLaunch {
 hipError_t launch_render(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane2(const RenderLaunch& r, hipStream_t st);
-hipError_t launch_render_lane3(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane4(const RenderLaunch& r, hipStream_t st);
-hipError_t launch_render_lane6(const RenderLaunch& r, hipStream_t st);
 hipError_t launch_render_lane7(const RenderLaunch& r, hipStream_t st);
 
 // voxel-grid compatibility mode (pm_voxel.hip): both maps splatted into

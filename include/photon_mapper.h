@@ -188,19 +188,19 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
  *                          8-byte (d^2, index) heap keys in LDS,
  *                      2 = as 1 with 4-byte heap keys (d^2 only) and a second
  *                          fixed-radius pass for the exact neighbour set,
- *                      3 = as 2 with 2-byte keys (d^2 buckets 2^-7 wide) and
+ *                      4 = as 2 with 2-byte keys (d^2 buckets 2^-7 wide),
  *                          the exact order restored inside the k-th bucket,
- *                      4 = as 3, leaf points loaded 4 at a time under a
+ *                          leaf points loaded 4 at a time under a
  *                          5-waves-per-SIMD register budget,
  *                      5 = auto (default): 7 when the frame has at least 16
  *                          8x8 tiles per compute unit and the maps hold at
  *                          least 2^20 records, else 1;
- *                      6 = as 4, with each march step's second pass merged
- *                          into the next step's first-pass traversal;
- *                      7 = as 6, with the march's first passes done by
- *                          counting in a window of 256 buckets around the
- *                          previous step's k-th-neighbour bucket (heap pass
- *                          on a miss);
+ *                      7 = as 4, with each march step's second pass merged
+ *                          into the next step's first-pass traversal and the
+ *                          first passes done by counting in a window of 256
+ *                          buckets around the previous step's
+ *                          k-th-neighbour bucket (heap pass on a miss);
+ *                      (3 and 6 are not used: PM_ERR_INVALID);
  *                      every mode gives the same neighbour sets and images;
  * PM_OPT_BLOCKS_PER_CU workgroups per compute unit of the persistent mode-0 grid;
  * PM_OPT_CACHE_FUSED_INFER 3 = cache inference through one fused kernel, all

@@ -61,11 +61,11 @@ def test_all_fallback_equals_plain(scene):
 
 
 def test_value_kernels_agree(scene):
-    say("case: cache frames with the exact values of gather mode 3 (k_values) and mode 7 (k_values7)")
+    say("case: cache frames with the exact values of gather mode 4 (k_values) and mode 7 (k_values7)")
     g = scene
     imgs, stats = [], []
     try:
-        for mode in (3, 7):
+        for mode in (4, 7):
             g.set_option(pmamd.PM_OPT_GATHER_MODE, mode)
             g.cache_reset(1)
             g.cache_configure(threshold=0.05, train_samples=16384, batch=4096, march_stride=4)
