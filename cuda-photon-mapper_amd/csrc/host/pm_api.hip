@@ -759,7 +759,12 @@ void launch_init_random_numbers_kernel(void) {
 }
 
 pm_context* pm_legacy_context(void) {
-  if (!g_legacy) launch_init_random_numbers_kernel();
+  if (!g_legacy) {
+    // as launch_init_random_numbers_kernel, but a failure returns NULL instead of exiting
+    pm_config cfg;
+    preset_config(1, &cfg);
+    if (pm_create(&cfg, &g_legacy) != PM_OK) g_legacy = nullptr;
+  }
   return g_legacy;
 }
 

@@ -67,8 +67,8 @@ void launch_photon_mapping_kernel(pm_uchar4* pos, unsigned int image_width, unsi
  * pm_set_option(pm_legacy_context(), PM_OPT_VOXEL_GRID, 1) for the
  * reference's own 32^3 voxel-grid estimate; with that option interpolateFlag
  * selects the bilinear window blend (photonMappingKernel.cu:591) instead of
- * the cone filter.  No reference counterpart: the reference chooses its
- * estimate at compile time. */
+ * the cone filter.  NULL when the context cannot be created.  No reference
+ * counterpart: the reference chooses its estimate at compile time. */
 struct pm_context* pm_legacy_context(void);
 
 /* ------------------------------------------------------------------ pm_* API

@@ -55,6 +55,16 @@ def test_presets_and_validation_without_gpu():
     assert lib.pm_status_string(pmamd.PM_ERR_STATE) == b"call order violated"
 
 
+def test_options_and_accessors_reject_bad_arguments_without_gpu():
+    lib = pmamd.load()
+    # no context: every option call is refused before any GPU work
+    for opt in (pmamd.PM_OPT_GATHER_MODE, pmamd.PM_OPT_VOXEL_GRID, pmamd.PM_OPT_TOP_LEVELS,
+                pmamd.PM_OPT_CACHE_NEAR_GROUPS, pmamd.PM_OPT_CACHE_EMA):
+        assert lib.pm_set_option(None, opt, 0) == pmamd.PM_ERR_INVALID
+    t = (ctypes.c_uint64 * 8)()
+    assert lib.pm_get_voxel_table(None, 0, t, 8) == pmamd.PM_ERR_INVALID
+
+
 def test_struct_layout_matches_c():
     # sizeof checks: the ctypes mirror must match the C layout of the header.
     assert ctypes.sizeof(pmamd.PmSphere) == 36

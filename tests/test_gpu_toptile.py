@@ -48,3 +48,18 @@ def test_top_tiles_bit_identical(variant):
     with pytest.raises(pmamd.PmError):
         g.set_option(pmamd.PM_OPT_TOP_LEVELS, 8)
     g.close()
+
+
+def test_option_ranges():
+    """The options added beside the gather modes take only their documented values."""
+    c = pmamd.preset(1)
+    c.n_emit = 1000
+    g = pmamd.PhotonMapper(c)
+    for opt, bad in ((pmamd.PM_OPT_TOP_LEVELS, -1), (pmamd.PM_OPT_VOXEL_GRID, 3), (pmamd.PM_OPT_CACHE_NEAR_GROUPS, 17),
+                     (pmamd.PM_OPT_CACHE_EMA, 1000), (pmamd.PM_OPT_CACHE_EMA, -1), (99, 0)):
+        with pytest.raises(pmamd.PmError):
+            g.set_option(opt, bad)
+    for opt, ok in ((pmamd.PM_OPT_TOP_LEVELS, 7), (pmamd.PM_OPT_VOXEL_GRID, 2), (pmamd.PM_OPT_CACHE_NEAR_GROUPS, 0),
+                    (pmamd.PM_OPT_CACHE_EMA, 0), (pmamd.PM_OPT_CACHE_EMA, 999)):
+        g.set_option(opt, ok)
+    g.close()
