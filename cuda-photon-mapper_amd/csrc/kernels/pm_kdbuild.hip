@@ -16,6 +16,13 @@
 //     needed.
 //  3. After the last level the x-list gives the leaf order (x key, index);
 //     positions are gathered into float4 (x, y, z, index) in tree order.
+// The points are first renumbered in Morton order (a 30-bit Morton code per
+// point, radix-sorted; within each tree of a forest), and the build works on
+// the renumbered ids and a copy of the coordinates in that order: the points
+// of a node are then close in memory as well as in space, so the per-level
+// coordinate gathers of step 2 hit cached lines.  The photon index stays the
+// tie-breaker of every order (the lists are sorted by (key, photon index)
+// before being renumbered), so the tree is the same.
 #include <string.h>
 
 #include <algorithm>
@@ -32,6 +39,50 @@ static constexpr int KB_TILE = KB_THREADS * KB_ITEMS;
 
 __device__ __forceinline__ float coord_of(const float* x, const float* y, const float* z, uint32_t i, int a) {
   return a == 0 ? x[i] : (a == 1 ? y[i] : z[i]);
+}
+
+// 10 bits spread to every third bit of 30 (bit b -> bit 3b).
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t quant10(float c) {
+  c = fminf(fmaxf(c, 0.0f), 1.0f);  // the room is the unit cube; NaN -> 0
+  const uint32_t q = (uint32_t)(c * 1024.0f);
+  return q > 1023u ? 1023u : q;
+}
+
+// Morton code of each point over the unit cube, with its index.
+__global__ void k_morton_keys(const float* __restrict__ x, const float* __restrict__ y, const float* __restrict__ z,
+                              uint32_t n, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = spread3(quant10(x[i])) | (spread3(quant10(y[i])) << 1) | (spread3(quant10(z[i])) << 2);
+  vals[i] = i;
+}
+
+// The renumbering: inv[orig[j]] = j and the coordinates in the new order.
+__global__ void k_renumber(const uint32_t* __restrict__ orig, uint32_t n, const float* __restrict__ x,
+                           const float* __restrict__ y, const float* __restrict__ z, uint32_t* __restrict__ inv,
+                           float* __restrict__ xr, float* __restrict__ yr, float* __restrict__ zr) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t i = orig[j];
+  inv[i] = j;
+  xr[j] = x[i];
+  yr[j] = y[i];
+  zr[j] = z[i];
+}
+
+// list[p] = inv[list[p]]: a list of photon indices as renumbered ids.
+__global__ void k_map_ids(uint32_t* __restrict__ list, uint32_t n, const uint32_t* __restrict__ inv) {
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) list[p] = inv[list[p]];
 }
 
 __global__ void k_make_keys(const float* __restrict__ c, uint32_t n, uint32_t* __restrict__ keys,
@@ -94,6 +145,7 @@ struct LevelArgs {
   uint32_t nblocks;
   uint32_t* blockcnt;  // [3][nblocks]
   uint8_t* flags;      // [n] per position: bit b = flag of list b, bit 7 = in a node of this level
+  const uint32_t* orig;  // photon index of each renumbered id (the tie-breaker)
 };
 
 // Flag of list b at position p: does the element order before the node's
@@ -106,7 +158,7 @@ __device__ __forceinline__ bool part_flag(const LevelArgs& A, int b, uint32_t p,
   uint32_t m = A.node_med[h];
   uint32_t ki = float_key(coord_of(A.F.x, A.F.y, A.F.z, i, (int)a));
   uint32_t km = float_key(coord_of(A.F.x, A.F.y, A.F.z, m, (int)a));
-  return ki < km || (ki == km && i < m);
+  return ki < km || (ki == km && A.orig[i] < A.orig[m]);
 }
 
 // Counts the flags per block and keeps them (A.flags) for k_level_scatter:
@@ -226,13 +278,14 @@ __global__ void k_tree_keys(const uint32_t* __restrict__ point_tree, const uint3
   if (i < n) keys[i] = point_tree[vals[i]];
 }
 
-// Leaf order: (x, y, z, bits(tree-local point index)).
-__global__ void k_tree_points(KdForest F, const uint32_t* __restrict__ l0) {
+// Leaf order: (x, y, z, bits(tree-local photon index)); F's coordinates are
+// in renumbered order, orig maps an id to its photon index.
+__global__ void k_tree_points(KdForest F, const uint32_t* __restrict__ l0, const uint32_t* __restrict__ orig) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= F.n) return;
-  uint32_t i = l0[p];
+  uint32_t j = l0[p];
   uint32_t pb = F.point_tree ? F.tree_pbase[F.point_tree[p]] : 0u;
-  F.tpts[p] = make_float4(F.x[i], F.y[i], F.z[i], u2f(i - pb));
+  F.tpts[p] = make_float4(F.x[j], F.y[j], F.z[j], u2f(orig[j] - pb));
 }
 
 size_t kdforest_temp_words(uint32_t n, uint32_t nnodes) {
@@ -246,6 +299,7 @@ size_t kdforest_temp_words(uint32_t n, uint32_t nnodes) {
   w += radix_temp_words(n);
   w += 2 * (size_t)nnodes + 8;  // node_med, node_axis
   w += ((size_t)n + 3) / 4;     // per-position flag bytes of a level
+  w += 5 * (size_t)n;           // renumbering: orig, inv, coordinates in the new order
   return w + 64;
 }
 
@@ -270,10 +324,24 @@ hipError_t kd_build_forest(const KdForest& F, const uint32_t* level_off, uint32_
   uint32_t* node_med = p; p += F.nnodes + 1;
   uint32_t* node_axis = p; p += F.nnodes + 1;
   uint8_t* pflags = (uint8_t*)p; p += ((size_t)n + 3) / 4;
+  uint32_t* orig = p; p += n;
+  uint32_t* inv = p; p += n;
+  float* xr = (float*)p; p += n;
+  float* yr = (float*)p; p += n;
+  float* zr = (float*)p; p += n;
 
   hipError_t e;
   const float* C[3] = {F.x, F.y, F.z};
   unsigned gb = (n + 255) / 256;
+  // renumbering in Morton order (within each tree: a stable pass on the tree id)
+  hipLaunchKernelGGL(k_morton_keys, dim3(gb), dim3(256), 0, st, F.x, F.y, F.z, n, keys, orig);
+  if ((e = radix_sort_pairs(keys, orig, n, sorttemp, st)) != hipSuccess) return e;
+  if (F.ntrees > 1) {
+    hipLaunchKernelGGL(k_tree_keys, dim3(gb), dim3(256), 0, st, F.point_tree, orig, n, tkeys);
+    if ((e = radix_pass(tkeys, orig, n, 0, sorttemp, keys, inv, st)) != hipSuccess) return e;
+    uint32_t* t = orig; orig = inv; inv = t;
+  }
+  hipLaunchKernelGGL(k_renumber, dim3(gb), dim3(256), 0, st, (const uint32_t*)orig, n, F.x, F.y, F.z, inv, xr, yr, zr);
   for (int a = 0; a < 3; ++a) {
     hipLaunchKernelGGL(k_make_keys, dim3(gb), dim3(256), 0, st, C[a], n, keys, lists[a]);
     if ((e = radix_sort_pairs(keys, lists[a], n, sorttemp, st)) != hipSuccess) return e;
@@ -283,15 +351,20 @@ hipError_t kd_build_forest(const KdForest& F, const uint32_t* level_off, uint32_
       if ((e = radix_pass(tkeys, lists[a], n, 0, sorttemp, keys, lists2[a], st)) != hipSuccess) return e;
       uint32_t* t = lists[a]; lists[a] = lists2[a]; lists2[a] = t;
     }
+    hipLaunchKernelGGL(k_map_ids, dim3(gb), dim3(256), 0, st, lists[a], n, (const uint32_t*)inv);
   }
-  hipLaunchKernelGGL(k_seg_init, dim3(gb), dim3(256), 0, st, F, seg);
+  // from here on the build reads the coordinates in renumbered order
+  KdForest R = F;
+  R.x = xr; R.y = yr; R.z = zr;
+  hipLaunchKernelGGL(k_seg_init, dim3(gb), dim3(256), 0, st, R, seg);
   for (uint32_t level = 0; level < nlevels; ++level) {
     const uint32_t first = level_off[level], count = level_off[level + 1] - level_off[level];
     if (count == 0) continue;
-    hipLaunchKernelGGL(k_level_nodes, dim3((count + 255) / 256), dim3(256), 0, st, F, lists[0], lists[1], lists[2],
+    hipLaunchKernelGGL(k_level_nodes, dim3((count + 255) / 256), dim3(256), 0, st, R, lists[0], lists[1], lists[2],
                        first, count, node_med, node_axis);
     LevelArgs A;
-    A.F = F;
+    A.F = R;
+    A.orig = orig;
     for (int b = 0; b < 3; ++b) { A.lin[b] = lists[b]; A.lout[b] = lists2[b]; }
     A.seg = seg; A.seg_out = seg2;
     A.node_med = node_med; A.node_axis = node_axis;
@@ -303,7 +376,7 @@ hipError_t kd_build_forest(const KdForest& F, const uint32_t* level_off, uint32_
     for (int b = 0; b < 3; ++b) { uint32_t* t = lists[b]; lists[b] = lists2[b]; lists2[b] = t; }
     uint32_t* t = seg; seg = seg2; seg2 = t;
   }
-  hipLaunchKernelGGL(k_tree_points, dim3(gb), dim3(256), 0, st, F, (const uint32_t*)lists[0]);
+  hipLaunchKernelGGL(k_tree_points, dim3(gb), dim3(256), 0, st, R, (const uint32_t*)lists[0], (const uint32_t*)orig);
   return hipGetLastError();
 }
 

@@ -89,6 +89,32 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const uint32_t* __r
   }
 }
 
+// One workgroup for a short array: tiles of SCAN_TILE in order with a
+// running carry (the kd-build's per-level block counts, a few thousand).
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_small(const uint32_t* __restrict__ in, size_t n,
+                                                              uint32_t* __restrict__ out,
+                                                              uint32_t* __restrict__ total_out) {
+  __shared__ uint32_t lw[SCAN_THREADS / 64];
+  uint32_t carry = 0;
+  for (size_t tile = 0; tile < n; tile += SCAN_TILE) {
+    const size_t base = tile + (size_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[SCAN_ITEMS];
+    uint32_t s = 0;
+    for (int r = 0; r < SCAN_ITEMS; ++r) {
+      v[r] = base + r < n ? in[base + r] : 0u;
+      s += v[r];
+    }
+    uint32_t tot;
+    uint32_t ex = carry + block_exclusive_scan(s, lw, &tot);
+    for (int r = 0; r < SCAN_ITEMS; ++r) {
+      if (base + r < n) out[base + r] = ex;
+      ex += v[r];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
 size_t scan_temp_words(size_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE + 1; }
 
 hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t* temp, uint32_t* total_dev,
@@ -98,6 +124,10 @@ hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t*
     return hipSuccess;
   }
   size_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (nb <= 16) {  // one launch instead of three
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(SCAN_THREADS), 0, st, in, n, out, total_dev);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, in, n, temp);
   hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(SCAN_THREADS), 0, st, temp, nb, total_dev);
   hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, in, n, temp, out);
