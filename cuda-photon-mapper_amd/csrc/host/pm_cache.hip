@@ -142,7 +142,7 @@ namespace pm {
 void cache_release(CacheState& k) {
   for (DevBuf* b : {&k.params, &k.grads, &k.adam_m, &k.adam_v, &k.params_bf16, &k.params_ema, &k.ema_bf16, &k.X, &k.y, &k.dy, &k.d0, &k.d1,
                     &k.row_err, &k.rgb, &k.err_sum, &k.err_cnt, &k.rec_cnt, &k.rec_off, &k.rec_in,
-                    &k.rec_wt, &k.rec_pix, &k.rec_val, &k.sel_in, &k.sel_t, &k.flags, &k.flags2, &k.flag_off, &k.list,
+                    &k.rec_wt, &k.rec_code, &k.rec_val, &k.sel_in, &k.sel_t, &k.list, &k.list_cnt, &k.cell_fb,
                     &k.XT, &k.dT[0], &k.dT[1], &k.dyT, &k.tpart})
     b->release();
   for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) {
@@ -152,8 +152,38 @@ void cache_release(CacheState& k) {
   k.ready = false;
 }
 
-// Encode + forward + decode for n device input rows, in chunks.
-int cache_predict_dev(pm_context* c, const float* in, uint32_t n, float* rgb) {
+// The two record lists of `rule` over the frame's N record codes, each in
+// record order: list 0 at list[0, n[0]), list 1 at list[n[0], n[0] + n[1]).
+// One count pass, a scan of the per-block counts, one scatter pass.
+int build_lists(pm_context* c, uint64_t N, const CacheListRule& rule, uint32_t* list, uint32_t n[2]) {
+  CacheState& k = c->cache;
+  hipStream_t st = c->stream;
+  n[0] = n[1] = 0u;
+  if (N == 0) return PM_OK;
+  const size_t nb = cache_list_blocks(N);
+  HIPCHK(k.list_cnt.ensure(4ull * 4 * nb));
+  HIPCHK(c->scratch.ensure(4ull * scan_temp_words(2 * nb)));
+  HIPCHK(c->small.ensure(64));
+  uint32_t* counts = k.list_cnt.as<uint32_t>();
+  uint32_t* offs = counts + 2 * nb;
+  uint32_t* total = c->small.as<uint32_t>() + 12;
+  HIPCHK(launch_cache_list_count(k.rec_code.as<uint16_t>(), N, rule, counts, st));
+  HIPCHK(exclusive_scan(counts, offs, 2 * nb, c->scratch.as<uint32_t>(), total, st));
+  HIPCHK(launch_cache_list_scatter(k.rec_code.as<uint16_t>(), N, rule, offs, list, st));
+  uint32_t h[2] = {0u, 0u};  // list 0's total (the offset of list 1's first block), both totals
+  HIPCHK(hipMemcpyAsync(&h[0], offs + nb, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&h[1], total, 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  n[0] = h[0];
+  n[1] = h[1] - h[0];
+  return PM_OK;
+}
+
+// Encode + forward + decode for n device input rows, in chunks.  With `list`
+// (nlist record indices) the fused kernel infers only the listed records;
+// the other forms infer all n.
+int cache_predict_dev(pm_context* c, const float* in, uint32_t n, float* rgb, const uint32_t* list = nullptr,
+                      uint32_t nlist = 0) {
   int r = ensure_cache(c);
   if (r != PM_OK) return r;
   CacheState& k = c->cache;
@@ -163,7 +193,7 @@ int cache_predict_dev(pm_context* c, const float* in, uint32_t n, float* rgb) {
     int maxlds = 0;
     HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
     if (k.fused_infer == 3 && (size_t)maxlds >= mlp_fused_lds_bytes()) {
-      HIPCHK(mlp_infer_fused(wi, in, n, rgb, c->stream));
+      HIPCHK(mlp_infer_fused(wi, in, list ? nlist : n, rgb, c->stream, list));
       return PM_OK;
     }
     if ((size_t)maxlds >= mlp_infer_lds_bytes()) {
@@ -371,13 +401,13 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   if (N >= 0xffffffffull) return PM_ERR_CAPACITY;
   HIPCHK(k.rec_in.ensure(4ull * MLP_IN * std::max<uint64_t>(N, 1)));
   HIPCHK(k.rec_wt.ensure(12ull * std::max<uint64_t>(N, 1)));
-  HIPCHK(k.rec_pix.ensure(4ull * std::max<uint64_t>(N, 1)));
+  HIPCHK(k.rec_code.ensure(2ull * std::max<uint64_t>(N, 1)));
   HIPCHK(k.rec_val.ensure(12ull * std::max<uint64_t>(N, 1)));
   rl.write = 1;
   rl.offsets = k.rec_off.as<uint32_t>();
   rl.in = k.rec_in.as<float>();
   rl.wt = k.rec_wt.as<float>();
-  rl.pix = k.rec_pix.as<uint32_t>();
+  rl.code = k.rec_code.as<uint16_t>();
   HIPCHK(launch_cache_record(rl, st));
   HIPCHK(hipEventRecord(ev[1], st));
   // 4: online training on a subset with exact kNN targets
@@ -392,26 +422,26 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   HIPCHK(hipMemsetAsync(k.err_cnt.p, 0, 4ull * ncell, st));
   double loss = 0.0;
   const uint32_t S = N > 0 ? k.train_samples : 0u;
+  // the volume records, when surface records are never cached: inference
+  // skips the surface records, whose values the exact fallback writes
+  const uint32_t* infer_list = nullptr;
+  uint32_t n_infer = 0;
   if (S > 0) {
     HIPCHK(k.sel_in.ensure(4ull * MLP_IN * S));
     HIPCHK(k.sel_t.ensure(12ull * S));
     // index lists of surface and volume records (in record order)
-    HIPCHK(k.flags.ensure(4ull * std::max<uint64_t>(N, 1)));
-    HIPCHK(k.flag_off.ensure(4ull * std::max<uint64_t>(N, 1)));
     HIPCHK(k.list.ensure(8ull * std::max<uint64_t>(N, 1)));
-    HIPCHK(c->scratch.ensure(4ull * scan_temp_words(std::max<uint64_t>(N, npix))));
-    HIPCHK(c->small.ensure(64));
-    uint32_t* list_s = k.list.as<uint32_t>();
-    uint32_t* list_v = list_s + N;
-    // one flag pass: the volume records are the complement
-    HIPCHK(launch_kind_flags(k.rec_in.as<float>(), N, 1, k.flags.as<uint32_t>(), st));
-    HIPCHK(exclusive_scan(k.flags.as<uint32_t>(), k.flag_off.as<uint32_t>(), N, c->scratch.as<uint32_t>(),
-                          c->small.as<uint32_t>() + 10, st));
-    HIPCHK(launch_cache_split(k.flags.as<uint32_t>(), k.flag_off.as<uint32_t>(), N, list_s, list_v, st));
+    CacheListRule kinds;
+    memset(&kinds, 0, sizeof(kinds));
     uint32_t nk[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(nk, c->small.as<uint32_t>() + 10, 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-    nk[1] = (uint32_t)N - nk[0];
+    int rl_err = build_lists(c, N, kinds, k.list.as<uint32_t>(), nk);
+    if (rl_err != PM_OK) return rl_err;
+    const uint32_t* list_s = k.list.as<uint32_t>();
+    const uint32_t* list_v = list_s + nk[0];
+    if (!k.cache_surface) {
+      infer_list = list_v;
+      n_infer = nk[1];
+    }
     // without surface caching the subset is drawn from the volume records only
     const uint32_t ns_sel = k.cache_surface ? nk[0] : 0u;
     HIPCHK(launch_cache_select(k.rec_in.as<float>(), list_s, ns_sel, list_v, nk[1], S, c->cfg.seed, k.frame,
@@ -429,37 +459,24 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
     if (r != PM_OK) return r;
   }
   HIPCHK(hipEventRecord(ev[2], st));
-  // 5: inference over every record
-  int r = cache_predict_dev(c, k.rec_in.as<float>(), (uint32_t)N, k.rec_val.as<float>());
+  // 5: inference over the records the cache may answer
+  int r = cache_predict_dev(c, k.rec_in.as<float>(), (uint32_t)N, k.rec_val.as<float>(), infer_list, n_infer);
   if (r != PM_OK) return r;
   HIPCHK(hipEventRecord(ev[3], st));
-  // 6: exact fallback where the cache's cell error is high
-  HIPCHK(k.flags.ensure(4ull * std::max<uint64_t>(N, 1)));
-  HIPCHK(k.flag_off.ensure(4ull * std::max<uint64_t>(N, 1)));
+  // 6: exact fallback where the cache's cell error is high.  The list holds
+  // the surface records, then the volume records, each in record order: a
+  // wave then runs one kind of gather
   HIPCHK(k.list.ensure(8ull * std::max<uint64_t>(N, 1)));
-  HIPCHK(c->scratch.ensure(4ull * scan_temp_words(std::max<uint64_t>(N, npix))));
-  CacheFlagParams fp;
-  fp.threshold = k.threshold;
-  fp.cache_surface = k.cache_surface;
-  for (int j = 0; j < 3; ++j) fp.light[j] = c->scene.light[j];
-  fp.near2 = rl.near2;
-  HIPCHK(c->small.ensure(64));
-  // the list holds the surface records, then the volume records, each in
-  // record order: a wave then runs one kind of gather
-  uint32_t* ntot = c->small.as<uint32_t>() + 8;
-  HIPCHK(k.flags2.ensure(4ull * std::max<uint64_t>(N, 1)));
-  fp.kind = 3;  // surface fallbacks in flags, volume ones in flags2, from one pass over the records
-  HIPCHK(launch_cache_flags(k.rec_in.as<float>(), N, k.err_sum.as<unsigned long long>(), k.err_cnt.as<unsigned int>(),
-                            fp, k.flags.as<uint32_t>(), st, k.flags2.as<uint32_t>()));
-  for (int kind = 1; kind <= 2; ++kind) {
-    const uint32_t* fl = kind == 1 ? k.flags.as<uint32_t>() : k.flags2.as<uint32_t>();
-    HIPCHK(exclusive_scan(fl, k.flag_off.as<uint32_t>(), N, c->scratch.as<uint32_t>(), ntot + kind - 1, st));
-    HIPCHK(launch_cache_compact(fl, k.flag_off.as<uint32_t>(), N, k.list.as<uint32_t>(), kind == 2 ? ntot : nullptr,
-                                st));
-  }
+  HIPCHK(k.cell_fb.ensure(CACHE_CELLS));
+  HIPCHK(launch_cache_cell_fb(k.err_sum.as<unsigned long long>(), k.err_cnt.as<unsigned int>(), k.threshold,
+                              k.cell_fb.as<uint8_t>(), st));
+  CacheListRule fbr;
+  fbr.fallback = 1;
+  fbr.cache_surface = k.cache_surface;
+  fbr.cell_fb = k.cell_fb.as<uint8_t>();
   uint32_t nk2[2] = {0, 0};
-  HIPCHK(hipMemcpyAsync(nk2, ntot, 8, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  r = build_lists(c, N, fbr, k.list.as<uint32_t>(), nk2);
+  if (r != PM_OK) return r;
   const uint32_t nfb = nk2[0] + nk2[1];
   for (int kind = 1; kind <= 2; ++kind) {  // one launch per kind of row
     if (nk2[kind - 1] == 0) continue;

@@ -74,7 +74,7 @@ struct CacheState {
   int near_groups = 2;     // near-light radius in merged march groups
   float ema_decay = 0.95f;  // inference on the moving average of the weights (decay per Adam step), 0: off
   bool ema_stale = true;    // the average must restart at the current weights before its next use
-  DevBuf rec_cnt, rec_off, rec_in, rec_wt, rec_pix, rec_val, sel_in, sel_t, flags, flags2, flag_off, list;
+  DevBuf rec_cnt, rec_off, rec_in, rec_wt, rec_code, rec_val, sel_in, sel_t, list, list_cnt, cell_fb;
   pm_cache_stats st;
 };
 

@@ -4,18 +4,21 @@
 // but every radiance query is first recorded instead of gathered:
 //   1. k_rec (count)  : per pixel, the number of queries of its samples
 //   2. scan           : per-pixel record offsets (records stay in pixel order)
-//   3. k_rec (write)  : query inputs (position, direction, normal or 0) and
+//   3. k_rec (write)  : query inputs (position, direction, normal or 0),
 //                       path weights (throughput x transmittance x step, with
-//                       the surface albedo folded in)
+//                       the surface albedo folded in) and record codes (error
+//                       cell, kind, near-light bit: record_code)
 //   4. training       : a Philox-chosen subset of the records gets exact kNN
 //                       targets (k_values7 under gather modes 7 and auto,
 //                       k_values otherwise) and trains the MLP (pm_mlp.hip);
 //                       each batch's pre-update errors are binned into a
 //                       16^3 x {surface, volume} grid (k_err_accum)
-//   5. inference      : MLP over all records (pm_mlp.hip)
+//   5. inference      : MLP over the volume records (all records when surface
+//                       records are cached too; pm_mlp_infer.hip)
 //   6. fallback       : records in cells whose mean training error exceeds
 //                       the threshold (or that saw no training sample) are
-//                       re-gathered exactly with kNN (k_flags, then
+//                       re-gathered exactly with kNN (k_cell_fb, the lists of
+//                       record_class by k_list_count / k_list_scatter, then
 //                       k_values7 / k_values over the surface rows and the
 //                       volume rows)
 //   7. k_accumulate   : per pixel, compensated sum of weight x value / spp.
@@ -25,6 +28,31 @@
 #include "pm_mlp.h"
 
 namespace pm {
+
+// The error-tracker cell of a record at (x, y, z): its 16^3 cell (clamped to
+// the grid), plus 4096 for volume records.
+__device__ __forceinline__ uint32_t err_cell_of(float x, float y, float z, bool surf) {
+  const float p[3] = {x, y, z};
+  int c[3];
+  for (int a = 0; a < 3; ++a) {
+    float v = p[a] * (float)CACHE_GRID;
+    int iv = (int)v;
+    c[a] = iv < 0 ? 0 : (iv >= CACHE_GRID ? CACHE_GRID - 1 : iv);
+  }
+  return (uint32_t)((((surf ? 0 : 1) * CACHE_GRID + c[2]) * CACHE_GRID + c[1]) * CACHE_GRID + c[0]);
+}
+
+// A record's code, everything the fallback rule reads of it: its error cell
+// (bits 0-12; bit 12 set for volume records) and, in bit 13, whether a volume
+// record lies within the near-light radius.
+constexpr uint32_t kCodeVolume = (uint32_t)(CACHE_CELLS / 2), kCodeNear = (uint32_t)CACHE_CELLS;
+static_assert(CACHE_CELLS == 8192, "record codes hold 13 cell bits and the near-light bit");
+__device__ __forceinline__ uint16_t record_code(V3f x, V3f nrm, V3f light, float near2) {
+  const bool surf = !(nrm.x == 0.0f && nrm.y == 0.0f && nrm.z == 0.0f);
+  const float lx = x.x - light.x, ly = x.y - light.y, lz = x.z - light.z;
+  const bool near = !surf && (lx * lx + ly * ly) + lz * lz < near2;
+  return (uint16_t)(err_cell_of(x.x, x.y, x.z, surf) | (near ? kCodeNear : 0u));
+}
 
 // Recording gather: stores each query and its weight, returns zero radiance.
 // With march stride s > 1, one volume record stands for up to s consecutive
@@ -36,7 +64,7 @@ namespace pm {
 // new record.  Within the near-light radius (4 s steps) of the point light,
 // where the in-scattered radiance grows as 1 / distance^2 and no low-order
 // rule over s steps holds, every march point keeps its own record (and is
-// gathered exactly, see k_flags).  The open volume record stays in registers
+// gathered exactly, see record_class).  The open volume record stays in registers
 // and is written once, when the next record starts or the pixel ends (flush).
 struct RecordGather {
   bool write;
@@ -45,7 +73,7 @@ struct RecordGather {
   uint32_t pixel;    // crop pixel index
   float* in;
   float* wt;
-  uint32_t* pix;
+  uint16_t* code;
   int stride;
   float reach2;      // (1.5 * stride * step)^2
   bool pending;      // the open volume record is not written yet
@@ -65,7 +93,7 @@ struct RecordGather {
     p[3] = d.x; p[4] = d.y; p[5] = d.z;
     p[6] = nrm.x; p[7] = nrm.y; p[8] = nrm.z;
     wt[3 * i] = w.x; wt[3 * i + 1] = w.y; wt[3 * i + 2] = w.z;
-    pix[i] = pixel;
+    code[i] = record_code(x, nrm, light, near2);
   }
   // Writes the open volume record: at the weighted centroid of its points
   // (the first point while there is one, or no weight).
@@ -135,7 +163,7 @@ __global__ void __launch_bounds__(64) k_rec(CacheRecordLaunch r) {
   g.pixel = pidx;
   g.in = r.in;
   g.wt = r.wt;
-  g.pix = r.pix;
+  g.code = r.code;
   g.stride = r.stride > 0 ? r.stride : 1;
   const float reach = 1.5f * (float)g.stride * r.rp.march_step;
   g.reach2 = reach * reach;
@@ -244,24 +272,8 @@ __global__ void k_select(const float* __restrict__ in, const uint32_t* __restric
   for (int c = 0; c < MLP_IN; ++c) out[(size_t)j * MLP_IN + c] = in[idx * MLP_IN + c];
 }
 
-// flags[i] = 1 for surface records (non-zero normal).
-__global__ void k_kind_flags(const float* __restrict__ in, uint64_t n, int want_surface, uint32_t* __restrict__ flags) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float* p = in + i * MLP_IN;
-  bool surf = !(p[6] == 0.0f && p[7] == 0.0f && p[8] == 0.0f);
-  flags[i] = (surf == (want_surface != 0)) ? 1u : 0u;
-}
-
 __device__ __forceinline__ uint32_t err_cell(const float* p) {
-  int c[3];
-  for (int a = 0; a < 3; ++a) {
-    float v = p[a] * (float)CACHE_GRID;
-    int iv = (int)v;
-    c[a] = iv < 0 ? 0 : (iv >= CACHE_GRID ? CACHE_GRID - 1 : iv);
-  }
-  const bool surf = !(p[6] == 0.0f && p[7] == 0.0f && p[8] == 0.0f);
-  return (uint32_t)((((surf ? 0 : 1) * CACHE_GRID + c[2]) * CACHE_GRID + c[1]) * CACHE_GRID + c[0]);
+  return err_cell_of(p[0], p[1], p[2], !(p[6] == 0.0f && p[7] == 0.0f && p[8] == 0.0f));
 }
 
 // Bins per-row squared errors; integer (fixed-point 2^-24) sums are order-independent.
@@ -276,31 +288,115 @@ __global__ void k_err_accum(const float* __restrict__ in, const float* __restric
   atomicAdd(&ecnt[cell], 1u);
 }
 
-// Fallback decision per record: surface queries (unless cached), volume
-// queries within the near-light radius (recorded one per march point, where
-// the radiance is steepest), and records in cells whose training error is
-// above the threshold or that saw no training row.
-__global__ void k_flags(const float* __restrict__ in, uint64_t n, const unsigned long long* __restrict__ esum,
-                        const unsigned int* __restrict__ ecnt, CacheFlagParams fp, uint32_t* __restrict__ flags,
-                        uint32_t* __restrict__ flags2) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float* p = in + i * MLP_IN;
-  const bool surf = !(p[6] == 0.0f && p[7] == 0.0f && p[8] == 0.0f);
-  const float lx = p[0] - fp.light[0], ly = p[1] - fp.light[1], lz = p[2] - fp.light[2];
-  const bool near = !surf && (lx * lx + ly * ly) + lz * lz < fp.near2;
-  uint32_t cell = err_cell(p);
-  uint32_t c = ecnt[cell];
-  // mean over the cell's training rows of the per-channel squared log error
+// Fallback decision per error cell: no training row in the cell, or a mean
+// (over its training rows and channels) squared log error above the threshold.
+__global__ void k_cell_fb(const unsigned long long* __restrict__ esum, const unsigned int* __restrict__ ecnt,
+                          float threshold, uint8_t* __restrict__ cell_fb) {
+  const uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= (uint32_t)CACHE_CELLS) return;
+  const uint32_t c = ecnt[cell];
   const double mse = (double)esum[cell] / 16777216.0 / ((double)c * (double)MLP_OUT);
-  bool fb = (surf && !fp.cache_surface) || near || c == 0u || mse > (double)fp.threshold;
-  if (fp.kind == 3) {
-    flags[i] = fb && surf ? 1u : 0u;
-    flags2[i] = fb && !surf ? 1u : 0u;
-    return;
+  cell_fb[cell] = c == 0u || mse > (double)threshold ? 1u : 0u;
+}
+
+// The list (0 or 1) a record goes into, or -1.  Fallback lists: surface
+// records (unless surface records are cached), volume records within the
+// near-light radius (recorded one per march point, where the radiance is
+// steepest), and the records of cells marked by k_cell_fb.
+__device__ __forceinline__ int record_class(uint32_t code, const CacheListRule& r) {
+  const bool vol = (code & kCodeVolume) != 0u;
+  if (!r.fallback) return vol ? 1 : 0;
+  const bool cfb = r.cell_fb[code & (kCodeNear - 1u)] != 0;
+  if (!vol) return !r.cache_surface || cfb ? 0 : -1;
+  return (code & kCodeNear) || cfb ? 1 : -1;
+}
+
+// List building: a block takes LIST_TILE consecutive records, LIST_ITEMS
+// consecutive ones per thread (two 16-byte code loads).
+constexpr int LIST_THREADS = 256, LIST_ITEMS = 16, LIST_TILE = LIST_THREADS * LIST_ITEMS;
+
+__device__ __forceinline__ void load_codes(const uint16_t* __restrict__ code, uint64_t n, uint64_t base,
+                                           uint16_t (&v)[LIST_ITEMS]) {
+  if (base + LIST_ITEMS <= n) {
+    const uint4 a = *(const uint4*)&code[base], b = *(const uint4*)&code[base + 8];
+    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      v[2 * r] = (uint16_t)(w[r] & 0xffffu);
+      v[2 * r + 1] = (uint16_t)(w[r] >> 16);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < LIST_ITEMS; ++r) v[r] = base + r < n ? code[base + r] : (uint16_t)0;
   }
-  if (fp.kind != 0) fb = fb && surf == (fp.kind == 1);
-  flags[i] = fb ? 1u : 0u;
+}
+
+// The two lists' counts of a thread's records, packed (list 0 in the low 16
+// bits, list 1 in the high 16; a block's totals are at most LIST_TILE each).
+__device__ __forceinline__ uint32_t class_counts(const uint16_t (&v)[LIST_ITEMS], uint64_t n, uint64_t base,
+                                                 const CacheListRule& rule) {
+  uint32_t c = 0u;
+#pragma unroll
+  for (int r = 0; r < LIST_ITEMS; ++r) {
+    const int k = base + r < n ? record_class(v[r], rule) : -1;
+    c += k == 0 ? 1u : (k == 1 ? 0x10000u : 0u);
+  }
+  return c;
+}
+
+// Exclusive scan of one packed value per thread over the block; *total gets
+// the block's sum.
+__device__ __forceinline__ uint32_t list_block_scan(uint32_t v, uint32_t* lds_waves, uint32_t* total) {
+  const int lane = lane_id(), wave = (int)(threadIdx.x >> 6);
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = lane_read(x, lane >= d ? lane - d : lane);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) lds_waves[wave] = x;
+  __syncthreads();
+  uint32_t before = 0u, tot = 0u;
+#pragma unroll
+  for (int w = 0; w < LIST_THREADS / 64; ++w) {
+    const uint32_t s = lds_waves[w];
+    before += w < wave ? s : 0u;
+    tot += s;
+  }
+  *total = tot;
+  return before + x - v;
+}
+
+__global__ void __launch_bounds__(LIST_THREADS) k_list_count(const uint16_t* __restrict__ code, uint64_t n,
+                                                             CacheListRule rule, uint32_t* __restrict__ counts) {
+  __shared__ uint32_t lw[LIST_THREADS / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * LIST_TILE + (uint64_t)threadIdx.x * LIST_ITEMS;
+  uint16_t v[LIST_ITEMS];
+  load_codes(code, n, base, v);
+  uint32_t tot;
+  (void)list_block_scan(class_counts(v, n, base, rule), lw, &tot);
+  if (threadIdx.x == 0) {
+    counts[blockIdx.x] = tot & 0xffffu;
+    counts[gridDim.x + blockIdx.x] = tot >> 16;
+  }
+}
+
+__global__ void __launch_bounds__(LIST_THREADS) k_list_scatter(const uint16_t* __restrict__ code, uint64_t n,
+                                                               CacheListRule rule, const uint32_t* __restrict__ offs,
+                                                               uint32_t* __restrict__ list) {
+  __shared__ uint32_t lw[LIST_THREADS / 64];
+  const uint64_t base = (uint64_t)blockIdx.x * LIST_TILE + (uint64_t)threadIdx.x * LIST_ITEMS;
+  uint16_t v[LIST_ITEMS];
+  load_codes(code, n, base, v);
+  uint32_t tot;
+  const uint32_t ex = list_block_scan(class_counts(v, n, base, rule), lw, &tot);
+  uint32_t o0 = offs[blockIdx.x] + (ex & 0xffffu), o1 = offs[gridDim.x + blockIdx.x] + (ex >> 16);
+#pragma unroll
+  for (int r = 0; r < LIST_ITEMS; ++r) {
+    const int k = base + r < n ? record_class(v[r], rule) : -1;
+    if (k == 0) list[o0++] = (uint32_t)(base + r);
+    if (k == 1) list[o1++] = (uint32_t)(base + r);
+  }
 }
 
 // acc += sum of v[0..n) in a fixed order (one workgroup): the frame's training
@@ -316,12 +412,6 @@ __global__ void __launch_bounds__(256) k_sum_rows(const float* __restrict__ v, u
     __syncthreads();
   }
   if (threadIdx.x == 0) *acc += part[0];
-}
-
-__global__ void k_compact(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ off, uint64_t n,
-                          uint32_t* __restrict__ list, const uint32_t* __restrict__ base) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && flags[i]) list[(base ? *base : 0u) + off[i]] = (uint32_t)i;
 }
 
 // Per pixel: compensated sum over its records of weight * value, / spp.
@@ -392,12 +482,6 @@ hipError_t launch_cache_select(const float* in, const uint32_t* list_s, uint32_t
   return hipGetLastError();
 }
 
-hipError_t launch_kind_flags(const float* in, uint64_t n, int want_surface, uint32_t* flags, hipStream_t st) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_kind_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, want_surface, flags);
-  return hipGetLastError();
-}
-
 hipError_t launch_err_accum(const float* in, const float* row_err, uint32_t rows, unsigned long long* esum,
                             unsigned int* ecnt, hipStream_t st) {
   if (rows == 0) return hipSuccess;
@@ -410,41 +494,30 @@ hipError_t launch_sum_rows(const float* v, uint32_t n, double* acc, hipStream_t 
   return hipGetLastError();
 }
 
-hipError_t launch_cache_flags(const float* in, uint64_t n, const unsigned long long* esum, const unsigned int* ecnt,
-                              const CacheFlagParams& fp, uint32_t* flags, hipStream_t st, uint32_t* flags2) {
-  if (n == 0) return hipSuccess;
-  if (fp.kind == 3 && !flags2) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, esum, ecnt, fp, flags,
-                     flags2);
+hipError_t launch_cache_cell_fb(const unsigned long long* esum, const unsigned int* ecnt, float threshold,
+                                uint8_t* cell_fb, hipStream_t st) {
+  hipLaunchKernelGGL(k_cell_fb, dim3((CACHE_CELLS + 255) / 256), dim3(256), 0, st, esum, ecnt, threshold, cell_fb);
   return hipGetLastError();
 }
 
-__global__ void k_split(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ off, uint64_t n,
-                        uint32_t* __restrict__ list_a, uint32_t* __restrict__ list_b) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t o = off[i];
-  if (flags[i]) list_a[o] = (uint32_t)i;
-  else list_b[(uint32_t)i - o] = (uint32_t)i;
-}
+size_t cache_list_blocks(uint64_t n) { return (size_t)((n + LIST_TILE - 1) / LIST_TILE); }
 
-hipError_t launch_cache_split(const uint32_t* flags, const uint32_t* off, uint64_t n, uint32_t* list_a,
-                              uint32_t* list_b, hipStream_t st) {
+hipError_t launch_cache_list_count(const uint16_t* code, uint64_t n, const CacheListRule& rule, uint32_t* counts,
+                                   hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_split, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, flags, off, n, list_a, list_b);
+  if (rule.fallback && !rule.cell_fb) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_list_count, dim3((unsigned)cache_list_blocks(n)), dim3(LIST_THREADS), 0, st, code, n, rule,
+                     counts);
   return hipGetLastError();
 }
 
-hipError_t launch_cache_compact(const uint32_t* flags, const uint32_t* off, uint64_t n, uint32_t* list,
-                                const uint32_t* base, hipStream_t st) {
+hipError_t launch_cache_list_scatter(const uint16_t* code, uint64_t n, const CacheListRule& rule,
+                                     const uint32_t* offs, uint32_t* list, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_compact, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, flags, off, n, list, base);
+  if (rule.fallback && !rule.cell_fb) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_list_scatter, dim3((unsigned)cache_list_blocks(n)), dim3(LIST_THREADS), 0, st, code, n, rule,
+                     offs, list);
   return hipGetLastError();
-}
-
-hipError_t launch_cache_compact(const uint32_t* flags, const uint32_t* off, uint64_t n, uint32_t* list,
-                                hipStream_t st) {
-  return launch_cache_compact(flags, off, n, list, nullptr, st);
 }
 
 hipError_t launch_cache_accumulate(const uint32_t* off, const uint32_t* cnt, uint32_t npix, const float* wt,

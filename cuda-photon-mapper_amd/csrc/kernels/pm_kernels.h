@@ -249,17 +249,18 @@ struct CacheRecordLaunch {
   const uint32_t* offsets;   // [w*h]
   float* in;                 // [N x 9]
   float* wt;                 // [N x 3]
-  uint32_t* pix;             // [N]
+  uint16_t* code;            // [N] record codes (error cell, kind, near-light bit; record_code)
   int stride;                // march points per volume record
   float near2;               // squared near-light radius: no merging inside
 };
-struct CacheFlagParams {
-  float threshold;           // per-cell mean squared log error
-  int cache_surface;         // 0: surface records always fall back
-  float light[3];
-  float near2;               // volume records this close to the light fall back
-  int kind;                  // flag 0: every fallback record, 1: surface ones only, 2: volume ones only,
-                             // 3: surface ones in flags, volume ones in flags2 (one pass)
+enum : int { CACHE_CELLS = 2 * CACHE_GRID * CACHE_GRID * CACHE_GRID };
+// The two record lists of a cache frame, each in record order, built from the
+// record codes: list 0 then list 1 (record_class in pm_cache_render.hip).
+struct CacheListRule {
+  int fallback;              // 0: list 0 = surface records, list 1 = volume records;
+                             // 1: the exact fallbacks, surface ones in list 0, volume ones in list 1
+  int cache_surface;         // 0: every surface record falls back
+  const uint8_t* cell_fb;    // fallback lists: per error cell, 1 when its records fall back
 };
 struct CacheValueLaunch {
   GatherMap gs, gv;
@@ -276,21 +277,22 @@ hipError_t launch_cache_record(const CacheRecordLaunch& r, hipStream_t st);
 hipError_t launch_cache_values(const CacheValueLaunch& v, hipStream_t st);
 hipError_t launch_cache_select(const float* in, const uint32_t* list_s, uint32_t n_s, const uint32_t* list_v,
                                uint32_t n_v, uint32_t count, uint32_t seed, uint32_t frame, float* out, hipStream_t st);
-hipError_t launch_kind_flags(const float* in, uint64_t n, int want_surface, uint32_t* flags, hipStream_t st);
 hipError_t launch_err_accum(const float* in, const float* row_err, uint32_t rows, unsigned long long* esum,
                             unsigned int* ecnt, hipStream_t st);
 hipError_t launch_sum_rows(const float* v, uint32_t n, double* acc, hipStream_t st);
-hipError_t launch_cache_flags(const float* in, uint64_t n, const unsigned long long* esum, const unsigned int* ecnt,
-                              const CacheFlagParams& fp, uint32_t* flags, hipStream_t st, uint32_t* flags2 = nullptr);
-// Both lists from one flag array: list_a[off[i]] = i where flags[i], else
-// list_b[i - off[i]] = i (the complement's exclusive scan is i - off[i]).
-hipError_t launch_cache_split(const uint32_t* flags, const uint32_t* off, uint64_t n, uint32_t* list_a,
-                              uint32_t* list_b, hipStream_t st);
-// list[base + off[i]] = i where flags[i] (base: a device word, or null for 0)
-hipError_t launch_cache_compact(const uint32_t* flags, const uint32_t* off, uint64_t n, uint32_t* list,
-                                const uint32_t* base, hipStream_t st);
-hipError_t launch_cache_compact(const uint32_t* flags, const uint32_t* off, uint64_t n, uint32_t* list,
-                                hipStream_t st);
+// cell_fb[c] = 1 when error cell c saw no training row or its mean squared
+// log error exceeds the threshold (CACHE_CELLS bytes).
+hipError_t launch_cache_cell_fb(const unsigned long long* esum, const unsigned int* ecnt, float threshold,
+                                uint8_t* cell_fb, hipStream_t st);
+// List building over n record codes in cache_list_blocks(n) blocks: counts
+// [2 x blocks] (list 0's per block, then list 1's); after an exclusive scan of
+// the counts into offs, the scatter writes list 0 at [0, total_0) and list 1
+// at [total_0, total_0 + total_1) of `list`.
+size_t cache_list_blocks(uint64_t n);
+hipError_t launch_cache_list_count(const uint16_t* code, uint64_t n, const CacheListRule& rule, uint32_t* counts,
+                                   hipStream_t st);
+hipError_t launch_cache_list_scatter(const uint16_t* code, uint64_t n, const CacheListRule& rule,
+                                     const uint32_t* offs, uint32_t* list, hipStream_t st);
 hipError_t launch_cache_accumulate(const uint32_t* off, const uint32_t* cnt, uint32_t npix, const float* wt,
                                    const float* val, int spp, float exposure, float* rgb, uchar4* u8, hipStream_t st);
 

@@ -110,7 +110,10 @@ size_t mlp_infer_lds_bytes();
 hipError_t mlp_infer(const MlpDev& w, const float* in, uint32_t n, uint16_t* h1, float* rgb, hipStream_t st,
                      int k32, uint32_t chunk_rows);
 // All five layers in one kernel with the activations in registers (pm_mlp_infer.hip).
+// With `list`, row i is record list[i] of `in` and `rgb` (n = the list's
+// length); the other records' outputs are left as they are.
 size_t mlp_fused_lds_bytes();
-hipError_t mlp_infer_fused(const MlpDev& w, const float* in, uint32_t n, float* rgb, hipStream_t st);
+hipError_t mlp_infer_fused(const MlpDev& w, const float* in, uint32_t n, float* rgb, hipStream_t st,
+                           const uint32_t* list = nullptr);
 
 }  // namespace pm

@@ -479,6 +479,7 @@ __host__ __device__ __forceinline__ int l1_feature_col(int ks, int g) {
 struct FusedArgs {
   const float* in;
   uint32_t rows;
+  const uint32_t* list;  // row i of the launch is record list[i] (in and rgb); null: record i
   const uint16_t* w[MLP_LAYERS];  // BF16 weights, row-major [nout x kin]
   const float* b[MLP_LAYERS];
   float* rgb;
@@ -548,7 +549,8 @@ __global__ void __launch_bounds__(IWF) k_mlp_fused(FusedArgs A) {
     for (int u = 0; u < FT; ++u) {
       const uint32_t row = t * FROWS + 16u * (uint32_t)u + (uint32_t)j;
       const bool ok = t < steps && row < A.rows;
-      const float* x = A.in + (size_t)(ok ? row : 0u) * MLP_IN;
+      const uint32_t rec = !ok ? 0u : (A.list ? A.list[row] : row);
+      const float* x = A.in + (size_t)rec * MLP_IN;
       x0[u] = ok ? x[2 * g] : 0.0f;
       x1[u] = ok ? x[2 * g + 1] : 0.0f;
       v8[u] = ok && g < 3 ? x[8] : 0.0f;
@@ -666,10 +668,11 @@ __global__ void __launch_bounds__(IWF) k_mlp_fused(FusedArgs A) {
     const f32x4 yg = g == 0 ? y[0] : (g == 1 ? y[1] : (g == 2 ? y[2] : y[3]));
     const uint32_t row = row0 + 16u * (uint32_t)g + (uint32_t)j;
     if (row < A.rows) {
+      const uint32_t orec = A.list ? A.list[row] : row;
 #pragma unroll
       for (int e = 0; e < MLP_OUT; ++e) {
         const float v = exp_(yg[e]) - 1.0f;
-        A.rgb[(size_t)row * MLP_OUT + e] = v > 0.0f ? v : 0.0f;
+        A.rgb[(size_t)orec * MLP_OUT + e] = v > 0.0f ? v : 0.0f;
       }
     }
   }
@@ -677,8 +680,10 @@ __global__ void __launch_bounds__(IWF) k_mlp_fused(FusedArgs A) {
 
 size_t mlp_fused_lds_bytes() { return FLDS; }
 
-// Inference for n rows through the single fused kernel (no activation buffer).
-hipError_t mlp_infer_fused(const MlpDev& w, const float* in, uint32_t n, float* rgb, hipStream_t st) {
+// Inference for n rows (or the n records of `list`) through the single fused
+// kernel (no activation buffer).
+hipError_t mlp_infer_fused(const MlpDev& w, const float* in, uint32_t n, float* rgb, hipStream_t st,
+                           const uint32_t* list) {
   if (n == 0) return hipSuccess;
   int dev = 0, cus = 0;
   hipError_t e = hipGetDevice(&dev);
@@ -688,6 +693,7 @@ hipError_t mlp_infer_fused(const MlpDev& w, const float* in, uint32_t n, float* 
   FusedArgs a;
   a.in = in;
   a.rows = n;
+  a.list = list;
   for (int l = 0; l < MLP_LAYERS; ++l) {
     a.w[l] = w.wb[l];
     a.b[l] = w.b[l];

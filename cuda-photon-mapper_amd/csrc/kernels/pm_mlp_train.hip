@@ -90,15 +90,15 @@ __device__ __forceinline__ void nt_stage(const NtArgs& g, uint16_t (*As)[NT_LDS]
   }
 }
 
-__global__ void __launch_bounds__(256) k_gemm_nt(NtArgs g) {
-  __shared__ __align__(16) uint16_t As[NT_TILE][NT_LDS];
-  __shared__ __align__(16) uint16_t Bs[NT_TILE][NT_LDS];
+// Tile (bx, by) of slice bz (of nz) of the product g.
+__device__ __forceinline__ void nt_tile(const NtArgs& g, int bx, int by, int bz, int nz, uint16_t (*As)[NT_LDS],
+                                        uint16_t (*Bs)[NT_LDS]) {
   const int lane = lane_id();
   const int wave = (int)(threadIdx.x >> 6);
-  const int m0 = (int)blockIdx.y * NT_TILE, n0 = (int)blockIdx.x * NT_TILE;
+  const int m0 = by * NT_TILE, n0 = bx * NT_TILE;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  const bool split = gridDim.z > 1 || g.part;
-  const int k_begin = split ? (int)blockIdx.z * g.kps : 0;
+  const bool split = nz > 1 || g.part;
+  const int k_begin = split ? bz * g.kps : 0;
   const int k_end = split ? min(g.K, k_begin + g.kps) : g.K;
   const int r16 = lane & 15, q = lane >> 4;
   f32x4_t acc[2][2];
@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(256) k_gemm_nt(NtArgs g) {
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
   float rowsum = 0.0f;  // bias gradient: row threadIdx.x of A over this slice (n-tile 0)
-  const bool want_rowsum = g.bpart && blockIdx.x == 0 && threadIdx.x < NT_TILE;
+  const bool want_rowsum = g.bpart && bx == 0 && threadIdx.x < NT_TILE;
   // the epilogue's bias and mask values, loaded before the K loop so that
   // their latency overlaps the operand loads
   float bias_v[2] = {0.0f, 0.0f};
@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(256) k_gemm_nt(NtArgs g) {
   }
   // lane holds C[wm + 16a + 4q + e][wn + 16b + r16]
   if (split) {
-    float* P = g.part + (size_t)blockIdx.z * g.M * g.N;
+    float* P = g.part + (size_t)bz * g.M * g.N;
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(256) k_gemm_nt(NtArgs g) {
           if (m < g.M && n < g.N) P[(size_t)m * g.N + n] = acc[a][b][e];
         }
     if (want_rowsum && m0 + (int)threadIdx.x < g.M)
-      g.bpart[(size_t)blockIdx.z * g.M + m0 + threadIdx.x] = rowsum;
+      g.bpart[(size_t)bz * g.M + m0 + threadIdx.x] = rowsum;
     return;
   }
   // unsplit: bias, ReLU, mask, then the BF16 / FP32 outputs; the transposed
@@ -201,9 +201,54 @@ __global__ void __launch_bounds__(256) k_gemm_nt(NtArgs g) {
   }
 }
 
+__global__ void __launch_bounds__(256) k_gemm_nt(NtArgs g) {
+  __shared__ __align__(16) uint16_t As[NT_TILE][NT_LDS];
+  __shared__ __align__(16) uint16_t Bs[NT_TILE][NT_LDS];
+  nt_tile(g, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.z, As, Bs);
+}
+
+// Two independent products in one launch: the first p.tiles[0] workgroups
+// take product 0 (n-tiles fastest, then m-tiles, then slices), the rest
+// product 1.  Used for a layer's weight gradient beside the delta of the
+// layer below, which read the same delta and write different buffers.
+struct NtPair {
+  NtArgs a[2];
+  int nx[2], ny[2], nz[2];
+  int tiles0;
+};
+
+__global__ void __launch_bounds__(256) k_gemm_nt_pair(NtPair p) {
+  __shared__ __align__(16) uint16_t As[NT_TILE][NT_LDS];
+  __shared__ __align__(16) uint16_t Bs[NT_TILE][NT_LDS];
+  const int b = (int)blockIdx.x;
+  if (b < p.tiles0) {
+    nt_tile(p.a[0], b % p.nx[0], (b / p.nx[0]) % p.ny[0], b / (p.nx[0] * p.ny[0]), p.nz[0], As, Bs);
+  } else {
+    const int c = b - p.tiles0;
+    nt_tile(p.a[1], c % p.nx[1], (c / p.nx[1]) % p.ny[1], c / (p.nx[1] * p.ny[1]), p.nz[1], As, Bs);
+  }
+}
+
 hipError_t gemm_nt(const NtArgs& a, int slices, hipStream_t st) {
   dim3 grid((unsigned)((a.N + NT_TILE - 1) / NT_TILE), (unsigned)((a.M + NT_TILE - 1) / NT_TILE), (unsigned)slices);
   hipLaunchKernelGGL(k_gemm_nt, grid, dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t gemm_nt_pair(const NtArgs& a, int slices_a, const NtArgs& b, int slices_b, hipStream_t st) {
+  NtPair p;
+  p.a[0] = a;
+  p.a[1] = b;
+  const NtArgs* g[2] = {&a, &b};
+  const int sl[2] = {slices_a, slices_b};
+  for (int i = 0; i < 2; ++i) {
+    p.nx[i] = (g[i]->N + NT_TILE - 1) / NT_TILE;
+    p.ny[i] = (g[i]->M + NT_TILE - 1) / NT_TILE;
+    p.nz[i] = sl[i];
+  }
+  p.tiles0 = p.nx[0] * p.ny[0] * p.nz[0];
+  const int total = p.tiles0 + p.nx[1] * p.ny[1] * p.nz[1];
+  hipLaunchKernelGGL(k_gemm_nt_pair, dim3((unsigned)total), dim3(256), 0, st, p);
   return hipGetLastError();
 }
 
@@ -369,7 +414,7 @@ hipError_t mlp_train_grads(const MlpDev& w, const float* in, uint32_t rows, uint
     a.kps = kps;
     a.part = part;
     a.bpart = part + (size_t)slices * nout * kl;
-    if ((e = gemm_nt(a, slices, st)) != hipSuccess) return e;
+    if (l == 0 && (e = gemm_nt(a, slices, st)) != hipSuccess) return e;
     S.part[l] = a.part;
     S.bpart[l] = a.bpart;
     tt.part_w[l] = a.part;
@@ -388,7 +433,9 @@ hipError_t mlp_train_grads(const MlpDev& w, const float* in, uint32_t rows, uint
     b.mask = H[l - 1]; b.ldm = MLP_HID;
     b.Cb = nd; b.ldc = MLP_HID;
     b.Ct = ndT; b.ldt = (int)ld;
-    if ((e = gemm_nt(b, 1, st)) != hipSuccess) return e;
+    // the weight gradient of layer l and the delta of layer l - 1 read the
+    // same delta and write different buffers: one launch
+    if ((e = gemm_nt_pair(a, slices, b, 1, st)) != hipSuccess) return e;
     delta = nd;
     deltaT = ndT;
     dcols = MLP_HID;
