@@ -575,7 +575,7 @@ int pm_set_option(pm_context* c, int option, int value) {
       c->cache.infer_chunk_log2 = value;
       return PM_OK;
     case PM_OPT_CACHE_FUSED_INFER:
-      if (value < 0 || value > 3) return PM_ERR_INVALID;
+      if (value < 0 || value > 4) return PM_ERR_INVALID;
       c->cache.fused_infer = value;
       return PM_OK;
     case PM_OPT_TILE_REGIONS:

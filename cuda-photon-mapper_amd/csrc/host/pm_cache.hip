@@ -143,6 +143,7 @@ void cache_release(CacheState& k) {
   for (DevBuf* b : {&k.params, &k.grads, &k.adam_m, &k.adam_v, &k.params_bf16, &k.params_ema, &k.ema_bf16, &k.X, &k.y, &k.dy, &k.d0, &k.d1,
                     &k.row_err, &k.rgb, &k.err_sum, &k.err_cnt, &k.rec_cnt, &k.rec_off, &k.rec_in,
                     &k.rec_wt, &k.rec_code, &k.rec_val, &k.sel_in, &k.sel_t, &k.list, &k.list_cnt, &k.cell_fb,
+                    &k.row_code, &k.row_list,
                     &k.XT, &k.dT[0], &k.dT[1], &k.dyT, &k.tpart})
     b->release();
   for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) {
@@ -152,10 +153,11 @@ void cache_release(CacheState& k) {
   k.ready = false;
 }
 
-// The two record lists of `rule` over the frame's N record codes, each in
+// The two record lists of `rule` over N record codes, each in
 // record order: list 0 at list[0, n[0]), list 1 at list[n[0], n[0] + n[1]).
 // One count pass, a scan of the per-block counts, one scatter pass.
-int build_lists(pm_context* c, uint64_t N, const CacheListRule& rule, uint32_t* list, uint32_t n[2]) {
+int build_lists(pm_context* c, const uint16_t* code, uint64_t N, const CacheListRule& rule, uint32_t* list,
+                uint32_t n[2]) {
   CacheState& k = c->cache;
   hipStream_t st = c->stream;
   n[0] = n[1] = 0u;
@@ -167,9 +169,9 @@ int build_lists(pm_context* c, uint64_t N, const CacheListRule& rule, uint32_t* 
   uint32_t* counts = k.list_cnt.as<uint32_t>();
   uint32_t* offs = counts + 2 * nb;
   uint32_t* total = c->small.as<uint32_t>() + 12;
-  HIPCHK(launch_cache_list_count(k.rec_code.as<uint16_t>(), N, rule, counts, st));
+  HIPCHK(launch_cache_list_count(code, N, rule, counts, st));
   HIPCHK(exclusive_scan(counts, offs, 2 * nb, c->scratch.as<uint32_t>(), total, st));
-  HIPCHK(launch_cache_list_scatter(k.rec_code.as<uint16_t>(), N, rule, offs, list, st));
+  HIPCHK(launch_cache_list_scatter(code, N, rule, offs, list, st));
   uint32_t h[2] = {0u, 0u};  // list 0's total (the offset of list 1's first block), both totals
   HIPCHK(hipMemcpyAsync(&h[0], offs + nb, 4, hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&h[1], total, 4, hipMemcpyDeviceToHost, st));
@@ -180,8 +182,8 @@ int build_lists(pm_context* c, uint64_t N, const CacheListRule& rule, uint32_t* 
 }
 
 // Encode + forward + decode for n device input rows, in chunks.  With `list`
-// (nlist record indices) the fused kernel infers only the listed records;
-// the other forms infer all n.
+// (nlist record indices, all of volume records) the fused kernel infers only
+// the listed records; the other forms infer all n.
 int cache_predict_dev(pm_context* c, const float* in, uint32_t n, float* rgb, const uint32_t* list = nullptr,
                       uint32_t nlist = 0) {
   int r = ensure_cache(c);
@@ -192,8 +194,27 @@ int cache_predict_dev(pm_context* c, const float* in, uint32_t n, float* rgb, co
   if (k.fused_infer) {
     int maxlds = 0;
     HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
-    if (k.fused_infer == 3 && (size_t)maxlds >= mlp_fused_lds_bytes()) {
-      HIPCHK(mlp_infer_fused(wi, in, list ? nlist : n, rgb, c->stream, list));
+    if (k.fused_infer >= 3 && (size_t)maxlds >= mlp_fused_lds_bytes()) {
+      const bool split = k.fused_infer == 4;
+      if (list) {
+        HIPCHK(mlp_infer_fused(wi, in, nlist, rgb, c->stream, list, split));
+        return PM_OK;
+      }
+      if (!split || n == 0) {
+        HIPCHK(mlp_infer_fused(wi, in, n, rgb, c->stream));
+        return PM_OK;
+      }
+      // surface rows on the full layer 1, volume rows on the short one
+      HIPCHK(k.row_code.ensure(2ull * n));
+      HIPCHK(k.row_list.ensure(4ull * n));
+      HIPCHK(launch_cache_row_codes(in, n, k.row_code.as<uint16_t>(), c->stream));
+      CacheListRule kinds;
+      memset(&kinds, 0, sizeof(kinds));
+      uint32_t nk[2] = {0, 0};
+      if ((r = build_lists(c, k.row_code.as<uint16_t>(), n, kinds, k.row_list.as<uint32_t>(), nk)) != PM_OK)
+        return r;
+      if (nk[0]) HIPCHK(mlp_infer_fused(wi, in, nk[0], rgb, c->stream, k.row_list.as<uint32_t>(), false));
+      if (nk[1]) HIPCHK(mlp_infer_fused(wi, in, nk[1], rgb, c->stream, k.row_list.as<uint32_t>() + nk[0], true));
       return PM_OK;
     }
     if ((size_t)maxlds >= mlp_infer_lds_bytes()) {
@@ -434,7 +455,7 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
     CacheListRule kinds;
     memset(&kinds, 0, sizeof(kinds));
     uint32_t nk[2] = {0, 0};
-    int rl_err = build_lists(c, N, kinds, k.list.as<uint32_t>(), nk);
+    int rl_err = build_lists(c, k.rec_code.as<uint16_t>(), N, kinds, k.list.as<uint32_t>(), nk);
     if (rl_err != PM_OK) return rl_err;
     const uint32_t* list_s = k.list.as<uint32_t>();
     const uint32_t* list_v = list_s + nk[0];
@@ -475,7 +496,7 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   fbr.cache_surface = k.cache_surface;
   fbr.cell_fb = k.cell_fb.as<uint8_t>();
   uint32_t nk2[2] = {0, 0};
-  r = build_lists(c, N, fbr, k.list.as<uint32_t>(), nk2);
+  r = build_lists(c, k.rec_code.as<uint16_t>(), N, fbr, k.list.as<uint32_t>(), nk2);
   if (r != PM_OK) return r;
   const uint32_t nfb = nk2[0] + nk2[1];
   for (int kind = 1; kind <= 2; ++kind) {  // one launch per kind of row

@@ -66,8 +66,10 @@ struct CacheState {
   uint32_t batch = 1u << 14;
   float lr = 1e-3f;
   uint32_t frame = 0;
-  int fused_infer = 3;     // pm_mlp_infer.hip kernels (when the LDS fits): 3 = one fused kernel, activations in
-                           // registers; 2 = layer 1 + layers 2..5 on 16x16x32 MFMA, 1 = 16x16x16; 0: generic GEMMs
+  int fused_infer = 4;     // pm_mlp_infer.hip kernels (when the LDS fits): 4 = one fused kernel, activations in
+                           // registers, volume records on the 5-step layer 1; 3 = the same kernel, full layer 1
+                           // for every row; 2 = layer 1 + layers 2..5 on 16x16x32 MFMA, 1 = 16x16x16; 0: generic GEMMs
+  DevBuf row_code, row_list;  // pm_cache_predict under mode 4: the rows' codes and kind lists
   int infer_chunk_log2 = 20;  // rows per fused-inference launch pair (log2)
   int stride = 1;          // march points per volume record
   int cache_surface = 0;   // 0: surface queries are always gathered exactly

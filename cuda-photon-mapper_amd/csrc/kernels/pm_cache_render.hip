@@ -276,6 +276,11 @@ __device__ __forceinline__ uint32_t err_cell(const float* p) {
   return err_cell_of(p[0], p[1], p[2], !(p[6] == 0.0f && p[7] == 0.0f && p[8] == 0.0f));
 }
 
+__global__ void k_row_codes(const float* __restrict__ in, uint64_t n, uint16_t* __restrict__ code) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) code[i] = (uint16_t)err_cell(in + i * MLP_IN);
+}
+
 // Bins per-row squared errors; integer (fixed-point 2^-24) sums are order-independent.
 __global__ void k_err_accum(const float* __restrict__ in, const float* __restrict__ row_err, uint32_t rows,
                             unsigned long long* __restrict__ esum, unsigned int* __restrict__ ecnt) {
@@ -501,6 +506,12 @@ hipError_t launch_cache_cell_fb(const unsigned long long* esum, const unsigned i
 }
 
 size_t cache_list_blocks(uint64_t n) { return (size_t)((n + LIST_TILE - 1) / LIST_TILE); }
+
+hipError_t launch_cache_row_codes(const float* in, uint64_t n, uint16_t* code, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_row_codes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, in, n, code);
+  return hipGetLastError();
+}
 
 hipError_t launch_cache_list_count(const uint16_t* code, uint64_t n, const CacheListRule& rule, uint32_t* counts,
                                    hipStream_t st) {

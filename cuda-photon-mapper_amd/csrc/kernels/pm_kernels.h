@@ -289,6 +289,8 @@ hipError_t launch_cache_cell_fb(const unsigned long long* esum, const unsigned i
 // the counts into offs, the scatter writes list 0 at [0, total_0) and list 1
 // at [total_0, total_0 + total_1) of `list`.
 size_t cache_list_blocks(uint64_t n);
+// code[i] = the record code of query row i (its error cell and kind; no near-light bit)
+hipError_t launch_cache_row_codes(const float* in, uint64_t n, uint16_t* code, hipStream_t st);
 hipError_t launch_cache_list_count(const uint16_t* code, uint64_t n, const CacheListRule& rule, uint32_t* counts,
                                    hipStream_t st);
 hipError_t launch_cache_list_scatter(const uint16_t* code, uint64_t n, const CacheListRule& rule,

@@ -111,9 +111,10 @@ hipError_t mlp_infer(const MlpDev& w, const float* in, uint32_t n, uint16_t* h1,
                      int k32, uint32_t chunk_rows);
 // All five layers in one kernel with the activations in registers (pm_mlp_infer.hip).
 // With `list`, row i is record list[i] of `in` and `rgb` (n = the list's
-// length); the other records' outputs are left as they are.
+// length); the other records' outputs are left as they are.  `volume`: every
+// row is a volume record (inputs 6..8 zero), which takes a shorter layer 1.
 size_t mlp_fused_lds_bytes();
 hipError_t mlp_infer_fused(const MlpDev& w, const float* in, uint32_t n, float* rgb, hipStream_t st,
-                           const uint32_t* list = nullptr);
+                           const uint32_t* list = nullptr, bool volume = false);
 
 }  // namespace pm

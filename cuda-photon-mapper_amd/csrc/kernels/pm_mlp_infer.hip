@@ -476,6 +476,22 @@ __host__ __device__ __forceinline__ int l1_feature_col(int ks, int g) {
   return i < 0 ? -1 : 2 * MLP_BANDS * i + 2 * l1_band0(ks, g);
 }
 
+// Volume records (normal 0) in 5 K steps: the features of inputs 6..8 are the
+// constants sin 0 = 0 and cos 0 = 1, so their products with W1 are folded into
+// layer 1's bias, and only the 144 features of position and direction remain.
+// Lane group g takes input g in K steps 0..2 (bands 0-3, 4-7, 8-11), and in
+// K steps 3..4 input 4 + (g & 1): groups 0 and 1 bands 0-3 then 4-7, groups
+// 2 and 3 bands 8-11 (then padding).  Two inputs and four sin/cos evaluations
+// per lane and row.
+__host__ __device__ __forceinline__ int l1v_input(int ks, int g) {
+  return ks < 3 ? g : (ks == 3 || g < 2 ? 4 + (g & 1) : -1);
+}
+__host__ __device__ __forceinline__ int l1v_band0(int ks, int g) { return ks < 3 ? 4 * ks : (ks == 3 ? (g < 2 ? 0 : 8) : 4); }
+__host__ __device__ __forceinline__ int l1v_feature_col(int ks, int g) {
+  const int i = l1v_input(ks, g);
+  return i < 0 ? -1 : 2 * MLP_BANDS * i + 2 * l1v_band0(ks, g);
+}
+
 struct FusedArgs {
   const float* in;
   uint32_t rows;
@@ -505,18 +521,21 @@ __device__ __forceinline__ bf16x8 next_b(const f32x4& d0, const f32x4& d1) {
   return *(const bf16x8*)&v;
 }
 
+// VOL: every row is a volume record (normal 0), layer 1 in KS1 = 5 K steps.
+template <bool VOL>
 __global__ void __launch_bounds__(IWF) k_mlp_fused(FusedArgs A) {
   extern __shared__ __align__(16) unsigned char flds[];
+  constexpr int KS1 = VOL ? 5 : MLP_KPAD / 32;  // layer 1's K steps
   // weights -> LDS: one 16-B chunk per step; layers 2..5 in the permuted K order
   {
-    const uint32_t c1 = 128u * 28u, c24 = 3u * 128u * 16u, c5 = 16u * 16u;
+    const uint32_t c1 = 128u * 4u * (uint32_t)KS1, c24 = 3u * 128u * 16u, c5 = 16u * 16u;
     for (uint32_t e = threadIdx.x; e < c1 + c24 + c5; e += IWF) {
       uint4 v;
       uint32_t dst;
       if (e < c1) {
         // W1 row r, K step ks, lane group g: 8 features (4 bands of one input)
-        const uint32_t r = e / 28u, k8 = e % 28u, ks = k8 >> 2, g = k8 & 3u;
-        const int col = l1_feature_col((int)ks, (int)g);
+        const uint32_t r = e / (4u * KS1), k8 = e % (4u * KS1), ks = k8 >> 2, g = k8 & 3u;
+        const int col = VOL ? l1v_feature_col((int)ks, (int)g) : l1_feature_col((int)ks, (int)g);
         v = col < 0 ? make_uint4(0u, 0u, 0u, 0u) : *(const uint4*)&A.w[0][(size_t)r * MLP_KPAD + col];
         dst = FW1 + fchunk(128u, ks, r, g);
       } else {
@@ -534,7 +553,17 @@ __global__ void __launch_bounds__(IWF) k_mlp_fused(FusedArgs A) {
       *(uint4*)&flds[dst] = v;
     }
     float* bl = (float*)&flds[FB14];
-    for (uint32_t e = threadIdx.x; e < 4u * 128u; e += IWF) bl[e] = A.b[e >> 7][e & 127u];
+    for (uint32_t e = threadIdx.x; e < 4u * 128u; e += IWF) {
+      float b = A.b[e >> 7][e & 127u];
+      if (VOL && e < 128u) {
+        // + W1 x the constant features of inputs 6..8 (the cos columns, 1)
+        const uint16_t* w = A.w[0] + (size_t)e * MLP_KPAD;
+        float f = 0.0f;
+        for (int c = 2 * MLP_BANDS * 6 + 1; c < MLP_FEAT; c += 2) f += bf16_to_f32(w[c]);
+        b += f;
+      }
+      bl[e] = b;
+    }
   }
   __syncthreads();
   const float* bias_l = (const float*)&flds[FB14];
@@ -551,9 +580,14 @@ __global__ void __launch_bounds__(IWF) k_mlp_fused(FusedArgs A) {
       const bool ok = t < steps && row < A.rows;
       const uint32_t rec = !ok ? 0u : (A.list ? A.list[row] : row);
       const float* x = A.in + (size_t)rec * MLP_IN;
-      x0[u] = ok ? x[2 * g] : 0.0f;
-      x1[u] = ok ? x[2 * g + 1] : 0.0f;
-      v8[u] = ok && g < 3 ? x[8] : 0.0f;
+      if (VOL) {
+        x0[u] = ok ? x[g] : 0.0f;
+        x1[u] = ok ? x[4 + (g & 1)] : 0.0f;
+      } else {
+        x0[u] = ok ? x[2 * g] : 0.0f;
+        x1[u] = ok ? x[2 * g + 1] : 0.0f;
+        v8[u] = ok && g < 3 ? x[8] : 0.0f;
+      }
     }
   };
   const uint32_t tstride = gridDim.x * (IWF / 64);
@@ -571,9 +605,13 @@ __global__ void __launch_bounds__(IWF) k_mlp_fused(FusedArgs A) {
       for (int u = 0; u < FT; ++u) {
         if (ks == 0) sincos_turns(x0[u] * 0.5f, &sn[u], &cs[u]);    // band 0: 2^-1 turns per unit
         if (ks == 2) sincos_turns(x0[u] * 128.0f, &sn[u], &cs[u]);  // band 8: 2^7 turns per unit
-        if (ks == 3) sincos_turns(x1[u] * 0.5f, &sn[u], &cs[u]);    // the lane's second input
-        if (ks == 5) sincos_turns(x1[u] * 128.0f, &sn[u], &cs[u]);
-        if (ks == 6) sincos_turns(v8[u] * u2f((uint32_t)(126 + 4 * g) << 23), &sn[u], &cs[u]);  // band 4g
+        if (VOL) {
+          if (ks == 3) sincos_turns(x1[u] * (g < 2 ? 0.5f : 128.0f), &sn[u], &cs[u]);  // band 0 or 8
+        } else {
+          if (ks == 3) sincos_turns(x1[u] * 0.5f, &sn[u], &cs[u]);  // the lane's second input
+          if (ks == 5) sincos_turns(x1[u] * 128.0f, &sn[u], &cs[u]);
+          if (ks == 6) sincos_turns(v8[u] * u2f((uint32_t)(126 + 4 * g) << 23), &sn[u], &cs[u]);  // band 4g
+        }
         float f[8];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -597,7 +635,7 @@ __global__ void __launch_bounds__(IWF) k_mlp_fused(FusedArgs A) {
       bf16x8 fa[FT], fb[FT];
       features(0, fa);
 #pragma unroll
-      for (int ks = 0; ks < MLP_KPAD / 32; ++ks) {
+      for (int ks = 0; ks < KS1; ++ks) {
         bf16x8* cur = (ks & 1) ? fb : fa;
         bf16x8* nxt = (ks & 1) ? fa : fb;
 #pragma unroll
@@ -608,7 +646,7 @@ __global__ void __launch_bounds__(IWF) k_mlp_fused(FusedArgs A) {
           for (int u = 0; u < FT; ++u)
             acc[c][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, cur[u], ks == 0 ? b0 : acc[c][u], 0, 0, 0);
         }
-        if (ks + 1 < MLP_KPAD / 32) features(ks + 1, nxt);
+        if (ks + 1 < KS1) features(ks + 1, nxt);
         // keep each step's work in its step: scheduling the unrolled steps
         // together would raise the register pressure past the budget
         __builtin_amdgcn_sched_barrier(0);
@@ -681,9 +719,9 @@ __global__ void __launch_bounds__(IWF) k_mlp_fused(FusedArgs A) {
 size_t mlp_fused_lds_bytes() { return FLDS; }
 
 // Inference for n rows (or the n records of `list`) through the single fused
-// kernel (no activation buffer).
+// kernel (no activation buffer); `volume`: every row is a volume record.
 hipError_t mlp_infer_fused(const MlpDev& w, const float* in, uint32_t n, float* rgb, hipStream_t st,
-                           const uint32_t* list) {
+                           const uint32_t* list, bool volume) {
   if (n == 0) return hipSuccess;
   int dev = 0, cus = 0;
   hipError_t e = hipGetDevice(&dev);
@@ -701,7 +739,8 @@ hipError_t mlp_infer_fused(const MlpDev& w, const float* in, uint32_t n, float* 
   a.rgb = rgb;
   const uint32_t steps = (n + FROWS - 1) / FROWS;
   const uint32_t grid = std::min<uint32_t>((uint32_t)cus, (steps + IWF / 64 - 1) / (IWF / 64));
-  hipLaunchKernelGGL(k_mlp_fused, dim3(grid), dim3(IWF), FLDS, st, a);
+  if (volume) hipLaunchKernelGGL(k_mlp_fused<true>, dim3(grid), dim3(IWF), FLDS, st, a);
+  else hipLaunchKernelGGL(k_mlp_fused<false>, dim3(grid), dim3(IWF), FLDS, st, a);
   return hipGetLastError();
 }
 

@@ -203,9 +203,12 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
  *                      (3 and 6 are not used: PM_ERR_INVALID);
  *                      every mode gives the same neighbour sets and images;
  * PM_OPT_BLOCKS_PER_CU workgroups per compute unit of the persistent mode-0 grid;
- * PM_OPT_CACHE_FUSED_INFER 3 = cache inference through one fused kernel, all
+ * PM_OPT_CACHE_FUSED_INFER 4 = cache inference through one fused kernel, all
  *                      five layers with the activations in registers and the
- *                      weights in LDS (default), 2 = two LDS-resident kernels
+ *                      weights in LDS, volume records (zero normal) through a
+ *                      layer 1 without the normal's constant features (5 K
+ *                      steps of 32 instead of 7; default), 3 = the same kernel
+ *                      with the full layer 1 for every row, 2 = two LDS-resident kernels
  *                      (layer 1, layers 2..5) with 16x16x32 BF16 MFMA, 1 = the
  *                      same with 16x16x16 BF16 MFMA, 0 = the generic GEMMs;
  * PM_OPT_CACHE_INFER_CHUNK log2 of the rows per fused-inference launch pair,

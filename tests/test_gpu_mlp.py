@@ -67,7 +67,7 @@ def test_forward_matches_reference(pm):
     assert err < 1e-2, err
 
 
-@pytest.mark.parametrize("fused_mode", [1, 2, 3])
+@pytest.mark.parametrize("fused_mode", [1, 2, 3, 4])
 @pytest.mark.parametrize("n", [1, 37, 5000, 300000, 1100000])
 def test_fused_inference_matches_generic(pm, n, fused_mode):
     say("case: fused (mode %d) vs generic inference, %d rows" % (fused_mode, n))
@@ -78,20 +78,49 @@ def test_fused_inference_matches_generic(pm, n, fused_mode):
     gen = pm.cache_predict(x).astype(np.float64)
     pm.set_option(pmamd.PM_OPT_CACHE_FUSED_INFER, fused_mode)
     fused = pm.cache_predict(x).astype(np.float64)
-    pm.set_option(pmamd.PM_OPT_CACHE_FUSED_INFER, 3)
+    pm.set_option(pmamd.PM_OPT_CACHE_FUSED_INFER, 4)
     if fused_mode == 1:
         # same K order as the generic GEMMs: differences are FP32 rounding only
         assert np.abs(fused - gen).max() <= 1e-3 * np.abs(gen).max() + 1e-6
     else:
-        # K = 32 accumulation order (permuted K in mode 3) and double-angle
-        # features change the FP32 rounding ahead of each BF16 conversion, so a
-        # few activations land one BF16 step apart; bounded in norm and in the
-        # worst case
+        # K = 32 accumulation order (permuted K in modes 3 and 4), double-angle
+        # features and, in mode 4, the normal's constant features of the
+        # volume rows summed into the bias change the FP32 rounding ahead of
+        # each BF16 conversion, so a few activations land one BF16 step apart;
+        # bounded in norm and in the worst case
         assert np.linalg.norm(fused - gen) <= 2e-3 * np.linalg.norm(gen) + 1e-6
         assert np.abs(fused - gen).max() <= 1e-2 * np.abs(gen).max() + 1e-6
     if n <= 5000:
         ref = R.predict(flat.astype(np.float64), x.astype(np.float64), bf=True)
         assert np.linalg.norm(fused - ref) / np.linalg.norm(ref) < 1e-2
+
+
+@pytest.mark.parametrize("n", [1, 2, 37, 5000, 1100000])
+def test_volume_layer1_matches_full(pm, n):
+    """Mode 4 sends the surface rows through the full fused kernel (as mode 3,
+    bit for bit) and the volume rows through the 5-step layer 1, whose only
+    change is the sum of W1's normal columns in the bias."""
+    say("case: fused inference, short layer 1 for volume rows, %d rows" % n)
+    flat = R.init(17).astype(np.float32)
+    pm.cache_params(flat)
+    x, _ = sample(n, 9)
+    x[n // 2:, 6:9] = 0.0
+    pm.set_option(pmamd.PM_OPT_CACHE_FUSED_INFER, 3)
+    full = pm.cache_predict(x).astype(np.float64)
+    pm.set_option(pmamd.PM_OPT_CACHE_FUSED_INFER, 4)
+    short = pm.cache_predict(x).astype(np.float64)
+    surf = np.any(x[:, 6:9] != 0.0, axis=1)
+    assert np.array_equal(short[surf], full[surf])
+    vol = ~surf
+    if vol.any():
+        d = short[vol] - full[vol]
+        say("volume rows: rel L2 %.3g, max %.3g" % (np.linalg.norm(d) / np.linalg.norm(full[vol]),
+                                                     np.abs(d).max() / np.abs(full[vol]).max()))
+        assert np.linalg.norm(d) <= 1e-3 * np.linalg.norm(full[vol]) + 1e-7
+        assert np.abs(d).max() <= 1e-2 * np.abs(full[vol]).max() + 1e-7
+    if n <= 5000:
+        ref = R.predict(flat.astype(np.float64), x.astype(np.float64), bf=True)
+        assert np.linalg.norm(short - ref) / np.linalg.norm(ref) < 1e-2
 
 
 @pytest.mark.parametrize("n", [256, 4096, 20000])
