@@ -182,6 +182,10 @@ def rel_rmse(a, b, exposure=None):
     return float(np.sqrt(((a.astype(np.float64) - b) ** 2).mean()) / np.abs(b).mean())
 
 
+FLOP_FULL = 2 * (216 * 128 + 3 * 128 * 128 + 128 * 3)    # 154,368
+FLOP_VOLUME = 2 * (144 * 128 + 3 * 128 * 128 + 128 * 3)  # 135,936
+
+
 def northstar_cfg():
     """The 1080p workload in the north-star precision form: photon power and
     direction stored BF16 (positions and split planes FP32), FP64
@@ -240,7 +244,9 @@ def run_northstar(steps, train_frames, stride, thresholds, batch, rows, near_gro
             frame()
             more_rows.append(rgb.download(np.float32, npx * 3).reshape(cfg.height, cfg.width, 3)[rows].copy())
         m = np.mean(np.array(per), axis=0)
-        infer_flop = 154368.0 * cs.records
+        # algorithmic FLOPs of the network per row: 2 x (216 x 128 + 3 x 128^2 + 128 x 3) on the full
+        # layer 1; 2 x (144 x 128 + ...) for volume rows, whose 72 normal features are constants
+        infer_flop = FLOP_FULL * (cs.inferred - cs.inferred_volume) + FLOP_VOLUME * cs.inferred_volume
         out.append({
             "threshold": th,
             "ms_per_frame": round((t1 - t0) / steps * 1e3, 3),
@@ -252,7 +258,8 @@ def run_northstar(steps, train_frames, stride, thresholds, batch, rows, near_gro
             "mlp_inference_mfma": {"bound": "mfma", "achieved": round(infer_flop / (m[2] * 1e-3) / 1e12, 2),
                                    "peak": 2500.0, "unit": "TFLOP/s",
                                    "frac": round(infer_flop / (m[2] * 1e-3) / 1e12 / 2500.0, 4),
-                                   "flop_per_query": 154368},
+                                   "rows": int(cs.inferred), "rows_volume_layer1": int(cs.inferred_volume),
+                                   "flop_per_row": {"full": FLOP_FULL, "volume": FLOP_VOLUME}},
             "_rows": img[rows].copy(),
             "_more_rows": more_rows,
         })

@@ -191,6 +191,8 @@ int cache_predict_dev(pm_context* c, const float* in, uint32_t n, float* rgb, co
   CacheState& k = c->cache;
   if ((r = sync_ema(c)) != PM_OK) return r;
   const MlpDev wi = mlp_inference_net(k.w);
+  k.infer_rows = list ? nlist : n;
+  k.infer_short = 0;
   if (k.fused_infer) {
     int maxlds = 0;
     HIPCHK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
@@ -198,6 +200,7 @@ int cache_predict_dev(pm_context* c, const float* in, uint32_t n, float* rgb, co
       const bool split = k.fused_infer == 4;
       if (list) {
         HIPCHK(mlp_infer_fused(wi, in, nlist, rgb, c->stream, list, split));
+        k.infer_short = split ? nlist : 0;
         return PM_OK;
       }
       if (!split || n == 0) {
@@ -215,6 +218,7 @@ int cache_predict_dev(pm_context* c, const float* in, uint32_t n, float* rgb, co
         return r;
       if (nk[0]) HIPCHK(mlp_infer_fused(wi, in, nk[0], rgb, c->stream, k.row_list.as<uint32_t>(), false));
       if (nk[1]) HIPCHK(mlp_infer_fused(wi, in, nk[1], rgb, c->stream, k.row_list.as<uint32_t>() + nk[0], true));
+      k.infer_short = nk[1];
       return PM_OK;
     }
     if ((size_t)maxlds >= mlp_infer_lds_bytes()) {
@@ -414,11 +418,13 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   HIPCHK(launch_cache_record(rl, st));
   HIPCHK(exclusive_scan(k.rec_cnt.as<uint32_t>(), k.rec_off.as<uint32_t>(), npix, c->scratch.as<uint32_t>(), nullptr,
                         st));
-  std::vector<uint32_t> cnt(npix);
-  HIPCHK(hipMemcpyAsync(cnt.data(), k.rec_cnt.p, 4ull * npix, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  // the record total in 64 bits: record offsets are 32-bit
+  HIPCHK(c->small.ensure(64));
+  unsigned long long* ntot_dev = (unsigned long long*)(c->small.as<uint32_t>() + 14);
+  HIPCHK(launch_count_total(k.rec_cnt.as<uint32_t>(), npix, ntot_dev, st));
   uint64_t N = 0;
-  for (uint32_t i = 0; i < npix; ++i) N += cnt[i];
+  HIPCHK(hipMemcpyAsync(&N, ntot_dev, 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
   if (N >= 0xffffffffull) return PM_ERR_CAPACITY;
   HIPCHK(k.rec_in.ensure(4ull * MLP_IN * std::max<uint64_t>(N, 1)));
   HIPCHK(k.rec_wt.ensure(12ull * std::max<uint64_t>(N, 1)));
@@ -530,6 +536,8 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   k.st.ms_accumulate = ev_ms(ev[4], ev[5]);
   k.st.ms_total = ev_ms(ev[0], ev[5]);
   k.st.frame = k.frame;
+  k.st.inferred = k.infer_rows;
+  k.st.inferred_volume = (uint32_t)k.infer_short;
   return PM_OK;
 }
 

@@ -386,22 +386,32 @@ __global__ void __launch_bounds__(LIST_THREADS) k_list_count(const uint16_t* __r
   }
 }
 
+// The block's entries of both lists are staged in LDS (list 0's, then list
+// 1's: at most LIST_TILE in all) and written out as two contiguous runs.
 __global__ void __launch_bounds__(LIST_THREADS) k_list_scatter(const uint16_t* __restrict__ code, uint64_t n,
                                                                CacheListRule rule, const uint32_t* __restrict__ offs,
                                                                uint32_t* __restrict__ list) {
   __shared__ uint32_t lw[LIST_THREADS / 64];
-  const uint64_t base = (uint64_t)blockIdx.x * LIST_TILE + (uint64_t)threadIdx.x * LIST_ITEMS;
+  __shared__ uint32_t stage[LIST_TILE];
+  const uint64_t tile0 = (uint64_t)blockIdx.x * LIST_TILE;
+  const uint64_t base = tile0 + (uint64_t)threadIdx.x * LIST_ITEMS;
   uint16_t v[LIST_ITEMS];
   load_codes(code, n, base, v);
   uint32_t tot;
   const uint32_t ex = list_block_scan(class_counts(v, n, base, rule), lw, &tot);
-  uint32_t o0 = offs[blockIdx.x] + (ex & 0xffffu), o1 = offs[gridDim.x + blockIdx.x] + (ex >> 16);
+  const uint32_t n0 = tot & 0xffffu, n1 = tot >> 16;
+  uint32_t l0 = ex & 0xffffu, l1 = n0 + (ex >> 16);
 #pragma unroll
   for (int r = 0; r < LIST_ITEMS; ++r) {
     const int k = base + r < n ? record_class(v[r], rule) : -1;
-    if (k == 0) list[o0++] = (uint32_t)(base + r);
-    if (k == 1) list[o1++] = (uint32_t)(base + r);
+    const uint32_t local = (uint32_t)(threadIdx.x * LIST_ITEMS + r);  // < LIST_TILE: fits the record index
+    if (k == 0) stage[l0++] = local;
+    if (k == 1) stage[l1++] = local;
   }
+  __syncthreads();
+  const uint32_t o0 = offs[blockIdx.x], o1 = offs[gridDim.x + blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < n0; i += LIST_THREADS) list[o0 + i] = (uint32_t)(tile0 + stage[i]);
+  for (uint32_t i = threadIdx.x; i < n1; i += LIST_THREADS) list[o1 + i] = (uint32_t)(tile0 + stage[n0 + i]);
 }
 
 // acc += sum of v[0..n) in a fixed order (one workgroup): the frame's training
@@ -491,6 +501,29 @@ hipError_t launch_err_accum(const float* in, const float* row_err, uint32_t rows
                             unsigned int* ecnt, hipStream_t st) {
   if (rows == 0) return hipSuccess;
   hipLaunchKernelGGL(k_err_accum, dim3((rows + 255) / 256), dim3(256), 0, st, in, row_err, rows, esum, ecnt);
+  return hipGetLastError();
+}
+
+// *total += the sum of the n per-pixel record counts, in 64 bits: one 64-bit
+// atomic add per workgroup (integer sums: any order gives the same total).
+__global__ void __launch_bounds__(256) k_count_total(const uint32_t* __restrict__ cnt, uint32_t n,
+                                                     unsigned long long* __restrict__ total) {
+  __shared__ unsigned long long part[256 / 64];
+  unsigned long long s = 0ull;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) s += cnt[i];
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d);
+  if ((threadIdx.x & 63u) == 0u) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(total, part[0] + part[1] + part[2] + part[3]);
+}
+
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+hipError_t launch_count_total(const uint32_t* cnt, uint32_t n, unsigned long long* total, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, total);
+  const uint32_t blocks = std::max(1u, std::min(1024u, (n + 2047u) / 2048u));
+  hipLaunchKernelGGL(k_count_total, dim3(blocks), dim3(256), 0, st, cnt, n, total);
   return hipGetLastError();
 }
 

@@ -280,6 +280,8 @@ hipError_t launch_cache_select(const float* in, const uint32_t* list_s, uint32_t
 hipError_t launch_err_accum(const float* in, const float* row_err, uint32_t rows, unsigned long long* esum,
                             unsigned int* ecnt, hipStream_t st);
 hipError_t launch_sum_rows(const float* v, uint32_t n, double* acc, hipStream_t st);
+// *total = cnt[0] + ... + cnt[n - 1] in 64 bits
+hipError_t launch_count_total(const uint32_t* cnt, uint32_t n, unsigned long long* total, hipStream_t st);
 // cell_fb[c] = 1 when error cell c saw no training row or its mean squared
 // log error exceeds the threshold (CACHE_CELLS bytes).
 hipError_t launch_cache_cell_fb(const unsigned long long* esum, const unsigned int* ecnt, float threshold,

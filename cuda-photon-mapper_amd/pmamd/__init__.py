@@ -126,7 +126,8 @@ class PmCacheStats(ctypes.Structure):
         ("ms_accumulate", ctypes.c_double),
         ("ms_total", ctypes.c_double),
         ("frame", ctypes.c_uint32),
-        ("pad0", ctypes.c_uint32),
+        ("inferred_volume", ctypes.c_uint32),
+        ("inferred", ctypes.c_uint64),
     ]
 
 

@@ -15,6 +15,7 @@
 #ifndef PHOTON_MAPPER_H
 #define PHOTON_MAPPER_H
 
+#include <stdbool.h> /* the reference launchers' bool flags, also for C (cgo) callers */
 #include <stddef.h>
 #include <stdint.h>
 
@@ -313,7 +314,8 @@ typedef struct pm_cache_stats {
   double train_loss;       /* mean loss of this frame's training steps */
   double ms_record, ms_train, ms_infer, ms_fallback, ms_accumulate, ms_total;
   uint32_t frame;          /* frames rendered with this cache */
-  uint32_t pad0;
+  uint32_t inferred_volume; /* of the inferred rows, volume records on the 5-step layer 1 */
+  uint64_t inferred;       /* rows through the network (the volume records when surface records are not cached) */
 } pm_cache_stats;
 
 /* Render the crop (x0, y0, w, h) of the configured frame through the cache:

@@ -5,6 +5,10 @@ validation works without a GPU."""
 import ctypes
 import os
 import re
+import shutil
+import subprocess
+
+import pytest
 
 import pmamd
 
@@ -69,3 +73,30 @@ def test_struct_layout_matches_c():
     # sizeof checks: the ctypes mirror must match the C layout of the header.
     assert ctypes.sizeof(pmamd.PmSphere) == 36
     assert ctypes.sizeof(pmamd.PmStats) == 24 + 12 * 8 + 4 * 8 + 16
+
+
+def test_header_compiles_as_c_with_the_mirror_layout(tmp_path):
+    """The header is plain C99 (what a cgo or C caller includes), and every
+    struct's size and every field's offset equal the ctypes mirror's."""
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no gcc")
+    structs = {"pm_sphere": pmamd.PmSphere, "pm_config": pmamd.PmConfig, "pm_stats": pmamd.PmStats,
+               "pm_cache_stats": pmamd.PmCacheStats, "pm_batch_tuning": pmamd.PmBatchTuning,
+               "pm_batch_stats": pmamd.PmBatchStats}
+    src = ["#include <stdio.h>", "#include <stddef.h>", '#include "photon_mapper.h"', "int main(void) {"]
+    want = []
+    for cname, py in structs.items():
+        src.append('  printf("%%zu\\n", sizeof(%s));' % cname)
+        want.append(ctypes.sizeof(py))
+        for f in py._fields_:
+            src.append('  printf("%%zu\\n", offsetof(%s, %s));' % (cname, f[0]))
+            want.append(getattr(py, f[0]).offset)
+    src.append("  return 0;\n}")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(REPO, "include")
+    subprocess.run([gcc, "-std=c99", "-Wall", "-Werror", "-I", inc, "-o", str(exe), str(c)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == want
