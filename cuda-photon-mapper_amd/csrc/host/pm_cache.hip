@@ -398,9 +398,10 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
       for (int i = 0; i < 7; ++i) (void)hipEventDestroy(e[i]);
     }
   } guard{ev};
-  const uint32_t npix = (uint32_t)w * (uint32_t)h;
+  // record-order slots of the crop (4x4 pixel blocks, padded)
+  const uint32_t npix = (uint32_t)cache_order_size(w, h);
   HIPCHK(hipEventRecord(ev[0], st));
-  // 1-3: record the frame's queries in pixel order
+  // 1-3: record the frame's queries in record order (4x4 pixel blocks)
   HIPCHK(k.rec_cnt.ensure(4ull * npix));
   HIPCHK(k.rec_off.ensure(4ull * npix));
   HIPCHK(c->scratch.ensure(4ull * scan_temp_words(npix)));
@@ -521,7 +522,7 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   }
   HIPCHK(hipEventRecord(ev[4], st));
   // 7: per-pixel accumulation
-  HIPCHK(launch_cache_accumulate(k.rec_off.as<uint32_t>(), k.rec_cnt.as<uint32_t>(), npix, k.rec_wt.as<float>(),
+  HIPCHK(launch_cache_accumulate(k.rec_off.as<uint32_t>(), k.rec_cnt.as<uint32_t>(), w, h, k.rec_wt.as<float>(),
                                  k.rec_val.as<float>(), c->rp.spp, c->cfg.exposure, rgb_dev, (uchar4*)out_dev, st));
   HIPCHK(hipEventRecord(ev[5], st));
   HIPCHK(hipStreamSynchronize(st));

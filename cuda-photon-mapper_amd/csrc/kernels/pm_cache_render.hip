@@ -147,15 +147,19 @@ struct RecordCount : RecordGather {
 };
 
 // One lane per crop pixel, 8x8 tiles per wave (same pixel order as the lane
-// render).  COUNT: the counting pass (r.write == 0).
+// render); counts and offsets by record-order slot.  COUNT: the counting pass
+// (r.write == 0), which also clears the padding slots of partial blocks.
 template <bool FP64T, bool COUNT>
 __global__ void __launch_bounds__(64) k_rec(CacheRecordLaunch r) {
   const int lane = lane_id();
   const int tiles_x = (r.w + 7) / 8;
   const int tile = (int)blockIdx.x;
   const int lx = (tile % tiles_x) * 8 + (lane & 7), ly = (tile / tiles_x) * 8 + (lane >> 3);
-  if (lx >= r.w || ly >= r.h) return;
-  const uint32_t pidx = (uint32_t)ly * (uint32_t)r.w + (uint32_t)lx;
+  if (lx >= r.w || ly >= r.h) {
+    if (!r.write && lx < (r.w + 3) / 4 * 4 && ly < (r.h + 3) / 4 * 4) r.counts[cache_pixel_order(lx, ly, r.w)] = 0u;
+    return;
+  }
+  const uint32_t pidx = cache_pixel_order(lx, ly, r.w);
   typename std::conditional<COUNT, RecordCount, RecordGather>::type g;
   g.write = r.write != 0;
   g.n = 0;
@@ -430,14 +434,15 @@ __global__ void __launch_bounds__(256) k_sum_rows(const float* __restrict__ v, u
 }
 
 // Per pixel: compensated sum over its records of weight * value, / spp.
-__global__ void k_accumulate(const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t npix,
+__global__ void k_accumulate(const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, int w, int h,
                              const float* __restrict__ wt, const float* __restrict__ val, int spp, float exposure,
                              float* __restrict__ rgb, uchar4* __restrict__ u8) {
-  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npix) return;
+  const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= (uint32_t)w * (uint32_t)h) return;
+  const uint32_t o = cache_pixel_order((int)(p % (uint32_t)w), (int)(p / (uint32_t)w), w);
   NSum3<float> acc;
   nsum3_init(acc);
-  const size_t b = off[p], e = b + cnt[p];
+  const size_t b = off[o], e = b + cnt[o];
   size_t i = b;
   // four records' loads issued before their adds (the adds stay in record order)
   for (; i + 4 <= e; i += 4) {
@@ -564,10 +569,11 @@ hipError_t launch_cache_list_scatter(const uint16_t* code, uint64_t n, const Cac
   return hipGetLastError();
 }
 
-hipError_t launch_cache_accumulate(const uint32_t* off, const uint32_t* cnt, uint32_t npix, const float* wt,
+hipError_t launch_cache_accumulate(const uint32_t* off, const uint32_t* cnt, int w, int h, const float* wt,
                                    const float* val, int spp, float exposure, float* rgb, uchar4* u8, hipStream_t st) {
+  const uint32_t npix = (uint32_t)w * (uint32_t)h;
   if (npix == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_accumulate, dim3((npix + 255) / 256), dim3(256), 0, st, off, cnt, npix, wt, val, spp, exposure,
+  hipLaunchKernelGGL(k_accumulate, dim3((npix + 255) / 256), dim3(256), 0, st, off, cnt, w, h, wt, val, spp, exposure,
                      rgb, u8);
   return hipGetLastError();
 }

@@ -240,13 +240,23 @@ hipError_t launch_render_batch(const BatchRenderLaunch& r, int fp64, int cone, i
 
 // learned-cache render (pm_cache_render.hip)
 enum : int { CACHE_GRID = 16 };  // error-tracker cells per axis (x 2 kinds)
+// Record order of a crop's pixels: 4x4 blocks in row-major block order,
+// row-major inside a block, so that consecutive records (and the rows of a
+// wave in the record lists) come from a compact patch of the image.  The
+// per-pixel counts and offsets are indexed by this order over the crop padded
+// to whole blocks (cache_order_size slots; padding slots hold no records).
+PM_HD inline uint32_t cache_pixel_order(int x, int y, int w) {
+  const uint32_t bx = (uint32_t)((w + 3) / 4);
+  return ((uint32_t)(y >> 2) * bx + (uint32_t)(x >> 2)) * 16u + (uint32_t)((y & 3) * 4 + (x & 3));
+}
+PM_HD inline size_t cache_order_size(int w, int h) { return (size_t)((w + 3) / 4) * (size_t)((h + 3) / 4) * 16u; }
 struct CacheRecordLaunch {
   Scene sc;
   RenderParams rp;
   int x0, y0, w, h;
   int write;                 // 0: count queries per pixel, 1: write records
-  uint32_t* counts;          // [w*h]
-  const uint32_t* offsets;   // [w*h]
+  uint32_t* counts;          // [cache_order_size(w, h)], by cache_pixel_order
+  const uint32_t* offsets;   // [cache_order_size(w, h)]
   float* in;                 // [N x 9]
   float* wt;                 // [N x 3]
   uint16_t* code;            // [N] record codes (error cell, kind, near-light bit; record_code)
@@ -297,7 +307,9 @@ hipError_t launch_cache_list_count(const uint16_t* code, uint64_t n, const Cache
                                    hipStream_t st);
 hipError_t launch_cache_list_scatter(const uint16_t* code, uint64_t n, const CacheListRule& rule,
                                      const uint32_t* offs, uint32_t* list, hipStream_t st);
-hipError_t launch_cache_accumulate(const uint32_t* off, const uint32_t* cnt, uint32_t npix, const float* wt,
+// Per crop pixel (row-major, w x h): the records [off[o], off[o] + cnt[o]) of
+// its record-order slot o = cache_pixel_order(x, y, w).
+hipError_t launch_cache_accumulate(const uint32_t* off, const uint32_t* cnt, int w, int h, const float* wt,
                                    const float* val, int spp, float exposure, float* rgb, uchar4* u8, hipStream_t st);
 
 }  // namespace pm
