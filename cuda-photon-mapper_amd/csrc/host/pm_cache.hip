@@ -267,8 +267,7 @@ int cache_train_dev(pm_context* c, const float* in, const float* target, uint32_
     MlpTrainBufs tb = k.tb;
     tb.row_err = k.row_err.as<float>() + ro;
     HIPCHK(mlp_train_grads(k.w, in + (size_t)o * MLP_IN, batch, k.X.as<uint16_t>(), H, k.y.as<float>(),
-                           target + (size_t)o * MLP_OUT, tb, tt, false, c->stream));
-    if (esum) HIPCHK(launch_err_accum(in + (size_t)o * MLP_IN, tb.row_err, batch, esum, ecnt, c->stream));
+                           target + (size_t)o * MLP_OUT, tb, tt, false, c->stream, esum, ecnt));
     k.step += 1;
     HIPCHK(mlp_adam(k.w, tb, lr, k.step, c->stream, &tt));
     ro += batch;

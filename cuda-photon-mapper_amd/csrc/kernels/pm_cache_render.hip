@@ -12,7 +12,8 @@
 //                       targets (k_values7 under gather modes 7 and auto,
 //                       k_values otherwise) and trains the MLP (pm_mlp.hip);
 //                       each batch's pre-update errors are binned into a
-//                       16^3 x {surface, volume} grid (k_err_accum)
+//                       16^3 x {surface, volume} grid (by the fused forward
+//                       pass, k_fwd_fused in pm_mlp_train.hip)
 //   5. inference      : MLP over the volume records (all records when surface
 //                       records are cached too; pm_mlp_infer.hip)
 //   6. fallback       : records in cells whose mean training error exceeds
@@ -28,19 +29,6 @@
 #include "pm_mlp.h"
 
 namespace pm {
-
-// The error-tracker cell of a record at (x, y, z): its 16^3 cell (clamped to
-// the grid), plus 4096 for volume records.
-__device__ __forceinline__ uint32_t err_cell_of(float x, float y, float z, bool surf) {
-  const float p[3] = {x, y, z};
-  int c[3];
-  for (int a = 0; a < 3; ++a) {
-    float v = p[a] * (float)CACHE_GRID;
-    int iv = (int)v;
-    c[a] = iv < 0 ? 0 : (iv >= CACHE_GRID ? CACHE_GRID - 1 : iv);
-  }
-  return (uint32_t)((((surf ? 0 : 1) * CACHE_GRID + c[2]) * CACHE_GRID + c[1]) * CACHE_GRID + c[0]);
-}
 
 // A record's code, everything the fallback rule reads of it: its error cell
 // (bits 0-12; bit 12 set for volume records) and, in bit 13, whether a volume
@@ -276,26 +264,11 @@ __global__ void k_select(const float* __restrict__ in, const uint32_t* __restric
   for (int c = 0; c < MLP_IN; ++c) out[(size_t)j * MLP_IN + c] = in[idx * MLP_IN + c];
 }
 
-__device__ __forceinline__ uint32_t err_cell(const float* p) {
-  return err_cell_of(p[0], p[1], p[2], !(p[6] == 0.0f && p[7] == 0.0f && p[8] == 0.0f));
-}
-
 __global__ void k_row_codes(const float* __restrict__ in, uint64_t n, uint16_t* __restrict__ code) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) code[i] = (uint16_t)err_cell(in + i * MLP_IN);
 }
 
-// Bins per-row squared errors; integer (fixed-point 2^-24) sums are order-independent.
-__global__ void k_err_accum(const float* __restrict__ in, const float* __restrict__ row_err, uint32_t rows,
-                            unsigned long long* __restrict__ esum, unsigned int* __restrict__ ecnt) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= rows) return;
-  uint32_t cell = err_cell(in + (size_t)i * MLP_IN);
-  float e = row_err[i];
-  e = e < 1e6f ? e : 1e6f;
-  atomicAdd(&esum[cell], (unsigned long long)(e * 16777216.0f));
-  atomicAdd(&ecnt[cell], 1u);
-}
 
 // Fallback decision per error cell: no training row in the cell, or a mean
 // (over its training rows and channels) squared log error above the threshold.
@@ -502,12 +475,6 @@ hipError_t launch_cache_select(const float* in, const uint32_t* list_s, uint32_t
   return hipGetLastError();
 }
 
-hipError_t launch_err_accum(const float* in, const float* row_err, uint32_t rows, unsigned long long* esum,
-                            unsigned int* ecnt, hipStream_t st) {
-  if (rows == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_err_accum, dim3((rows + 255) / 256), dim3(256), 0, st, in, row_err, rows, esum, ecnt);
-  return hipGetLastError();
-}
 
 // *total += the sum of the n per-pixel record counts, in 64 bits: one 64-bit
 // atomic add per workgroup (integer sums: any order gives the same total).

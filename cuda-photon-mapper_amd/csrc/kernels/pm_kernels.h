@@ -250,6 +250,22 @@ PM_HD inline uint32_t cache_pixel_order(int x, int y, int w) {
   return ((uint32_t)(y >> 2) * bx + (uint32_t)(x >> 2)) * 16u + (uint32_t)((y & 3) * 4 + (x & 3));
 }
 PM_HD inline size_t cache_order_size(int w, int h) { return (size_t)((w + 3) / 4) * (size_t)((h + 3) / 4) * 16u; }
+// The error-tracker cell of a record at (x, y, z): its 16^3 cell (clamped to
+// the grid), plus 4096 for volume records.
+PM_HD inline uint32_t err_cell_of(float x, float y, float z, bool surf) {
+  const float p[3] = {x, y, z};
+  int c[3];
+  for (int a = 0; a < 3; ++a) {
+    float v = p[a] * (float)CACHE_GRID;
+    int iv = (int)v;
+    c[a] = iv < 0 ? 0 : (iv >= CACHE_GRID ? CACHE_GRID - 1 : iv);
+  }
+  return (uint32_t)((((surf ? 0 : 1) * CACHE_GRID + c[2]) * CACHE_GRID + c[1]) * CACHE_GRID + c[0]);
+}
+// The cell of a query row (position, direction, normal or 0).
+PM_HD inline uint32_t err_cell(const float* p) {
+  return err_cell_of(p[0], p[1], p[2], !(p[6] == 0.0f && p[7] == 0.0f && p[8] == 0.0f));
+}
 struct CacheRecordLaunch {
   Scene sc;
   RenderParams rp;
@@ -287,8 +303,6 @@ hipError_t launch_cache_record(const CacheRecordLaunch& r, hipStream_t st);
 hipError_t launch_cache_values(const CacheValueLaunch& v, hipStream_t st);
 hipError_t launch_cache_select(const float* in, const uint32_t* list_s, uint32_t n_s, const uint32_t* list_v,
                                uint32_t n_v, uint32_t count, uint32_t seed, uint32_t frame, float* out, hipStream_t st);
-hipError_t launch_err_accum(const float* in, const float* row_err, uint32_t rows, unsigned long long* esum,
-                            unsigned int* ecnt, hipStream_t st);
 hipError_t launch_sum_rows(const float* v, uint32_t n, double* acc, hipStream_t st);
 // *total = cnt[0] + ... + cnt[n - 1] in 64 bits
 hipError_t launch_count_total(const uint32_t* cnt, uint32_t n, unsigned long long* total, hipStream_t st);

@@ -85,9 +85,12 @@ size_t mlp_train_partial_floats(uint32_t rows);
 // inputs / targets: y the outputs, tb.row_err the per-row squared errors, and
 // the gradients as split partials (tt.part_w / part_b); with `reduce` they are
 // also summed into tb.gw / tb.gb (mlp_adam can instead sum them itself).
+// With esum / ecnt, each row's squared error (capped at 1e6, in 2^-24 fixed
+// point) is also added to the error tracker's cell of its input (err_cell),
+// and the cell's row count incremented.
 hipError_t mlp_train_grads(const MlpDev& w, const float* in, uint32_t rows, uint16_t* X, uint16_t* const* H,
                            float* y, const float* target, MlpTrainBufs& tb, MlpTrainT& tt, bool reduce,
-                           hipStream_t st);
+                           hipStream_t st, unsigned long long* esum = nullptr, unsigned int* ecnt = nullptr);
 
 hipError_t mlp_encode(const float* in, uint32_t n, uint16_t* out, hipStream_t st);
 hipError_t mlp_forward(const MlpDev& w, const uint16_t* X, uint32_t rows, uint16_t* const* H, float* y,

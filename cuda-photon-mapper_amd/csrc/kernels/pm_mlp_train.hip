@@ -1,6 +1,7 @@
 // pm_mlp_train.hip -- the learned cache's training pass (forward with stored
-// activations, backward, weight and bias gradients) on gfx950, built from one
-// "NT" GEMM kernel whose operands are both contiguous along K.
+// activations, backward, weight and bias gradients) on gfx950: the forward
+// pass and loss gradient in one fused kernel (k_fwd_fused), the backward
+// products on one "NT" GEMM kernel whose operands are both contiguous along K.
 //
 // No reference counterpart (the reference has no learned component).
 //
@@ -290,27 +291,173 @@ __global__ void k_encode_t(const float* __restrict__ in, uint32_t n, uint16_t* _
     }
 }
 
-// Loss gradient of k_loss_grad, also written transposed (dy^T [OUTPAD x ld]).
-__global__ void k_loss_grad_t(const float* __restrict__ y, const float* __restrict__ target, int rows,
-                              uint16_t* __restrict__ dy, uint16_t* __restrict__ dyT, uint32_t ld,
-                              float* __restrict__ row_err) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= rows) return;
-  float e2 = 0.0f;
-  const float inv = 2.0f / (float)(rows * MLP_OUT);
-  for (int c = 0; c < MLP_OUTPAD; ++c) {
-    float gr = 0.0f;
-    if (c < MLP_OUT) {
-      const float t = log_(1.0f + fmax_(target[(size_t)r * MLP_OUT + c], 0.0f));
-      const float d = y[(size_t)r * MLP_OUTPAD + c] - t;
-      e2 += d * d;
-      gr = d * inv;
-    }
-    const uint16_t h = f32_to_bf16(gr);
-    dy[(size_t)r * MLP_OUTPAD + c] = h;
-    dyT[(size_t)c * ld + r] = h;
+// ---------------------------------------------------------------- fused forward
+// The forward pass of a training batch in one launch: a workgroup takes 64
+// rows through the five layers with the activations in LDS, writes each
+// hidden layer's output row-major (H_l) and transposed (H_l^T), then the
+// output layer's y and the loss gradient: with t = log(1 + max(target, 0)),
+// d = y - t per channel, dy = 2 d / (3 rows) (BF16, also transposed; zero in
+// the padding columns) and row_err = sum of d^2.  When esum is set, each
+// row's row_err (capped at 1e6) is added in 2^-24 fixed point to the error
+// tracker's cell of its input, and the cell's count incremented (integer
+// atomics: the bins do not depend on the order).  Every
+// product is k_gemm_nt's (the same MFMA operands, K in ascending steps of 32
+// from a zero accumulator, then bias, ReLU and BF16 rounding), so the outputs
+// equal those of one launch per layer bit for bit.  A wave computes the
+// tile's 64 rows x 32 of a hidden layer's 128 outputs, or 16 rows of the 16
+// output-layer columns.
+constexpr int FF_XS = MLP_KPAD + 8;  // LDS row length (bf16) of X and W1
+constexpr int FF_HS = MLP_HID + 8;   // of the hidden activations and W2..W5
+constexpr size_t FF_LDS = 2u * (64u * FF_XS + 2u * 64u * FF_HS + 128u * FF_XS) + 4u * 64u * MLP_OUTPAD;
+
+struct FwdArgs {
+  const uint16_t* X;  // [rows x KPAD] encoded inputs
+  uint32_t rows;
+  const uint16_t* w[MLP_LAYERS];
+  const float* b[MLP_LAYERS];
+  uint16_t* H[MLP_HIDDEN_LAYERS];   // [rows x HID]
+  uint16_t* HT[MLP_HIDDEN_LAYERS];  // [HID x ld]
+  uint32_t ld;
+  float* y;                         // [rows x OUTPAD]
+  const float* target;              // [rows x OUT]
+  uint16_t* dy;                     // [rows x OUTPAD]
+  uint16_t* dyT;                    // [OUTPAD x ld]
+  float* row_err;                   // [rows]
+  const float* in;                  // [rows x IN] query rows (error-tracker cells)
+  unsigned long long* esum;         // error tracker (null: none)
+  unsigned int* ecnt;
+};
+
+__global__ void __launch_bounds__(256) k_fwd_fused(FwdArgs A) {
+  extern __shared__ __align__(16) unsigned char fl[];
+  uint16_t* Xs = (uint16_t*)fl;                  // [64][FF_XS]
+  uint16_t* Hs[2] = {Xs + 64 * FF_XS, Xs + 64 * FF_XS + 64 * FF_HS};  // [64][FF_HS] each
+  uint16_t* Ws = Xs + 64 * FF_XS + 2 * 64 * FF_HS;  // [128][FF_XS] or [nout][FF_HS]
+  float* ys = (float*)(Ws + 128 * FF_XS);        // [64][OUTPAD]
+  const int tid = (int)threadIdx.x, lane = lane_id(), wave = tid >> 6, r16 = lane & 15, q = lane >> 4;
+  const uint32_t m0 = blockIdx.x * 64u;
+  const uint32_t rows = A.rows;
+  // X rows of the tile (zero past the batch)
+  for (int v = tid; v < 64 * (MLP_KPAD / 8); v += 256) {
+    const int r = v / (MLP_KPAD / 8), kk = (v % (MLP_KPAD / 8)) * 8;
+    uint4 val = make_uint4(0u, 0u, 0u, 0u);
+    if (m0 + (uint32_t)r < rows) val = *(const uint4*)&A.X[(size_t)(m0 + r) * MLP_KPAD + kk];
+    *(uint4*)&Xs[r * FF_XS + kk] = val;
   }
-  row_err[r] = e2;
+  const uint16_t* act = Xs;
+  int as = FF_XS, kin = MLP_KPAD;
+  for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) {
+    const int ws = l == 0 ? FF_XS : FF_HS;
+    __syncthreads();  // the previous layer is done with Ws (and its output tile is complete)
+    for (int v = tid; v < MLP_HID * (kin / 8); v += 256) {
+      const int r = v / (kin / 8), kk = (v % (kin / 8)) * 8;
+      *(uint4*)&Ws[r * ws + kk] = *(const uint4*)&A.w[l][(size_t)r * kin + kk];
+    }
+    __syncthreads();
+    f32x4_t acc[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int ks = 0; ks < kin; ks += 32) {
+      bf16x8_t af[4], bfr[2];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const uint4 v = *(const uint4*)&act[(16 * a + r16) * as + ks + 8 * q];
+        af[a] = *(const bf16x8_t*)&v;
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const uint4 v = *(const uint4*)&Ws[(32 * wave + 16 * b + r16) * ws + ks + 8 * q];
+        bfr[b] = *(const bf16x8_t*)&v;
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+    }
+    // lane holds C[16a + 4q + e][32 wave + 16b + r16]
+    uint16_t* out = Hs[l & 1];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int n = 32 * wave + 16 * b + r16;
+      const float bias = A.b[l][n];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          float v = acc[a][b][e] + bias;
+          v = v > 0.0f ? v : 0.0f;
+          out[(16 * a + 4 * q + e) * FF_HS + n] = f32_to_bf16(v);
+        }
+    }
+    __syncthreads();
+    // H_l rows of the batch, and H_l^T (64 columns of each of its 128 rows)
+    for (int v = tid; v < 64 * (MLP_HID / 8); v += 256) {
+      const int r = v / (MLP_HID / 8), j = (v % (MLP_HID / 8)) * 8;
+      if (m0 + (uint32_t)r < rows) *(uint4*)&A.H[l][(size_t)(m0 + r) * MLP_HID + j] = *(const uint4*)&out[r * FF_HS + j];
+    }
+    for (int v = tid; v < MLP_HID * 8; v += 256) {
+      const int n = v >> 3, j = (v & 7) * 8;
+      uint32_t p[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        p[t] = (uint32_t)out[(j + 2 * t) * FF_HS + n] | ((uint32_t)out[(j + 2 * t + 1) * FF_HS + n] << 16);
+      *(uint4*)&A.HT[l][(size_t)n * A.ld + m0 + j] = make_uint4(p[0], p[1], p[2], p[3]);
+    }
+    act = out;
+    as = FF_HS;
+    kin = MLP_HID;
+  }
+  // output layer: 16 columns (3 used), no ReLU, FP32 y
+  __syncthreads();
+  for (int v = tid; v < MLP_OUTPAD * (MLP_HID / 8); v += 256) {
+    const int r = v / (MLP_HID / 8), kk = (v % (MLP_HID / 8)) * 8;
+    *(uint4*)&Ws[r * FF_HS + kk] = *(const uint4*)&A.w[MLP_HIDDEN_LAYERS][(size_t)r * MLP_HID + kk];
+  }
+  __syncthreads();
+  {
+    f32x4_t acc = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int ks = 0; ks < MLP_HID; ks += 32) {
+      const uint4 va = *(const uint4*)&act[(16 * wave + r16) * FF_HS + ks + 8 * q];
+      const uint4 vb = *(const uint4*)&Ws[r16 * FF_HS + ks + 8 * q];
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*(const bf16x8_t*)&va, *(const bf16x8_t*)&vb, acc, 0, 0, 0);
+    }
+    const float bias = A.b[MLP_HIDDEN_LAYERS][r16];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int m = 16 * wave + 4 * q + e;
+      const float v = acc[e] + bias;
+      ys[m * MLP_OUTPAD + r16] = v;
+      if (m0 + (uint32_t)m < rows) A.y[(size_t)(m0 + m) * MLP_OUTPAD + r16] = v;
+    }
+  }
+  __syncthreads();
+  // loss gradient and error bins, one thread per row
+  if (tid < 64 && m0 + (uint32_t)tid < rows) {
+    const uint32_t r = m0 + (uint32_t)tid;
+    float e2 = 0.0f;
+    const float inv = 2.0f / (float)((int)rows * MLP_OUT);
+    for (int c = 0; c < MLP_OUTPAD; ++c) {
+      float gr = 0.0f;
+      if (c < MLP_OUT) {
+        const float t = log_(1.0f + fmax_(A.target[(size_t)r * MLP_OUT + c], 0.0f));
+        const float d = ys[tid * MLP_OUTPAD + c] - t;
+        e2 += d * d;
+        gr = d * inv;
+      }
+      const uint16_t h = f32_to_bf16(gr);
+      A.dy[(size_t)r * MLP_OUTPAD + c] = h;
+      A.dyT[(size_t)c * A.ld + r] = h;
+    }
+    A.row_err[r] = e2;
+    if (A.esum) {
+      const uint32_t cell = err_cell(A.in + (size_t)r * MLP_IN);
+      const float e = e2 < 1e6f ? e2 : 1e6f;
+      atomicAdd(&A.esum[cell], (unsigned long long)(e * 16777216.0f));
+      atomicAdd(&A.ecnt[cell], 1u);
+    }
+  }
 }
 
 // The gradients of every layer from the split partials: gw[l][i] = sum over
@@ -357,7 +504,7 @@ uint32_t mlp_train_ld(uint32_t rows) { return (rows + 63u) / 64u * 64u; }
 
 hipError_t mlp_train_grads(const MlpDev& w, const float* in, uint32_t rows, uint16_t* X, uint16_t* const* H,
                            float* y, const float* target, MlpTrainBufs& tb, MlpTrainT& tt, bool reduce,
-                           hipStream_t st) {
+                           hipStream_t st, unsigned long long* esum, unsigned int* ecnt) {
   if (rows == 0) return hipSuccess;
   const uint32_t ld = tt.ld;
   if (ld < mlp_train_ld(rows)) return hipErrorInvalidValue;
@@ -367,32 +514,32 @@ hipError_t mlp_train_grads(const MlpDev& w, const float* in, uint32_t rows, uint
     hipLaunchKernelGGL(k_encode_t, dim3((unsigned)((th + 255) / 256)), dim3(256), 0, st, in, rows, X, tt.XT, ld);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  // forward: H_l and H_l^T, then y
-  const uint16_t* act = X;
-  int kin = MLP_KPAD;
-  for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) {
-    NtArgs a = nt_args((int)rows, MLP_HID, kin);
-    a.A = act; a.lda = kin;
-    a.B = w.wb[l]; a.ldb = kin;
-    a.bias = w.b[l];
-    a.relu = 1;
-    a.Cb = H[l]; a.ldc = MLP_HID;
-    a.Ct = tt.HT[l]; a.ldt = (int)ld;
-    if ((e = gemm_nt(a, 1, st)) != hipSuccess) return e;
-    act = H[l];
-    kin = MLP_HID;
-  }
+  // forward: H_l and H_l^T, y, the loss gradient (and the error bins), one launch
   {
-    NtArgs a = nt_args((int)rows, MLP_OUTPAD, MLP_HID);
-    a.A = act; a.lda = MLP_HID;
-    a.B = w.wb[MLP_HIDDEN_LAYERS]; a.ldb = MLP_HID;
-    a.bias = w.b[MLP_HIDDEN_LAYERS];
-    a.Cf = y; a.ldf = MLP_OUTPAD;
-    if ((e = gemm_nt(a, 1, st)) != hipSuccess) return e;
+    FwdArgs f;
+    memset(&f, 0, sizeof(f));
+    f.X = X;
+    f.rows = rows;
+    for (int l = 0; l < MLP_LAYERS; ++l) {
+      f.w[l] = w.wb[l];
+      f.b[l] = w.b[l];
+    }
+    for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) {
+      f.H[l] = H[l];
+      f.HT[l] = tt.HT[l];
+    }
+    f.ld = ld;
+    f.y = y;
+    f.target = target;
+    f.dy = tb.dy;
+    f.dyT = tt.dyT;
+    f.row_err = tb.row_err;
+    f.in = in;
+    f.esum = esum;
+    f.ecnt = ecnt;
+    hipLaunchKernelGGL(k_fwd_fused, dim3((rows + 63) / 64), dim3(256), FF_LDS, st, f);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_loss_grad_t, dim3((rows + 255) / 256), dim3(256), 0, st, y, target, (int)rows, tb.dy, tt.dyT,
-                     ld, tb.row_err);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
   // backward: per layer the split weight-gradient product, then the delta
   // of the layer below (with its transpose)
   const int slices = mlp_train_slices(rows);
