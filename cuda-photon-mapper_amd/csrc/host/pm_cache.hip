@@ -82,12 +82,8 @@ int ensure_cache(pm_context* c) {
   for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) HIPCHK(k.H[l].ensure(2ull * R * MLP_HID));
   HIPCHK(k.y.ensure(4ull * R * MLP_OUTPAD));
   HIPCHK(k.dy.ensure(2ull * R * MLP_OUTPAD));
-  HIPCHK(k.d0.ensure(2ull * R * MLP_HID));
-  HIPCHK(k.d1.ensure(2ull * R * MLP_HID));
   HIPCHK(k.row_err.ensure(4ull * R));
   k.tb.dy = k.dy.as<uint16_t>();
-  k.tb.d0 = k.d0.as<uint16_t>();
-  k.tb.d1 = k.d1.as<uint16_t>();
   k.tb.row_err = k.row_err.as<float>();
   k.rows_cap = R;
   k.ready = true;
@@ -109,12 +105,12 @@ int ensure_train_t(CacheState& k, uint32_t rows, MlpTrainT& tt) {
   const uint32_t ld = mlp_train_ld(rows);
   HIPCHK(k.XT.ensure(2ull * MLP_KPAD * ld));
   for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) HIPCHK(k.HT[l].ensure(2ull * MLP_HID * ld));
-  for (int j = 0; j < 2; ++j) HIPCHK(k.dT[j].ensure(2ull * MLP_HID * ld));
+  for (int j = 0; j < MLP_HIDDEN_LAYERS; ++j) HIPCHK(k.dT[j].ensure(2ull * MLP_HID * ld));
   HIPCHK(k.dyT.ensure(2ull * MLP_OUTPAD * ld));
   HIPCHK(k.tpart.ensure(4ull * mlp_train_partial_floats(rows)));
   tt.XT = k.XT.as<uint16_t>();
   for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) tt.HT[l] = k.HT[l].as<uint16_t>();
-  for (int j = 0; j < 2; ++j) tt.dT[j] = k.dT[j].as<uint16_t>();
+  for (int j = 0; j < MLP_HIDDEN_LAYERS; ++j) tt.dT[j] = k.dT[j].as<uint16_t>();
   tt.dyT = k.dyT.as<uint16_t>();
   tt.ld = ld;
   tt.part = k.tpart.as<float>();
@@ -140,11 +136,11 @@ uint16_t* const* hidden_ptrs(CacheState& k, uint16_t** tmp) {
 
 namespace pm {
 void cache_release(CacheState& k) {
-  for (DevBuf* b : {&k.params, &k.grads, &k.adam_m, &k.adam_v, &k.params_bf16, &k.params_ema, &k.ema_bf16, &k.X, &k.y, &k.dy, &k.d0, &k.d1,
+  for (DevBuf* b : {&k.params, &k.grads, &k.adam_m, &k.adam_v, &k.params_bf16, &k.params_ema, &k.ema_bf16, &k.X, &k.y, &k.dy,
                     &k.row_err, &k.rgb, &k.err_sum, &k.err_cnt, &k.rec_cnt, &k.rec_off, &k.rec_in,
                     &k.rec_wt, &k.rec_code, &k.rec_val, &k.sel_in, &k.sel_t, &k.list, &k.list_cnt, &k.cell_fb,
                     &k.row_code, &k.row_list,
-                    &k.XT, &k.dT[0], &k.dT[1], &k.dyT, &k.tpart})
+                    &k.XT, &k.dT[0], &k.dT[1], &k.dT[2], &k.dT[3], &k.dyT, &k.tpart})
     b->release();
   for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) {
     k.H[l].release();

@@ -51,8 +51,8 @@ struct MapBufs {
 // Learned radiance cache state: parameters, Adam moments, training scratch.
 struct CacheState {
   DevBuf params, grads, adam_m, adam_v, params_bf16, params_ema, ema_bf16;
-  DevBuf X, H[MLP_HIDDEN_LAYERS], y, dy, d0, d1, row_err, rgb;
-  DevBuf XT, HT[MLP_HIDDEN_LAYERS], dT[2], dyT, tpart;  // training: transposes and split partials
+  DevBuf X, H[MLP_HIDDEN_LAYERS], y, dy, row_err, rgb;
+  DevBuf XT, HT[MLP_HIDDEN_LAYERS], dT[MLP_HIDDEN_LAYERS], dyT, tpart;  // training: transposes and split partials
   MlpDev w;
   MlpTrainBufs tb;
   uint32_t rows_cap = 0;

@@ -55,8 +55,6 @@ struct MlpDev {
 
 struct MlpTrainBufs {
   uint16_t* dy;       // [rows x OUTPAD]
-  uint16_t* d0;       // [rows x HID] deltas (ping-pong)
-  uint16_t* d1;
   float* row_err;     // [rows]
   float* gw[MLP_LAYERS];
   float* gb[MLP_LAYERS];
@@ -68,7 +66,7 @@ struct MlpTrainBufs {
 struct MlpTrainT {
   uint16_t* XT;                     // [KPAD x ld]
   uint16_t* HT[MLP_HIDDEN_LAYERS];  // [HID x ld]
-  uint16_t* dT[2];                  // [HID x ld], ping-pong
+  uint16_t* dT[MLP_HIDDEN_LAYERS];  // [HID x ld]: dT[l] = the delta of hidden layer l's output, transposed
   uint16_t* dyT;                    // [OUTPAD x ld]
   uint32_t ld;
   float* part;                      // mlp_train_partial_floats(rows)

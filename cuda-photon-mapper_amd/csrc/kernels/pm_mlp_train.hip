@@ -208,25 +208,34 @@ __global__ void __launch_bounds__(256) k_gemm_nt(NtArgs g) {
   nt_tile(g, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, (int)gridDim.z, As, Bs);
 }
 
-// Two independent products in one launch: the first p.tiles[0] workgroups
-// take product 0 (n-tiles fastest, then m-tiles, then slices), the rest
-// product 1.  Used for a layer's weight gradient beside the delta of the
-// layer below, which read the same delta and write different buffers.
-struct NtPair {
-  NtArgs a[2];
-  int nx[2], ny[2], nz[2];
-  int tiles0;
+// Independent products in one launch (the five weight gradients of a batch):
+// workgroups [first[s], first[s + 1]) take product s, n-tiles fastest, then
+// m-tiles, then slices.
+struct NtMulti {
+  NtArgs a[MLP_LAYERS];
+  int nx[MLP_LAYERS], ny[MLP_LAYERS], nz[MLP_LAYERS];
+  int first[MLP_LAYERS + 1];
 };
 
-__global__ void __launch_bounds__(256) k_gemm_nt_pair(NtPair p) {
+template <int S>
+__device__ __forceinline__ void nt_multi_tile(const NtMulti& p, int c, uint16_t (*As)[NT_LDS], uint16_t (*Bs)[NT_LDS]) {
+  nt_tile(p.a[S], c % p.nx[S], (c / p.nx[S]) % p.ny[S], c / (p.nx[S] * p.ny[S]), p.nz[S], As, Bs);
+}
+
+__global__ void __launch_bounds__(256) k_gemm_nt_multi(NtMulti p) {
   __shared__ __align__(16) uint16_t As[NT_TILE][NT_LDS];
   __shared__ __align__(16) uint16_t Bs[NT_TILE][NT_LDS];
   const int b = (int)blockIdx.x;
-  if (b < p.tiles0) {
-    nt_tile(p.a[0], b % p.nx[0], (b / p.nx[0]) % p.ny[0], b / (p.nx[0] * p.ny[0]), p.nz[0], As, Bs);
-  } else {
-    const int c = b - p.tiles0;
-    nt_tile(p.a[1], c % p.nx[1], (c / p.nx[1]) % p.ny[1], c / (p.nx[1] * p.ny[1]), p.nz[1], As, Bs);
+  int s = 0;
+  while (s + 1 < MLP_LAYERS && b >= p.first[s + 1]) ++s;
+  const int c = b - p.first[s];
+  static_assert(MLP_LAYERS == 5, "one case per layer");
+  switch (s) {
+    case 0: nt_multi_tile<0>(p, c, As, Bs); break;
+    case 1: nt_multi_tile<1>(p, c, As, Bs); break;
+    case 2: nt_multi_tile<2>(p, c, As, Bs); break;
+    case 3: nt_multi_tile<3>(p, c, As, Bs); break;
+    default: nt_multi_tile<4>(p, c, As, Bs); break;
   }
 }
 
@@ -236,20 +245,18 @@ hipError_t gemm_nt(const NtArgs& a, int slices, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t gemm_nt_pair(const NtArgs& a, int slices_a, const NtArgs& b, int slices_b, hipStream_t st) {
-  NtPair p;
-  p.a[0] = a;
-  p.a[1] = b;
-  const NtArgs* g[2] = {&a, &b};
-  const int sl[2] = {slices_a, slices_b};
-  for (int i = 0; i < 2; ++i) {
-    p.nx[i] = (g[i]->N + NT_TILE - 1) / NT_TILE;
-    p.ny[i] = (g[i]->M + NT_TILE - 1) / NT_TILE;
-    p.nz[i] = sl[i];
+// The products g[0..MLP_LAYERS) with slices[s] K slices each, one launch.
+hipError_t gemm_nt_multi(const NtArgs* g, const int* slices, hipStream_t st) {
+  NtMulti p;
+  p.first[0] = 0;
+  for (int i = 0; i < MLP_LAYERS; ++i) {
+    p.a[i] = g[i];
+    p.nx[i] = (g[i].N + NT_TILE - 1) / NT_TILE;
+    p.ny[i] = (g[i].M + NT_TILE - 1) / NT_TILE;
+    p.nz[i] = slices[i];
+    p.first[i + 1] = p.first[i] + p.nx[i] * p.ny[i] * p.nz[i];
   }
-  p.tiles0 = p.nx[0] * p.ny[0] * p.nz[0];
-  const int total = p.tiles0 + p.nx[1] * p.ny[1] * p.nz[1];
-  hipLaunchKernelGGL(k_gemm_nt_pair, dim3((unsigned)total), dim3(256), 0, st, p);
+  hipLaunchKernelGGL(k_gemm_nt_multi, dim3((unsigned)p.first[MLP_LAYERS]), dim3(256), 0, st, p);
   return hipGetLastError();
 }
 
@@ -460,6 +467,112 @@ __global__ void __launch_bounds__(256) k_fwd_fused(FwdArgs A) {
   }
 }
 
+// ---------------------------------------------------------------- fused deltas
+// The deltas of the four hidden layers of a training batch in one launch: a
+// workgroup takes 64 rows from dy down to layer 0, D_{l-1} = (D_l . W_l) x
+// [H_{l-1} > 0] with D_4 = dy, the deltas in LDS and each written transposed
+// (dT[l - 1], the weight-gradient operand).  Every product is k_gemm_nt's
+// (dy's 16 columns zero-padded to one K step of 32, the mask applied to the
+// FP32 sum before the BF16 rounding), so the outputs equal those of one launch
+// per layer bit for bit.  A wave computes the tile's 64 rows x 32 of the 128
+// columns.
+constexpr size_t FB_LDS = 2u * (2u * 64u * FF_HS + 128u * FF_HS + 64u * FF_HS);
+
+struct BwdArgs {
+  const uint16_t* dy;                    // [rows x OUTPAD]
+  uint32_t rows;
+  const uint16_t* wbt[MLP_LAYERS];       // W_l^T [kin x nout]
+  const uint16_t* H[MLP_HIDDEN_LAYERS];  // [rows x HID], the ReLU masks
+  uint16_t* dT[MLP_HIDDEN_LAYERS];       // [HID x ld]
+  uint32_t ld;
+};
+
+__global__ void __launch_bounds__(256) k_bwd_fused(BwdArgs A) {
+  extern __shared__ __align__(16) unsigned char fl[];
+  uint16_t* const D0 = (uint16_t*)fl;         // [64][FF_HS]: the deltas, ping-pong
+  uint16_t* const D1 = D0 + 64 * FF_HS;
+  uint16_t* Ws = D0 + 2 * 64 * FF_HS;         // [128][FF_HS]
+  uint16_t* Ms = Ws + 128 * FF_HS;                                 // [64][FF_HS]
+  const int tid = (int)threadIdx.x, lane = lane_id(), wave = tid >> 6, r16 = lane & 15, q = lane >> 4;
+  const uint32_t m0 = blockIdx.x * 64u;
+  const uint32_t rows = A.rows;
+  // dy rows of the tile: 16 columns, then zeros to 32 (zero past the batch)
+  for (int v = tid; v < 64 * 4; v += 256) {
+    const int r = v >> 2, j = (v & 3) * 8;
+    uint4 val = make_uint4(0u, 0u, 0u, 0u);
+    if (j < MLP_OUTPAD && m0 + (uint32_t)r < rows) val = *(const uint4*)&A.dy[(size_t)(m0 + r) * MLP_OUTPAD + j];
+    *(uint4*)&D0[r * FF_HS + j] = val;
+  }
+  const uint16_t* act = D0;
+  for (int l = MLP_HIDDEN_LAYERS; l >= 1; --l) {
+    const int nout = l == MLP_HIDDEN_LAYERS ? MLP_OUTPAD : MLP_HID;  // K of this product
+    const int kpad = (nout + 31) / 32 * 32;
+    __syncthreads();  // the previous step is done with Ws and Ms
+    // W_l^T [128 x nout], zero-padded to kpad columns
+    for (int v = tid; v < MLP_HID * (kpad / 8); v += 256) {
+      const int r = v / (kpad / 8), j = (v % (kpad / 8)) * 8;
+      uint4 val = make_uint4(0u, 0u, 0u, 0u);
+      if (j < nout) val = *(const uint4*)&A.wbt[l][(size_t)r * nout + j];
+      *(uint4*)&Ws[r * FF_HS + j] = val;
+    }
+    // the mask rows H_{l-1}
+    for (int v = tid; v < 64 * (MLP_HID / 8); v += 256) {
+      const int r = v / (MLP_HID / 8), j = (v % (MLP_HID / 8)) * 8;
+      uint4 val = make_uint4(0u, 0u, 0u, 0u);
+      if (m0 + (uint32_t)r < rows) val = *(const uint4*)&A.H[l - 1][(size_t)(m0 + r) * MLP_HID + j];
+      *(uint4*)&Ms[r * FF_HS + j] = val;
+    }
+    __syncthreads();
+    f32x4_t acc[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int ks = 0; ks < kpad; ks += 32) {
+      bf16x8_t af[4], bfr[2];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const uint4 v = *(const uint4*)&act[(16 * a + r16) * FF_HS + ks + 8 * q];
+        af[a] = *(const bf16x8_t*)&v;
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const uint4 v = *(const uint4*)&Ws[(32 * wave + 16 * b + r16) * FF_HS + ks + 8 * q];
+        bfr[b] = *(const bf16x8_t*)&v;
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+    }
+    // lane holds D_{l-1}[16a + 4q + e][32 wave + 16b + r16] before the mask
+    uint16_t* out = ((MLP_HIDDEN_LAYERS - l) & 1) ? D0 : D1;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int n = 32 * wave + 16 * b + r16;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int m = 16 * a + 4 * q + e;
+          float v = acc[a][b][e];
+          if (!(bf_to_f(Ms[m * FF_HS + n]) > 0.0f)) v = 0.0f;
+          out[m * FF_HS + n] = f32_to_bf16(v);
+        }
+    }
+    __syncthreads();
+    for (int v = tid; v < MLP_HID * 8; v += 256) {
+      const int n = v >> 3, j = (v & 7) * 8;
+      uint32_t p[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        p[t] = (uint32_t)out[(j + 2 * t) * FF_HS + n] | ((uint32_t)out[(j + 2 * t + 1) * FF_HS + n] << 16);
+      *(uint4*)&A.dT[l - 1][(size_t)n * A.ld + m0 + j] = make_uint4(p[0], p[1], p[2], p[3]);
+    }
+    act = out;
+  }
+}
+
 // The gradients of every layer from the split partials: gw[l][i] = sum over
 // slices z (in order) of part_l[z][i], gb[l][m] = sum over z of bpart_l[z][m].
 struct GradSum {
@@ -540,28 +653,43 @@ hipError_t mlp_train_grads(const MlpDev& w, const float* in, uint32_t rows, uint
     hipLaunchKernelGGL(k_fwd_fused, dim3((rows + 63) / 64), dim3(256), FF_LDS, st, f);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  // backward: per layer the split weight-gradient product, then the delta
-  // of the layer below (with its transpose)
+  // backward: the four hidden layers' deltas in one launch, then the five
+  // split weight-gradient products in one launch
+  {
+    BwdArgs bw;
+    memset(&bw, 0, sizeof(bw));
+    bw.dy = tb.dy;
+    bw.rows = rows;
+    for (int l = 0; l < MLP_LAYERS; ++l) bw.wbt[l] = w.wbt[l];
+    for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) {
+      bw.H[l] = H[l];
+      bw.dT[l] = tt.dT[l];
+    }
+    bw.ld = ld;
+    hipLaunchKernelGGL(k_bwd_fused, dim3((rows + 63) / 64), dim3(256), FB_LDS, st, bw);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
   const int slices = mlp_train_slices(rows);
   const int kps = (int)(((rows + (uint32_t)slices - 1u) / (uint32_t)slices + 31u) / 32u * 32u);
   GradSum S;
   memset(&S, 0, sizeof(S));
   S.slices = slices;
-  float* part = tt.part;
-  const uint16_t* delta = tb.dy;
-  const uint16_t* deltaT = tt.dyT;
-  int dcols = MLP_OUTPAD;
+  NtArgs wg[MLP_LAYERS];
+  int wsl[MLP_LAYERS];
+  float* part = tt.part;  // partials of layer 4 first, then 3, ..., 0
   for (int l = MLP_HIDDEN_LAYERS; l >= 0; --l) {
     const int nout = l == MLP_HIDDEN_LAYERS ? MLP_OUTPAD : MLP_HID;
     const int kl = l == 0 ? MLP_KPAD : MLP_HID;
-    const uint16_t* actT = l == 0 ? tt.XT : tt.HT[l - 1];
     NtArgs a = nt_args(nout, kl, (int)rows);
-    a.A = deltaT; a.lda = (int)ld;
-    a.B = actT; a.ldb = (int)ld;
+    a.A = l == MLP_HIDDEN_LAYERS ? tt.dyT : tt.dT[l];
+    a.lda = (int)ld;
+    a.B = l == 0 ? tt.XT : tt.HT[l - 1];
+    a.ldb = (int)ld;
     a.kps = kps;
     a.part = part;
     a.bpart = part + (size_t)slices * nout * kl;
-    if (l == 0 && (e = gemm_nt(a, slices, st)) != hipSuccess) return e;
+    wg[l] = a;
+    wsl[l] = slices;
     S.part[l] = a.part;
     S.bpart[l] = a.bpart;
     tt.part_w[l] = a.part;
@@ -571,22 +699,8 @@ hipError_t mlp_train_grads(const MlpDev& w, const float* in, uint32_t rows, uint
     S.nw[l] = (uint32_t)(nout * kl);
     S.nb[l] = (uint32_t)nout;
     part += (size_t)slices * (nout * kl + nout);
-    if (l == 0) break;
-    uint16_t* nd = (l & 1) ? tb.d1 : tb.d0;
-    uint16_t* ndT = (l & 1) ? tt.dT[1] : tt.dT[0];
-    NtArgs b = nt_args((int)rows, MLP_HID, nout);
-    b.A = delta; b.lda = dcols;
-    b.B = w.wbt[l]; b.ldb = nout;  // W_l^T [in x out]
-    b.mask = H[l - 1]; b.ldm = MLP_HID;
-    b.Cb = nd; b.ldc = MLP_HID;
-    b.Ct = ndT; b.ldt = (int)ld;
-    // the weight gradient of layer l and the delta of layer l - 1 read the
-    // same delta and write different buffers: one launch
-    if ((e = gemm_nt_pair(a, slices, b, 1, st)) != hipSuccess) return e;
-    delta = nd;
-    deltaT = ndT;
-    dcols = MLP_HID;
   }
+  if ((e = gemm_nt_multi(wg, wsl, st)) != hipSuccess) return e;
   tt.slices = slices;
   if (!reduce) return hipSuccess;
   uint32_t tot = 0;
