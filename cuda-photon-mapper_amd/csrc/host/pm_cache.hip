@@ -262,7 +262,7 @@ int cache_train_dev(pm_context* c, const float* in, const float* target, uint32_
     }
     MlpTrainBufs tb = k.tb;
     tb.row_err = k.row_err.as<float>() + ro;
-    HIPCHK(mlp_train_grads(k.w, in + (size_t)o * MLP_IN, batch, k.X.as<uint16_t>(), H, k.y.as<float>(),
+    HIPCHK(mlp_train_grads(k.w, in + (size_t)o * MLP_IN, batch, H, k.y.as<float>(),
                            target + (size_t)o * MLP_OUT, tb, tt, false, c->stream, esum, ecnt));
     k.step += 1;
     HIPCHK(mlp_adam(k.w, tb, lr, k.step, c->stream, &tt));
@@ -314,7 +314,7 @@ int pm_cache_grad(pm_context* c, const float* in_dev, const float* target_dev, u
   uint16_t* const* H = hidden_ptrs(k, tmp);
   MlpTrainT tt;
   if ((r = ensure_train_t(k, n, tt)) != PM_OK) return r;
-  HIPCHK(mlp_train_grads(k.w, in_dev, n, k.X.as<uint16_t>(), H, k.y.as<float>(), target_dev, k.tb, tt, true,
+  HIPCHK(mlp_train_grads(k.w, in_dev, n, H, k.y.as<float>(), target_dev, k.tb, tt, true,
                          c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy(grad_host, k.grads.p, 4ull * count, hipMemcpyDeviceToHost));

@@ -86,7 +86,7 @@ size_t mlp_train_partial_floats(uint32_t rows);
 // With esum / ecnt, each row's squared error (capped at 1e6, in 2^-24 fixed
 // point) is also added to the error tracker's cell of its input (err_cell),
 // and the cell's row count incremented.
-hipError_t mlp_train_grads(const MlpDev& w, const float* in, uint32_t rows, uint16_t* X, uint16_t* const* H,
+hipError_t mlp_train_grads(const MlpDev& w, const float* in, uint32_t rows, uint16_t* const* H,
                            float* y, const float* target, MlpTrainBufs& tb, MlpTrainT& tt, bool reduce,
                            hipStream_t st, unsigned long long* esum = nullptr, unsigned int* ecnt = nullptr);
 

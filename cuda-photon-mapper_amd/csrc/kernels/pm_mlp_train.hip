@@ -269,35 +269,6 @@ NtArgs nt_args(int M, int N, int K) {
   return a;
 }
 
-// Encoding of the inputs as rows (X [rows x KPAD]) and as columns
-// (X^T [KPAD x ld]); one lane per (input, row), rows fastest, so the X^T
-// stores of a wave are contiguous.  Features follow k_encode: out[i*24 + 2b +
-// {0,1}] = {sin, cos}(2 pi 2^(b-1) v).
-__global__ void k_encode_t(const float* __restrict__ in, uint32_t n, uint16_t* __restrict__ X,
-                           uint16_t* __restrict__ XT, uint32_t ld) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t i = t / n, r = t % n;
-  if (i >= (uint32_t)MLP_IN) return;
-  const float v = in[(size_t)r * MLP_IN + i];
-  uint16_t* o = X + (size_t)r * MLP_KPAD + i * 2 * MLP_BANDS;
-  float scale = 0.5f;
-  for (int b = 0; b < MLP_BANDS; ++b) {
-    float s, c;
-    sincos_turns(v * scale, &s, &c);
-    const uint16_t hs = f32_to_bf16(s), hc = f32_to_bf16(c);
-    o[2 * b] = hs;
-    o[2 * b + 1] = hc;
-    XT[(size_t)(i * 2 * MLP_BANDS + 2 * b) * ld + r] = hs;
-    XT[(size_t)(i * 2 * MLP_BANDS + 2 * b + 1) * ld + r] = hc;
-    scale *= 2.0f;
-  }
-  if (i == MLP_IN - 1)
-    for (int k = MLP_FEAT; k < MLP_KPAD; ++k) {
-      X[(size_t)r * MLP_KPAD + k] = 0;
-      XT[(size_t)k * ld + r] = 0;
-    }
-}
-
 // ---------------------------------------------------------------- fused forward
 // The forward pass of a training batch in one launch: a workgroup takes 64
 // rows through the five layers with the activations in LDS, writes each
@@ -318,7 +289,7 @@ constexpr int FF_HS = MLP_HID + 8;   // of the hidden activations and W2..W5
 constexpr size_t FF_LDS = 2u * (64u * FF_XS + 2u * 64u * FF_HS + 128u * FF_XS) + 4u * 64u * MLP_OUTPAD;
 
 struct FwdArgs {
-  const uint16_t* X;  // [rows x KPAD] encoded inputs
+  uint16_t* XT;       // [KPAD x ld] encoded inputs, transposed (written here)
   uint32_t rows;
   const uint16_t* w[MLP_LAYERS];
   const float* b[MLP_LAYERS];
@@ -330,7 +301,7 @@ struct FwdArgs {
   uint16_t* dy;                     // [rows x OUTPAD]
   uint16_t* dyT;                    // [OUTPAD x ld]
   float* row_err;                   // [rows]
-  const float* in;                  // [rows x IN] query rows (error-tracker cells)
+  const float* in;                  // [rows x IN] query rows
   unsigned long long* esum;         // error tracker (null: none)
   unsigned int* ecnt;
 };
@@ -344,12 +315,33 @@ __global__ void __launch_bounds__(256) k_fwd_fused(FwdArgs A) {
   const int tid = (int)threadIdx.x, lane = lane_id(), wave = tid >> 6, r16 = lane & 15, q = lane >> 4;
   const uint32_t m0 = blockIdx.x * 64u;
   const uint32_t rows = A.rows;
-  // X rows of the tile (zero past the batch)
-  for (int v = tid; v < 64 * (MLP_KPAD / 8); v += 256) {
-    const int r = v / (MLP_KPAD / 8), kk = (v % (MLP_KPAD / 8)) * 8;
-    uint4 val = make_uint4(0u, 0u, 0u, 0u);
-    if (m0 + (uint32_t)r < rows) val = *(const uint4*)&A.X[(size_t)(m0 + r) * MLP_KPAD + kk];
-    *(uint4*)&Xs[r * FF_XS + kk] = val;
+  // the tile's encoded inputs: {sin, cos}(2 pi 2^(b-1) v) of each input v
+  // at column 24 i + 2b (+1), padding columns and rows past the batch zero
+  for (int v = tid; v < 64 * MLP_IN; v += 256) {
+    const int r = v / MLP_IN, i = v % MLP_IN;
+    const bool ok = m0 + (uint32_t)r < rows;
+    const float x = ok ? A.in[(size_t)(m0 + r) * MLP_IN + i] : 0.0f;
+    uint16_t* o = &Xs[r * FF_XS + i * 2 * MLP_BANDS];
+    float scale = 0.5f;
+    for (int b = 0; b < MLP_BANDS; ++b) {
+      float sn, cs;
+      sincos_turns(x * scale, &sn, &cs);
+      o[2 * b] = ok ? f32_to_bf16(sn) : (uint16_t)0;
+      o[2 * b + 1] = ok ? f32_to_bf16(cs) : (uint16_t)0;
+      scale *= 2.0f;
+    }
+    if (i == MLP_IN - 1)
+      for (int k = MLP_FEAT; k < MLP_KPAD; ++k) Xs[r * FF_XS + k] = 0;
+  }
+  __syncthreads();
+  // X^T: 64 columns of each of its KPAD rows
+  for (int v = tid; v < MLP_KPAD * 8; v += 256) {
+    const int f = v >> 3, j = (v & 7) * 8;
+    uint32_t p[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      p[t] = (uint32_t)Xs[(j + 2 * t) * FF_XS + f] | ((uint32_t)Xs[(j + 2 * t + 1) * FF_XS + f] << 16);
+    *(uint4*)&A.XT[(size_t)f * A.ld + m0 + j] = make_uint4(p[0], p[1], p[2], p[3]);
   }
   const uint16_t* act = Xs;
   int as = FF_XS, kin = MLP_KPAD;
@@ -615,23 +607,19 @@ int mlp_train_slices(uint32_t rows) { return (int)std::max<uint32_t>(1u, std::mi
 
 uint32_t mlp_train_ld(uint32_t rows) { return (rows + 63u) / 64u * 64u; }
 
-hipError_t mlp_train_grads(const MlpDev& w, const float* in, uint32_t rows, uint16_t* X, uint16_t* const* H,
+hipError_t mlp_train_grads(const MlpDev& w, const float* in, uint32_t rows, uint16_t* const* H,
                            float* y, const float* target, MlpTrainBufs& tb, MlpTrainT& tt, bool reduce,
                            hipStream_t st, unsigned long long* esum, unsigned int* ecnt) {
   if (rows == 0) return hipSuccess;
   const uint32_t ld = tt.ld;
   if (ld < mlp_train_ld(rows)) return hipErrorInvalidValue;
   hipError_t e;
-  {
-    const uint64_t th = (uint64_t)rows * MLP_IN;
-    hipLaunchKernelGGL(k_encode_t, dim3((unsigned)((th + 255) / 256)), dim3(256), 0, st, in, rows, X, tt.XT, ld);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
-  // forward: H_l and H_l^T, y, the loss gradient (and the error bins), one launch
+  // encoding, forward (X^T, H_l and H_l^T, y), the loss gradient and the
+  // error bins: one launch
   {
     FwdArgs f;
     memset(&f, 0, sizeof(f));
-    f.X = X;
+    f.XT = tt.XT;
     f.rows = rows;
     for (int l = 0; l < MLP_LAYERS; ++l) {
       f.w[l] = w.wb[l];
