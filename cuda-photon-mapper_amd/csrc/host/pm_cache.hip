@@ -139,7 +139,7 @@ void cache_release(CacheState& k) {
   for (DevBuf* b : {&k.params, &k.grads, &k.adam_m, &k.adam_v, &k.params_bf16, &k.params_ema, &k.ema_bf16, &k.X, &k.y, &k.dy,
                     &k.row_err, &k.rgb, &k.err_sum, &k.err_cnt, &k.rec_cnt, &k.rec_off, &k.rec_in,
                     &k.rec_wt, &k.rec_code, &k.rec_val, &k.sel_in, &k.sel_t, &k.list, &k.list_cnt, &k.cell_fb,
-                    &k.row_code, &k.row_list,
+                    &k.row_code, &k.row_list, &k.loss_part, &k.loss_acc,
                     &k.XT, &k.dT[0], &k.dT[1], &k.dT[2], &k.dT[3], &k.dyT, &k.tpart})
     b->release();
   for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) {
@@ -238,8 +238,11 @@ int cache_predict_dev(pm_context* c, const float* in, uint32_t n, float* rgb, co
 }
 
 // One Adam step per `batch` rows of (inputs, targets); returns the mean loss.
+// With loss_terms, the loss is not read back: its sum stays in k.loss_acc on
+// the device and *loss_terms gets the number of terms (no host wait).
 int cache_train_dev(pm_context* c, const float* in, const float* target, uint32_t n, uint32_t batch, float lr,
-                    double* loss, unsigned long long* esum = nullptr, unsigned int* ecnt = nullptr) {
+                    double* loss, unsigned long long* esum = nullptr, unsigned int* ecnt = nullptr,
+                    uint64_t* loss_terms = nullptr) {
   int r = ensure_cache(c);
   if (r != PM_OK) return r;
   CacheState& k = c->cache;
@@ -251,13 +254,14 @@ int cache_train_dev(pm_context* c, const float* in, const float* target, uint32_
   if ((r = sync_ema(c)) != PM_OK) return r;
   uint64_t cnt = 0;
   uint32_t ro = 0;  // rows of squared errors in k.row_err not yet summed
-  HIPCHK(c->small.ensure(64));
-  double* dloss = reinterpret_cast<double*>(c->small.as<uint32_t>() + 12);
+  HIPCHK(k.loss_acc.ensure(8));
+  double* dloss = k.loss_acc.as<double>();
+  HIPCHK(k.loss_part.ensure(8 * sum_rows_parts()));
   HIPCHK(hipMemsetAsync(dloss, 0, sizeof(double), c->stream));
   for (uint32_t o = 0; o + batch <= n; o += batch) {
     // each step's squared errors go to the next rows of k.row_err, summed in one pass when it is full
     if (ro + batch > k.rows_cap) {
-      HIPCHK(launch_sum_rows(k.row_err.as<float>(), ro, dloss, c->stream));
+      HIPCHK(launch_sum_rows(k.row_err.as<float>(), ro, dloss, k.loss_part.as<double>(), c->stream));
       ro = 0;
     }
     MlpTrainBufs tb = k.tb;
@@ -269,7 +273,11 @@ int cache_train_dev(pm_context* c, const float* in, const float* target, uint32_
     ro += batch;
     cnt += (uint64_t)batch * MLP_OUT;
   }
-  if (ro) HIPCHK(launch_sum_rows(k.row_err.as<float>(), ro, dloss, c->stream));
+  if (ro) HIPCHK(launch_sum_rows(k.row_err.as<float>(), ro, dloss, k.loss_part.as<double>(), c->stream));
+  if (loss_terms) {
+    *loss_terms = cnt;
+    return PM_OK;
+  }
   double total = 0.0;
   HIPCHK(hipMemcpyAsync(&total, dloss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -444,6 +452,7 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   HIPCHK(hipMemsetAsync(k.err_sum.p, 0, 8ull * ncell, st));
   HIPCHK(hipMemsetAsync(k.err_cnt.p, 0, 4ull * ncell, st));
   double loss = 0.0;
+  uint64_t loss_terms = 0;  // the training loss's sum is read after the frame
   const uint32_t S = N > 0 ? k.train_samples : 0u;
   // the volume records, when surface records are never cached: inference
   // skips the surface records, whose values the exact fallback writes
@@ -477,8 +486,8 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
     vl.out = k.sel_t.as<float>();
     vl.mode7 = mode7;
     HIPCHK(launch_cache_values(vl, st));
-    int r = cache_train_dev(c, k.sel_in.as<float>(), k.sel_t.as<float>(), S, k.batch, k.lr, &loss,
-                            k.err_sum.as<unsigned long long>(), k.err_cnt.as<unsigned int>());
+    int r = cache_train_dev(c, k.sel_in.as<float>(), k.sel_t.as<float>(), S, k.batch, k.lr, nullptr,
+                            k.err_sum.as<unsigned long long>(), k.err_cnt.as<unsigned int>(), &loss_terms);
     if (r != PM_OK) return r;
   }
   HIPCHK(hipEventRecord(ev[2], st));
@@ -521,6 +530,11 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
                                  k.rec_val.as<float>(), c->rp.spp, c->cfg.exposure, rgb_dev, (uchar4*)out_dev, st));
   HIPCHK(hipEventRecord(ev[5], st));
   HIPCHK(hipStreamSynchronize(st));
+  if (loss_terms) {
+    double total = 0.0;
+    HIPCHK(hipMemcpy(&total, k.loss_acc.p, sizeof(double), hipMemcpyDeviceToHost));
+    loss = total / (double)loss_terms;
+  }
   k.frame += 1;
   k.st.records = N;
   k.st.fallbacks = nfb;

@@ -391,19 +391,29 @@ __global__ void __launch_bounds__(LIST_THREADS) k_list_scatter(const uint16_t* _
   for (uint32_t i = threadIdx.x; i < n1; i += LIST_THREADS) list[o1 + i] = (uint32_t)(tile0 + stage[n0 + i]);
 }
 
-// acc += sum of v[0..n) in a fixed order (one workgroup): the frame's training
-// loss without a host round trip per step.
-__global__ void __launch_bounds__(256) k_sum_rows(const float* __restrict__ v, uint32_t n, double* __restrict__ acc) {
-  __shared__ double part[256];
+// acc += sum of v[0..n) in a fixed order: workgroup b sums its chunk of
+// SUM_CHUNK values into part[b] (a fixed tree), then one workgroup adds the
+// partials in order.  The frame's training loss without a host round trip
+// per step.
+constexpr uint32_t SUM_CHUNK = 4096, SUM_PARTS = 256;
+__global__ void __launch_bounds__(256) k_sum_rows(const float* __restrict__ v, uint32_t n, double* __restrict__ part) {
+  __shared__ double red[256];
+  const uint32_t b0 = blockIdx.x * SUM_CHUNK, b1 = min(n, b0 + SUM_CHUNK);
   double s = 0.0;
-  for (uint32_t i = threadIdx.x; i < n; i += 256u) s += (double)v[i];
-  part[threadIdx.x] = s;
+  for (uint32_t i = b0 + threadIdx.x; i < b1; i += 256u) s += (double)v[i];
+  red[threadIdx.x] = s;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *acc += part[0];
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_sum_parts(const double* __restrict__ part, uint32_t np, double* __restrict__ acc) {
+  double s = 0.0;
+  for (uint32_t b = 0; b < np; ++b) s += part[b];
+  *acc += s;
 }
 
 // Per pixel: compensated sum over its records of weight * value, / spp.
@@ -499,8 +509,16 @@ hipError_t launch_count_total(const uint32_t* cnt, uint32_t n, unsigned long lon
   return hipGetLastError();
 }
 
-hipError_t launch_sum_rows(const float* v, uint32_t n, double* acc, hipStream_t st) {
-  hipLaunchKernelGGL(k_sum_rows, dim3(1), dim3(256), 0, st, v, n, acc);
+size_t sum_rows_parts() { return SUM_PARTS; }
+
+hipError_t launch_sum_rows(const float* v, uint32_t n, double* acc, double* part, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  for (uint32_t o = 0; o < n; o += SUM_CHUNK * SUM_PARTS) {
+    const uint32_t m = std::min(n - o, SUM_CHUNK * SUM_PARTS);
+    const uint32_t np = (m + SUM_CHUNK - 1) / SUM_CHUNK;
+    hipLaunchKernelGGL(k_sum_rows, dim3(np), dim3(256), 0, st, v + o, m, part);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(1), 0, st, part, np, acc);
+  }
   return hipGetLastError();
 }
 

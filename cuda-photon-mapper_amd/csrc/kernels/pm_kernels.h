@@ -303,7 +303,10 @@ hipError_t launch_cache_record(const CacheRecordLaunch& r, hipStream_t st);
 hipError_t launch_cache_values(const CacheValueLaunch& v, hipStream_t st);
 hipError_t launch_cache_select(const float* in, const uint32_t* list_s, uint32_t n_s, const uint32_t* list_v,
                                uint32_t n_v, uint32_t count, uint32_t seed, uint32_t frame, float* out, hipStream_t st);
-hipError_t launch_sum_rows(const float* v, uint32_t n, double* acc, hipStream_t st);
+// *acc += v[0] + ... + v[n - 1] in double, in a fixed order; `part` holds
+// sum_rows_parts() doubles of scratch
+size_t sum_rows_parts();
+hipError_t launch_sum_rows(const float* v, uint32_t n, double* acc, double* part, hipStream_t st);
 // *total = cnt[0] + ... + cnt[n - 1] in 64 bits
 hipError_t launch_count_total(const uint32_t* cnt, uint32_t n, unsigned long long* total, hipStream_t st);
 // cell_fb[c] = 1 when error cell c saw no training row or its mean squared
