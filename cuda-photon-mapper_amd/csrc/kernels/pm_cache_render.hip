@@ -49,7 +49,7 @@ __device__ __forceinline__ uint16_t record_code(V3f x, V3f nrm, V3f light, float
 // second order in the step, where the first point alone would be first
 // order).  A point farther than 1.5 s steps from the open record's first
 // point, a surface query, or a point on a ray of another direction starts a
-// new record.  Within the near-light radius (4 s steps) of the point light,
+// new record.  Within the near-light radius (near2) of the point light,
 // where the in-scattered radiance grows as 1 / distance^2 and no low-order
 // rule over s steps holds, every march point keeps its own record (and is
 // gathered exactly, see record_class).  The open volume record stays in registers
@@ -72,7 +72,7 @@ struct RecordGather {
   float open_sw;     // sum of the merged points' scalar weights (w.x + w.y + w.z)
   V3f open_swx;      // sum of scalar weight x position
   V3f light;
-  float near2;       // (4 s steps)^2: no merging closer than this to the light
+  float near2;       // squared near-light radius: no merging closer than this to the light
   __device__ void store(uint32_t k, V3f x, V3f d, V3f nrm, V3f w) {
     if (!write) return;
     size_t i = (size_t)base + k;

@@ -84,6 +84,12 @@ def test_value_kernels_agree(scene):
 
 
 def test_cache_accuracy_cache_scene():
+    """At equal samples the cache trades a bounded loss of accuracy for speed.
+    It learns the plain estimator's own values, so against the 64-spp
+    reference it cannot beat the plain frame of the same samples; it stays
+    within 15 % of that frame's error, within 3 % of the frame itself, and
+    renders in under half its time.  The gain at equal frame time is the next
+    test."""
     say("case: cache scene 960x540, cache (volume in-scatter cached, march stride 4) after 20 frames vs 64-spp plain")
     c = pmamd.preset(5)
     c.width, c.height = 960, 540
@@ -112,6 +118,7 @@ def test_cache_accuracy_cache_scene():
            t_plain))
     assert e_same < 0.03
     assert e_cache < 1.15 * e_plain
+    assert s.ms_total < 0.5 * t_plain
 
 
 def test_cache_gain_on_equal_frame_budget():
