@@ -48,16 +48,57 @@ def rel_rmse(a, b, exposure):
     return float(np.sqrt(((da - db) ** 2).mean()) / db.mean())
 
 
-def test_all_fallback_equals_plain(scene):
-    say("case: cache render with threshold 0 (all exact)")
+# crops whose sides are not multiples of the record order's 4x4 pixel blocks
+# (padding slots) and that do not start on a block
+CROPS = [None, (5, 3, 37, 22), (0, 0, 1, 1), (90, 70, 6, 2), (13, 0, 4, 72)]
+
+
+@pytest.mark.parametrize("crop", CROPS)
+def test_all_fallback_equals_plain(scene, crop):
+    say("case: cache render with threshold 0 (all exact), crop %s" % (crop,))
     g = scene
     g.cache_configure(threshold=0.0, train_samples=16384, batch=16384)
-    img = g.render_cache()
-    ref = g.render()
+    box = crop or ()
+    img, u8 = g.render_cache(*box, want_u8=True)
+    ref = g.render(*box)
     s = g.cache_stats()
     assert s.fallbacks == s.records > 0
     err = np.abs(img.astype(np.float64) - ref)
     assert (err <= 1e-4 * np.abs(ref) + 1e-6).all(), float(err.max())
+    # the uchar4 frame is the tonemap of the cache's own FP32 frame: c * 255
+    # in double (c = value x exposure in FP32), capped at 255, truncated
+    c = img.astype(np.float32) * np.float32(g.cfg.exposure)
+    t = np.minimum(c.astype(np.float64) * 255.0, 255.0)
+    want = np.floor(np.where(t > 0.0, t, 0.0)).astype(np.uint8)
+    assert np.array_equal(u8[..., :3], want)
+
+
+def test_inference_rows(scene):
+    """Inference runs over the volume records only (the exact fallback writes
+    every surface record), on the 5-step layer 1 under fused mode 4 and the
+    full one under mode 3; the frames agree to the rounding of layer 1's
+    sums."""
+    say("case: cache frame inference rows under fused modes 4 and 3")
+    g = scene
+    imgs, stats = [], []
+    try:
+        for mode in (4, 3):
+            g.set_option(pmamd.PM_OPT_CACHE_FUSED_INFER, mode)
+            g.cache_reset(1)
+            g.cache_configure(threshold=0.05, train_samples=16384, batch=4096, march_stride=4)
+            for _ in range(2):
+                imgs.append(g.render_cache())
+            stats.append(g.cache_stats())
+    finally:
+        g.set_option(pmamd.PM_OPT_CACHE_FUSED_INFER, 4)
+    s4, s3 = stats
+    say("records %d, inferred %d (volume layer 1: %d / %d)" % (s4.records, s4.inferred, s4.inferred_volume,
+                                                                s3.inferred_volume))
+    assert 0 < s4.inferred < s4.records
+    assert s4.inferred_volume == s4.inferred
+    assert s3.inferred == s4.inferred and s3.inferred_volume == 0
+    d = np.abs(imgs[1].astype(np.float64) - imgs[3])
+    assert np.linalg.norm(d) <= 1e-2 * np.linalg.norm(imgs[3])
 
 
 def test_value_kernels_agree(scene):
