@@ -186,8 +186,8 @@ FLOP_FULL = 2 * (216 * 128 + 3 * 128 * 128 + 128 * 3)    # 154,368
 FLOP_VOLUME = 2 * (144 * 128 + 3 * 128 * 128 + 128 * 3)  # 135,936
 
 
-def northstar_cfg():
-    """The 1080p workload in the north-star precision form: photon power and
+def bf16_cache_cfg():
+    """The 1080p workload in its BF16 precision form: photon power and
     direction stored BF16 (positions and split planes FP32), FP64
     transmittance on long segments through the density field."""
     c = workload_cfg()
@@ -196,7 +196,7 @@ def northstar_cfg():
     return c
 
 
-def run_northstar(steps, train_frames, stride, thresholds, batch, rows, near_groups=2, extra_error_frames=0,
+def run_cache(steps, train_frames, stride, thresholds, batch, rows, near_groups=2, extra_error_frames=0,
                   ema=950):
     """Cache frames of the 1080p workload in the BF16 form: emit + build + cache
     render (record queries, train on 2^18 exact kNN targets, MLP inference,
@@ -205,7 +205,7 @@ def run_northstar(steps, train_frames, stride, thresholds, batch, rows, near_gro
     Returns the timings and the frame rows `rows` of each setting (their error
     against the FP64 oracle is computed by the CPU leg)."""
     lib = pmamd.load()
-    cfg = northstar_cfg()
+    cfg = bf16_cache_cfg()
     g = pmamd.PhotonMapper(cfg)
     npx = cfg.width * cfg.height
     rgb = pmamd.DeviceBuffer(npx * 12)
@@ -614,7 +614,7 @@ def main():
     ns = None
     if args.cache_steps > 0:
         ths = [float(v) for v in args.cache_thresholds.split(",") if v]
-        ns = run_northstar(args.cache_steps, args.cache_train_frames, args.cache_stride, ths, args.cache_batch,
+        ns = run_cache(args.cache_steps, args.cache_train_frames, args.cache_stride, ths, args.cache_batch,
                            err_rows, args.cache_near, args.cache_error_frames, args.cache_ema)
         for st in ns["settings"]:
             st["speedup_vs_plain_fp32_frame"] = round(out["ms_per_step"] / st["ms_per_frame"], 3)
