@@ -412,6 +412,7 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   memset(&rl, 0, sizeof(rl));
   rl.sc = c->scene;
   rl.rp = c->rp;
+  if (!c->render_media) rl.rp.media_on = 0;  // PM_OPT_RENDER_MEDIA 0: surface queries only
   rl.x0 = x0; rl.y0 = y0; rl.w = w; rl.h = h;
   rl.counts = k.rec_cnt.as<uint32_t>();
   rl.write = 0;

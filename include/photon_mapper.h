@@ -225,10 +225,11 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
  *                      pixel in adjacent lanes (default; a wave renders 64 / spp
  *                      pixels in raster tile order), 0 = one lane loops over a
  *                      pixel's samples (PM_OPT_TILE_REGIONS applies to this form);
- * PM_OPT_RENDER_MEDIA  1 = pm_render / pm_render_crop march the medium (default,
- *                      when the scene has one), 0 = they gather at surface hits
- *                      only: the surface-gather measurement on the same photon
- *                      maps (the volume map is built but not read);
+ * PM_OPT_RENDER_MEDIA  1 = pm_render / pm_render_crop / pm_render_cache_crop
+ *                      march the medium (default, when the scene has one),
+ *                      0 = they gather at surface hits only: the surface-gather
+ *                      measurement on the same photon maps (the volume map is
+ *                      built but not read);
  * PM_OPT_CACHE_NEAR_GROUPS the cache render's near-light radius, in merged
  *                      march groups (0..16, default 2; a group is the march
  *                      stride's worth of steps): inside it every march point

@@ -73,6 +73,34 @@ def test_all_fallback_equals_plain(scene, crop):
     assert np.array_equal(u8[..., :3], want)
 
 
+@pytest.mark.parametrize("case", ["surface_only", "cache_surface"])
+def test_all_fallback_other_paths(scene, case):
+    """Threshold 0 through the list paths the default frame does not take:
+    no volume records at all (media off: an empty volume list), and surface
+    records cached too (inference over every record, split by kind inside)."""
+    say("case: cache render with threshold 0, %s" % case)
+    g = scene
+    try:
+        if case == "surface_only":
+            g.set_option(pmamd.PM_OPT_RENDER_MEDIA, 0)
+        g.cache_configure(threshold=0.0, train_samples=16384, batch=16384,
+                          cache_surface=1 if case == "cache_surface" else 0)
+        img = g.render_cache()
+        ref = g.render()
+        s = g.cache_stats()
+    finally:
+        g.set_option(pmamd.PM_OPT_RENDER_MEDIA, 1)
+        g.cache_configure(threshold=0.0, train_samples=16384, batch=16384, cache_surface=0)
+    say("records %d, inferred %d (volume layer 1: %d)" % (s.records, s.inferred, s.inferred_volume))
+    assert s.fallbacks == s.records > 0
+    if case == "surface_only":
+        assert s.inferred == 0
+    else:
+        assert s.inferred == s.records and 0 < s.inferred_volume < s.records
+    err = np.abs(img.astype(np.float64) - ref)
+    assert (err <= 1e-4 * np.abs(ref) + 1e-6).all(), float(err.max())
+
+
 def test_inference_rows(scene):
     """Inference runs over the volume records only (the exact fallback writes
     every surface record), on the 5-step layer 1 under fused mode 4 and the
