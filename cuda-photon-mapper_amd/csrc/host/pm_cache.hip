@@ -187,7 +187,7 @@ int cache_predict_dev(pm_context* c, const float* in, uint32_t n, float* rgb, co
   CacheState& k = c->cache;
   if ((r = sync_ema(c)) != PM_OK) return r;
   const MlpDev wi = mlp_inference_net(k.w);
-  k.infer_rows = list ? nlist : n;
+  k.infer_rows = n;  // the fused list path below infers the listed rows only
   k.infer_short = 0;
   if (k.fused_infer) {
     int maxlds = 0;
@@ -196,6 +196,7 @@ int cache_predict_dev(pm_context* c, const float* in, uint32_t n, float* rgb, co
       const bool split = k.fused_infer == 4;
       if (list) {
         HIPCHK(mlp_infer_fused(wi, in, nlist, rgb, c->stream, list, split));
+        k.infer_rows = nlist;
         k.infer_short = split ? nlist : 0;
         return PM_OK;
       }

@@ -105,12 +105,12 @@ def test_inference_rows(scene):
     """Inference runs over the volume records only (the exact fallback writes
     every surface record), on the 5-step layer 1 under fused mode 4 and the
     full one under mode 3; the frames agree to the rounding of layer 1's
-    sums."""
-    say("case: cache frame inference rows under fused modes 4 and 3")
+    sums.  The two-kernel form (mode 2) infers every record."""
+    say("case: cache frame inference rows under fused modes 4, 3 and 2")
     g = scene
     imgs, stats = [], []
     try:
-        for mode in (4, 3):
+        for mode in (4, 3, 2):
             g.set_option(pmamd.PM_OPT_CACHE_FUSED_INFER, mode)
             g.cache_reset(1)
             g.cache_configure(threshold=0.05, train_samples=16384, batch=4096, march_stride=4)
@@ -119,12 +119,13 @@ def test_inference_rows(scene):
             stats.append(g.cache_stats())
     finally:
         g.set_option(pmamd.PM_OPT_CACHE_FUSED_INFER, 4)
-    s4, s3 = stats
+    s4, s3, s2 = stats
     say("records %d, inferred %d (volume layer 1: %d / %d)" % (s4.records, s4.inferred, s4.inferred_volume,
                                                                 s3.inferred_volume))
     assert 0 < s4.inferred < s4.records
     assert s4.inferred_volume == s4.inferred
     assert s3.inferred == s4.inferred and s3.inferred_volume == 0
+    assert s2.inferred == s2.records and s2.inferred_volume == 0
     d = np.abs(imgs[1].astype(np.float64) - imgs[3])
     assert np.linalg.norm(d) <= 1e-2 * np.linalg.norm(imgs[3])
 
