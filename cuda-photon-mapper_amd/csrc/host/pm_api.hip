@@ -215,11 +215,26 @@ namespace {
 
 int build_map(pm_context* c, MapBufs& mb) {
   DevMap& m = mb.m;
-  std::vector<uint32_t> lo, hi, lc, lb;
-  shape_tables(m.n, m.L, lo, hi, lc, lb);
+  if (!mb.shape.p || mb.shape_n != m.n || mb.shape_L != m.L) {
+    // the shape tables depend on (n, L) only: computed and uploaded once per size
+    std::vector<uint32_t> lo, hi, lc, lb;
+    shape_tables(m.n, m.L, lo, hi, lc, lb);
+    const size_t stride = kd_shape_table_words(m.n, m.L) / 4;
+    std::vector<uint32_t> t(4 * stride, 0u);
+    for (size_t i = 0; i < lo.size(); ++i) {
+      t[i] = lo[i];
+      t[stride + i] = hi[i];
+      t[2 * stride + i] = lc[i];
+      t[3 * stride + i] = lb[i];
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));  // no build still reads the old tables
+    HIPCHK(mb.shape.ensure(4ull * t.size()));
+    HIPCHK(hipMemcpy(mb.shape.p, t.data(), 4ull * t.size(), hipMemcpyHostToDevice));
+    mb.shape_n = m.n;
+    mb.shape_L = m.L;
+  }
   HIPCHK(c->scratch.ensure(4ull * kdbuild_temp_words(m.n, m.L)));
-  HIPCHK(kd_build_impl(m, c->scratch.as<uint32_t>(), lo.data(), hi.data(), lc.data(), lb.data(), c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));  // host tables must outlive the copies
+  HIPCHK(kd_build_impl(m, c->scratch.as<uint32_t>(), mb.shape.as<uint32_t>(), c->stream));
   return PM_OK;
 }
 

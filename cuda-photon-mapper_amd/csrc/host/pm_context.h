@@ -41,10 +41,13 @@ struct DevBuf {
 
 struct MapBufs {
   DevBuf x, y, z, payload, tpts, nodes, nrange, bmin, bmax, tpayload;
+  DevBuf shape;                         // the kd-tree's shape tables for (shape_n, shape_L)
+  uint32_t shape_n = 0, shape_L = 0;
   DevMap m;
   void release() {
     x.release(); y.release(); z.release(); payload.release(); tpts.release(); nodes.release(); nrange.release();
-    bmin.release(); bmax.release(); tpayload.release();
+    bmin.release(); bmax.release(); tpayload.release(); shape.release();
+    shape_n = shape_L = 0;
   }
 };
 

@@ -383,14 +383,20 @@ hipError_t kd_build_forest(const KdForest& F, const uint32_t* level_off, uint32_
 size_t kdbuild_temp_words(uint32_t n, uint32_t L) {
   uint32_t nl = kd_num_leaves(n, L);
   size_t ni = nl > 0 ? nl - 1 : 0;
-  return kdforest_temp_words(n, (uint32_t)ni) + 4 * (ni + 1);
+  return kdforest_temp_words(n, (uint32_t)ni);
+}
+
+size_t kd_shape_table_words(uint32_t n, uint32_t L) {
+  uint32_t nl = kd_num_leaves(n, L);
+  size_t ni = nl > 0 ? nl - 1 : 0;
+  return 4 * (ni + 1);
 }
 
 // Single tree: m.tpts, m.nodes, m.nrange.  temp must hold
-// kdbuild_temp_words(n, L) words; host_lo/host_hi/host_lc/host_lb are the
-// shape tables of the nl-1 internal nodes (heap order = level order).
-hipError_t kd_build_impl(DevMap& m, uint32_t* temp, const uint32_t* host_lo, const uint32_t* host_hi,
-                         const uint32_t* host_lc, const uint32_t* host_lb, hipStream_t st) {
+// kdbuild_temp_words(n, L) words; `tables` holds the device shape tables of
+// the nl-1 internal nodes (heap order = level order), lo, hi, lc and lb at a
+// stride of nl words.
+hipError_t kd_build_impl(DevMap& m, uint32_t* temp, const uint32_t* tables, hipStream_t st) {
   const uint32_t n = m.n;
   const uint32_t nl = kd_num_leaves(n, m.L);
   m.nl = nl;
@@ -399,18 +405,11 @@ hipError_t kd_build_impl(DevMap& m, uint32_t* temp, const uint32_t* host_lo, con
     return hipSuccess;
   }
   const size_t ni = nl - 1;
-  uint32_t* tab = temp + kdforest_temp_words(n, (uint32_t)ni);
-  uint32_t* node_lo = tab;
-  uint32_t* node_hi = node_lo + ni + 1;
-  uint32_t* node_lc = node_hi + ni + 1;
-  uint32_t* node_lb = node_lc + ni + 1;
+  const uint32_t* node_lo = tables;
+  const uint32_t* node_hi = node_lo + ni + 1;
+  const uint32_t* node_lc = node_hi + ni + 1;
+  const uint32_t* node_lb = node_lc + ni + 1;
   hipError_t e;
-  if (ni > 0) {
-    if ((e = hipMemcpyAsync(node_lo, host_lo, 4 * ni, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(node_hi, host_hi, 4 * ni, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(node_lc, host_lc, 4 * ni, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(node_lb, host_lb, 4 * ni, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
-  }
   KdForest F;
   memset(&F, 0, sizeof(F));
   F.n = n;

@@ -123,10 +123,12 @@ size_t kdforest_temp_words(uint32_t n, uint32_t nnodes);
 hipError_t kd_build_forest(const KdForest& F, const uint32_t* level_off, uint32_t nlevels, uint32_t* temp,
                            hipStream_t st);
 size_t kdbuild_temp_words(uint32_t n, uint32_t L);
-// Shape tables (host arrays of the nl-1 internal nodes): point range [lo, hi),
-// left count lc, and lb = left counts of the same level's earlier nodes.
-hipError_t kd_build_impl(DevMap& m, uint32_t* temp, const uint32_t* host_lo, const uint32_t* host_hi,
-                         const uint32_t* host_lc, const uint32_t* host_lb, hipStream_t st);
+// Shape tables of the ni = nl-1 internal nodes, in device memory: point range
+// [lo, hi), left count lc, and lb = left counts of the same level's earlier
+// nodes, each at a stride of ni + 1 words (kd_shape_table_words in all).  They
+// depend on n and L only, so a caller keeps them across builds of that size.
+size_t kd_shape_table_words(uint32_t n, uint32_t L);
+hipError_t kd_build_impl(DevMap& m, uint32_t* temp, const uint32_t* tables, hipStream_t st);
 // Tight boxes of every node's points into m.bmin / m.bmax (when allocated).
 hipError_t kd_node_boxes(const DevMap& m, hipStream_t st);
 // The payload in tree order into m.tpayload (when allocated).
