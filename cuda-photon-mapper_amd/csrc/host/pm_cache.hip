@@ -139,7 +139,7 @@ void cache_release(CacheState& k) {
   for (DevBuf* b : {&k.params, &k.grads, &k.adam_m, &k.adam_v, &k.params_bf16, &k.params_ema, &k.ema_bf16, &k.X, &k.y, &k.dy,
                     &k.row_err, &k.rgb, &k.err_sum, &k.err_cnt, &k.rec_cnt, &k.rec_off, &k.rec_in,
                     &k.rec_wt, &k.rec_code, &k.rec_val, &k.sel_in, &k.sel_t, &k.list, &k.list_cnt, &k.cell_fb,
-                    &k.row_code, &k.row_list, &k.loss_part, &k.loss_acc,
+                    &k.row_code, &k.row_list, &k.loss_part, &k.loss_acc, &k.rec_q, &k.map_hash,
                     &k.XT, &k.dT[0], &k.dT[1], &k.dT[2], &k.dT[3], &k.dyT, &k.tpart})
     b->release();
   for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) {
@@ -504,6 +504,11 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   HIPCHK(k.cell_fb.ensure(CACHE_CELLS));
   HIPCHK(launch_cache_cell_fb(k.err_sum.as<unsigned long long>(), k.err_cnt.as<unsigned int>(), k.threshold,
                               k.cell_fb.as<uint8_t>(), st));
+  // the maps' fingerprints, read with the fallback lists' lengths below
+  HIPCHK(k.map_hash.ensure(16));
+  unsigned long long* mh = (unsigned long long*)k.map_hash.p;
+  HIPCHK(launch_point_hash(c->ms.m.tpts, c->ms.m.n, mh, st));
+  HIPCHK(launch_point_hash(c->mv.m.tpts, c->mv.m.n, mh + 1, st));
   CacheListRule fbr;
   fbr.fallback = 1;
   fbr.cache_surface = k.cache_surface;
@@ -512,6 +517,25 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   r = build_lists(c, k.rec_code.as<uint16_t>(), N, fbr, k.list.as<uint32_t>(), nk2);
   if (r != PM_OK) return r;
   const uint32_t nfb = nk2[0] + nk2[1];
+  // seeds of the counting first pass: each fallback record's bucket from the
+  // previous frame, kept while the records and the photon positions are
+  // unchanged (the same record count and map fingerprints; a static scene
+  // re-emitted with the same seed, or a reused map).  Any seed gives the same
+  // values, but seeds from other photon positions miss often and cost more
+  // than the heap passes they replace, so they are dropped.
+  uint16_t* seeds = nullptr;
+  if (mode7 && nfb > 0) {
+    unsigned long long h[2] = {0ull, 0ull};
+    HIPCHK(hipMemcpy(h, mh, 16, hipMemcpyDeviceToHost));
+    if (k.rec_q_n != N || !k.rec_q.p || h[0] != k.rec_q_hash[0] || h[1] != k.rec_q_hash[1]) {
+      HIPCHK(k.rec_q.ensure(2ull * std::max<uint64_t>(N, 1)));
+      HIPCHK(hipMemsetAsync(k.rec_q.p, 0xff, 2ull * N, st));
+      k.rec_q_n = N;
+      k.rec_q_hash[0] = h[0];
+      k.rec_q_hash[1] = h[1];
+    }
+    seeds = k.rec_q.as<uint16_t>();
+  }
   for (int kind = 1; kind <= 2; ++kind) {  // one launch per kind of row
     if (nk2[kind - 1] == 0) continue;
     CacheValueLaunch vl;
@@ -524,6 +548,7 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
     vl.out_by_list = 1;
     vl.mode7 = mode7;
     vl.kind = kind;
+    vl.seed_q = seeds;
     HIPCHK(launch_cache_values(vl, st));
   }
   HIPCHK(hipEventRecord(ev[4], st));

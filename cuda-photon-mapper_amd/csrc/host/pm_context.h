@@ -74,6 +74,10 @@ struct CacheState {
                            // for every row; 2 = layer 1 + layers 2..5 on 16x16x32 MFMA, 1 = 16x16x16; 0: generic GEMMs
   DevBuf row_code, row_list;  // pm_cache_predict under mode 4: the rows' codes and kind lists
   DevBuf loss_part, loss_acc;  // partial sums of the training loss, and its sum
+  DevBuf rec_q;               // per record, the k-th neighbour's bucket of its last exact gather (seeds)
+  uint64_t rec_q_n = 0;       // the record count rec_q was kept for
+  DevBuf map_hash;            // fingerprints of the two maps' point positions (this frame)
+  unsigned long long rec_q_hash[2] = {0ull, 0ull};  // ... and of the maps rec_q was kept for
   uint64_t infer_rows = 0;    // rows of the last inference, and of them on the 5-step layer 1
   uint64_t infer_short = 0;
   int infer_chunk_log2 = 20;  // rows per fused-inference launch pair (log2)

@@ -236,12 +236,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5))) k_
   V3f x = v3(p[0], p[1], p[2]), d = v3(p[3], p[4], p[5]), nrm = v3(p[6], p[7], p[8]);
   V3f c;
   const bool surf = KIND == 0 ? !(nrm.x == 0.0f && nrm.y == 0.0f && nrm.z == 0.0f) : KIND == 1;
+  // the seed bucket of this record from the previous frame (none: heap pass)
+  using G = LaneGather5<false, true>;
+  const uint32_t seed = v.seed_q ? (uint32_t)v.seed_q[rec] : G::kNoSeed;
+  uint32_t q = G::kNoSeed;
   if (surf) {
-    c = g.surface_lone(x, nrm, one);
+    c = g.surface_lone(x, nrm, one, seed, &q);
   } else {
-    (void)g.volume_defer(x, d, one, c);  // nothing pending: queues x
-    (void)g.volume_flush(d, c);
+    c = g.volume_lone(x, d, one, seed, &q);
   }
+  if (v.seed_q) v.seed_q[rec] = (uint16_t)q;
   put_value(v, o, c);
 }
 
@@ -501,6 +505,38 @@ __global__ void __launch_bounds__(256) k_count_total(const uint32_t* __restrict_
 }
 
 __global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+// *out += a 64-bit fingerprint of n points' positions (the wrapping sum of a
+// mix of each point's coordinates and position in the array): equal for equal
+// arrays, in any summation order.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+__global__ void __launch_bounds__(256) k_point_hash(const float4* __restrict__ p, uint32_t n,
+                                                    unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long part[256 / 64];
+  unsigned long long s = 0ull;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    const float4 q = p[i];
+    const unsigned long long a = ((unsigned long long)f2u(q.x) << 32) | f2u(q.y);
+    s += mix64(a ^ mix64(((unsigned long long)f2u(q.z) << 32) | i));
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) s += __shfl_xor(s, d);
+  if ((threadIdx.x & 63u) == 0u) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
+}
+
+hipError_t launch_point_hash(const float4* p, uint32_t n, unsigned long long* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, out);
+  if (n == 0) return hipGetLastError();
+  const uint32_t blocks = std::max(1u, std::min(1024u, (n + 2047u) / 2048u));
+  hipLaunchKernelGGL(k_point_hash, dim3(blocks), dim3(256), 0, st, p, n, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_count_total(const uint32_t* cnt, uint32_t n, unsigned long long* total, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, total);

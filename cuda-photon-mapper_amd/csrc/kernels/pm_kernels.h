@@ -300,6 +300,9 @@ struct CacheValueLaunch {
   int out_by_list;           // 1: out[list[i]], 0: out[i]
   int mode7;                 // 1: the gather of mode 7 (k_values7), 0: the 2-byte heap gather (k_values)
   int kind;                  // rows are 0: of either kind, 1: all surface queries, 2: all volume queries
+  uint16_t* seed_q;          // k_values7, rows with out_by_list: per record, the k-th neighbour's bucket
+                             // of its last gather (read as the counting pass's seed, then rewritten;
+                             // 0xffff: none); null: heap passes
 };
 hipError_t launch_cache_record(const CacheRecordLaunch& r, hipStream_t st);
 hipError_t launch_cache_values(const CacheValueLaunch& v, hipStream_t st);
@@ -309,6 +312,9 @@ hipError_t launch_cache_select(const float* in, const uint32_t* list_s, uint32_t
 // sum_rows_parts() doubles of scratch
 size_t sum_rows_parts();
 hipError_t launch_sum_rows(const float* v, uint32_t n, double* acc, double* part, hipStream_t st);
+// *out = a 64-bit fingerprint of the positions of n points (equal arrays,
+// equal fingerprints)
+hipError_t launch_point_hash(const float4* p, uint32_t n, unsigned long long* out, hipStream_t st);
 // *total = cnt[0] + ... + cnt[n - 1] in 64 bits
 hipError_t launch_count_total(const uint32_t* cnt, uint32_t n, unsigned long long* total, hipStream_t st);
 // cell_fb[c] = 1 when error cell c saw no training row or its mean squared

@@ -384,6 +384,26 @@ struct SumOnlyVisitor {
   }
 };
 
+// A counting first pass on its own (a point gathered alone, seeded with a
+// bucket near its k-th neighbour's): the window's bound is fixed, so it runs
+// on the speculative traversal with node boxes.
+struct HistOnlyVisitor {
+  V3f q;
+  HistVisitor* hv;
+  __device__ bool far(int a, float split) const {
+    const float d = axis_of(q, a) - split;
+    return d * d <= u2f(hv->bound());
+  }
+  __device__ bool near_box(float4 mn, float4 mx) const { return box_d2(q, mn, mx) <= u2f(hv->bound()); }
+  template <int NB>
+  __device__ void points(const float4* p, int cnt, uint32_t) {
+    uint32_t d[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) d[j] = f2u(dist2(q.x, q.y, q.z, p[j].x, p[j].y, p[j].z));
+    hv->template points<NB>(d, cnt);
+  }
+};
+
 // The heap pass of one point on the speculative traversal with node boxes.
 // Its bound shrinks as the heap fills, so a lane decides on far children and
 // kept leaves with the bound of that moment, which may be larger than the one
@@ -500,10 +520,48 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
     this->pend_w = w;
   }
 
-  // A surface query on its own, without the cone filter: heap pass and second
-  // pass on the speculative traversal, then the bucket selection of
-  // knn16_gather (pm_lane_knn3.h); the same value as surface() with CONE off.
-  __device__ V3f surface_lone(V3f x, V3f n, V3f albedo) {
+  // No seed bucket (a seed is a 16-bit bucket, at most 0x7f80).
+  static constexpr uint32_t kNoSeed = 0xffffu;
+
+  // The first pass of x on its own from a seed bucket Qs near its k-th
+  // neighbour's (the same query's bucket in an earlier frame): counting in the
+  // window around Qs.  On success kv and hp.size are what the heap pass would
+  // leave (Q, or the count when fewer than k points lie within the cap);
+  // false when the k-th neighbour lies outside the window or a counter
+  // wrapped, and the heap pass must run.  Any seed gives the same result.
+  __device__ bool seeded_count(const GatherMap& gm, V3f x, uint32_t Qs, uint32_t caph, Knn16Visitor& kv) {
+    HistVisitor hv = window(Qs, caph);
+    hv.clear();
+    HistOnlyVisitor w{x, &hv};
+    lane_visit_dual_spec<4, TOP>(gm, x.x, x.y, x.z, w);
+    const uint32_t k = (uint32_t)gm.k;
+    uint32_t Q;
+    if (hv.select(k, Q)) {
+      kv.full = true;
+      kv.root = Q;
+      this->hp.size = gm.k;
+      return true;
+    }
+    if (hv.top_h == caph && hv.below + hv.inwin < k) {
+      this->hp.size = (int)(hv.below + hv.inwin);
+      return true;
+    }
+    return false;
+  }
+
+  // The seed for the next gather of the same query: Q, or the cap's bucket
+  // when fewer than k points lie within the cap (the window then reaches the
+  // cap and counts them all).
+  __device__ uint32_t next_seed(const Knn16Visitor& kv, uint32_t caph) const {
+    return kv.full ? kv.root : dq16(caph);
+  }
+
+  // A surface query on its own, without the cone filter: heap pass (or, from
+  // a seed bucket, the counting pass) and second pass on the speculative
+  // traversal, then the bucket selection of knn16_gather (pm_lane_knn3.h); the
+  // same value as surface() with CONE off, whatever the seed.  *qout (when
+  // set) gets the seed for the next gather of the same query.
+  __device__ V3f surface_lone(V3f x, V3f n, V3f albedo, uint32_t seed = kNoSeed, uint32_t* qout = nullptr) {
     static_assert(!CONE, "surface_lone has no cone filter");
     ++this->nsurf;
     const GatherMap& gs = this->gs;
@@ -511,8 +569,12 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
     this->hp.k = gs.k;
     this->hp.size = 0;
     Knn16Visitor kv{&this->hp, caph, 0u, false};
-    HeapSpecVisitor hs{x, &kv};
-    lane_visit_dual_spec<4, TOP>(gs, x.x, x.y, x.z, hs);
+    if (seed == kNoSeed || !seeded_count(gs, x, seed, caph, kv)) {
+      this->hp.size = 0;
+      HeapSpecVisitor hs{x, &kv};
+      lane_visit_dual_spec<4, TOP>(gs, x.x, x.y, x.z, hs);
+    }
+    if (qout) *qout = next_seed(kv, caph);
     if (this->hp.size == 0) return v3(0.0f, 0.0f, 0.0f);
     SurfAddT add{&gs, n, {}};
     nsum3_init(add.s);
@@ -552,6 +614,27 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
     V3f sum = nsum3_get(add.s);
     V3f sc = v3(gs.scale[0], gs.scale[1], gs.scale[2]);
     return mul(mul(sum, albedo), sc) * surface_norm(r2, false);
+  }
+
+  // A volume query on its own (nothing pending): the heap pass, or the
+  // counting pass from a seed bucket, then the second pass; the value of
+  // volume_defer + volume_flush, whatever the seed.  *qout as surface_lone.
+  __device__ V3f volume_lone(V3f x, V3f wo, V3f w, uint32_t seed, uint32_t* qout) {
+    ++this->nvol;
+    const GatherMap& gv = this->gv;
+    const uint32_t caph = f2u(gv.rcap2);
+    this->hp.k = gv.k;
+    this->hp.size = 0;
+    Knn16Visitor kv{&this->hp, caph, 0u, false};
+    if (seed == kNoSeed || !seeded_count(gv, x, seed, caph, kv)) {
+      this->hp.size = 0;
+      this->heap_pass(x, caph, caph, kv);
+    }
+    if (qout) *qout = next_seed(kv, caph);
+    queue(x, w, kv);
+    V3f c = v3(0.0f, 0.0f, 0.0f);
+    (void)volume_flush(wo, c);
+    return c;
   }
 
   // Surface queries of the march: surface_lone without the cone filter.

@@ -73,6 +73,43 @@ def test_all_fallback_equals_plain(scene, crop):
     assert np.array_equal(u8[..., :3], want)
 
 
+def test_seeded_fallback_equals_plain():
+    """The exact fallback's first passes count from each record's bucket of
+    the previous frame while the records and the photon positions are
+    unchanged, and run the heap pass otherwise.  Every frame equals the plain
+    render: the first (heap passes), the second (counting from the first
+    frame's buckets), and one on moved photons with the same records (the
+    library drops the old map's buckets)."""
+    say("case: cache render with threshold 0, seeded from the previous frame, then from another map")
+    c = pmamd.preset(5)
+    c.width, c.height = 64, 48
+    g = pmamd.PhotonMapper(c)
+    g.emit()
+    g.build()
+    g.cache_reset(1)
+    g.cache_configure(threshold=0.0, train_samples=16384, batch=16384)
+    ref = g.render().astype(np.float64)
+    for frame in range(2):
+        img = g.render_cache()
+        err = np.abs(img.astype(np.float64) - ref)
+        assert (err <= 1e-4 * np.abs(ref) + 1e-6).all(), (frame, float(err.max()))
+    n_records = g.cache_stats().records
+    # another map: every record moved by up to 0.02 (the seeds are dropped)
+    rng = np.random.default_rng(3)
+    for which in (0, 1):
+        pos, power, dirs = g.records(which)
+        pos = np.clip(pos + rng.uniform(-0.02, 0.02, pos.shape).astype(np.float32), 0.001, 0.999)
+        g.set_records(which, pos, power, dirs, g.stats().n_emit_surface if which == 0 else g.stats().n_emit_volume)
+    g.build()
+    ref2 = g.render().astype(np.float64)
+    img = g.render_cache()
+    assert g.cache_stats().records == n_records
+    err = np.abs(img.astype(np.float64) - ref2)
+    assert (err <= 1e-4 * np.abs(ref2) + 1e-6).all(), float(err.max())
+    assert np.abs(ref2 - ref).max() > 0
+    g.close()
+
+
 @pytest.mark.parametrize("case", ["surface_only", "cache_surface"])
 def test_all_fallback_other_paths(scene, case):
     """Threshold 0 through the list paths the default frame does not take:
