@@ -404,6 +404,41 @@ struct HistOnlyVisitor {
   }
 };
 
+// The second pass from a seed bucket, counting besides the bucket-Q points
+// within the cap (n_eq): with n_less < k <= n_less + n_eq the seed is the
+// k-th neighbour's bucket and the sums are those of the second pass from it.
+template <class Add>
+struct Sum16SeedVisitor : Sum16CVisitor<Add> {
+  uint32_t n_eq;
+  template <int NB>
+  __device__ void points(const float4* p, const uint32_t* dh, int cnt, uint32_t base) {
+    if (!this->inclusive) {
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+        n_eq += j < cnt && dq16(dh[j]) == this->Q && dh[j] <= this->cap_h ? 1u : 0u;
+    }
+    Sum16CVisitor<Add>::template points<NB>(p, dh, cnt, base);
+  }
+};
+
+template <class Add>
+struct SeedSumVisitor {
+  V3f q;
+  Sum16SeedVisitor<Add>* sv;
+  __device__ bool far(int a, float split) const {
+    const float d = axis_of(q, a) - split;
+    return d * d <= u2f(sv->bound());
+  }
+  __device__ bool near_box(float4 mn, float4 mx) const { return box_d2(q, mn, mx) <= u2f(sv->bound()); }
+  template <int NB>
+  __device__ void points(const float4* p, int cnt, uint32_t base) {
+    uint32_t d[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) d[j] = f2u(dist2(q.x, q.y, q.z, p[j].x, p[j].y, p[j].z));
+    sv->template points<NB>(p, d, cnt, base);
+  }
+};
+
 // The heap pass of one point on the speculative traversal with node boxes.
 // Its bound shrinks as the heap fills, so a lane decides on far children and
 // kept leaves with the bound of that moment, which may be larger than the one
@@ -520,8 +555,44 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
     this->pend_w = w;
   }
 
-  // No seed bucket (a seed is a 16-bit bucket, at most 0x7f80).
+  // Seeds are 16-bit buckets (at most 0x7f80), or: no seed, or fewer than k
+  // points within the cap.
   static constexpr uint32_t kNoSeed = 0xffffu;
+  static constexpr uint32_t kFewSeed = 0xfffeu;
+
+  // One pass from a seed that the query's own previous gather left (the same
+  // query on the same photon positions gives the same seed): the second pass
+  // from the seed's bucket (or over the cap for kFewSeed), checked against the
+  // counts it gathers.  True when the seed was right: kv, hp.size and the sums
+  // in sv / add are then those of the first and second passes.  False: sv and
+  // add hold nothing usable.
+  template <class Add>
+  __device__ bool seeded_sums(const GatherMap& gm, V3f x, uint32_t seed, uint32_t caph, Knn16Visitor& kv,
+                              Sum16SeedVisitor<Add>& sv, Add& add) {
+    const uint32_t k = (uint32_t)gm.k;
+    const bool few = seed == kFewSeed;
+    if (!few && seed > dq16(caph)) return false;  // not a bucket within the cap
+    sv.Q = few ? 0u : seed;
+    sv.cap_h = caph;
+    sv.inclusive = few;
+    sv.n_less = 0u;
+    sv.d0 = sv.i0 = sv.d1 = sv.i1 = LKEY_INF;
+    sv.add = &add;
+    sv.n_eq = 0u;
+    SeedSumVisitor<Add> w{x, &sv};
+    lane_visit_dual_spec<4, TOP>(gm, x.x, x.y, x.z, w);
+    if (few) {
+      if (sv.n_less >= k) return false;
+      kv.full = false;
+      this->hp.size = (int)sv.n_less;
+      return true;
+    }
+    if (sv.n_less >= k || sv.n_less + sv.n_eq < k) return false;
+    kv.full = true;
+    kv.root = seed;
+    this->hp.size = gm.k;
+    return true;
+  }
 
   // The first pass of x on its own from a seed bucket Qs near its k-th
   // neighbour's (the same query's bucket in an earlier frame): counting in the
@@ -552,9 +623,10 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
   // The seed for the next gather of the same query: Q, or the cap's bucket
   // when fewer than k points lie within the cap (the window then reaches the
   // cap and counts them all).
-  __device__ uint32_t next_seed(const Knn16Visitor& kv, uint32_t caph) const {
-    return kv.full ? kv.root : dq16(caph);
-  }
+  __device__ uint32_t next_seed(const Knn16Visitor& kv) const { return kv.full ? kv.root : kFewSeed; }
+  // The counting window for a seed: around the seed's bucket, or reaching the
+  // cap for kFewSeed (it then counts every point within the cap).
+  __device__ uint32_t window_seed(uint32_t seed, uint32_t caph) const { return seed == kFewSeed ? dq16(caph) : seed; }
 
   // A surface query on its own, without the cone filter: heap pass (or, from
   // a seed bucket, the counting pass) and second pass on the speculative
@@ -569,24 +641,29 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
     this->hp.k = gs.k;
     this->hp.size = 0;
     Knn16Visitor kv{&this->hp, caph, 0u, false};
-    if (seed == kNoSeed || !seeded_count(gs, x, seed, caph, kv)) {
-      this->hp.size = 0;
-      HeapSpecVisitor hs{x, &kv};
-      lane_visit_dual_spec<4, TOP>(gs, x.x, x.y, x.z, hs);
-    }
-    if (qout) *qout = next_seed(kv, caph);
-    if (this->hp.size == 0) return v3(0.0f, 0.0f, 0.0f);
     SurfAddT add{&gs, n, {}};
     nsum3_init(add.s);
-    Sum16CVisitor<SurfAddT> sv;
-    sv.Q = kv.root;
-    sv.cap_h = caph;
-    sv.inclusive = !kv.full;
-    sv.n_less = 0u;
-    sv.d0 = sv.i0 = sv.d1 = sv.i1 = LKEY_INF;
-    sv.add = &add;
-    SumOnlyVisitor<SurfAddT> w{x, &sv};
-    lane_visit_dual_spec<4, TOP>(gs, x.x, x.y, x.z, w);
+    Sum16SeedVisitor<SurfAddT> sv;
+    if (seed == kNoSeed || !seeded_sums(gs, x, seed, caph, kv, sv, add)) {
+      if (seed == kNoSeed || !seeded_count(gs, x, window_seed(seed, caph), caph, kv)) {
+        this->hp.size = 0;
+        HeapSpecVisitor hs{x, &kv};
+        lane_visit_dual_spec<4, TOP>(gs, x.x, x.y, x.z, hs);
+      }
+      if (this->hp.size > 0) {
+        nsum3_init(add.s);
+        sv.Q = kv.root;
+        sv.cap_h = caph;
+        sv.inclusive = !kv.full;
+        sv.n_less = 0u;
+        sv.d0 = sv.i0 = sv.d1 = sv.i1 = LKEY_INF;
+        sv.add = &add;
+        SumOnlyVisitor<SurfAddT> w{x, &sv};
+        lane_visit_dual_spec<4, TOP>(gs, x.x, x.y, x.z, w);
+      }
+    }
+    if (qout) *qout = next_seed(kv);
+    if (this->hp.size == 0) return v3(0.0f, 0.0f, 0.0f);
     float r2 = u2f(caph);
     if (kv.full) {
       const uint32_t need = (uint32_t)gs.k - sv.n_less;
@@ -626,13 +703,26 @@ struct LaneGather5 : LaneGather4<CONE, 4> {
     this->hp.k = gv.k;
     this->hp.size = 0;
     Knn16Visitor kv{&this->hp, caph, 0u, false};
-    if (seed == kNoSeed || !seeded_count(gv, x, seed, caph, kv)) {
+    V3f c = v3(0.0f, 0.0f, 0.0f);
+    if (seed != kNoSeed) {
+      VolAdd add{&gv, wo, this->hg_g, {}};
+      nsum3_init(add.s);
+      Sum16SeedVisitor<VolAdd> sv;
+      if (seeded_sums(gv, x, seed, caph, kv, sv, add)) {
+        // the pending point's second pass is done: finish it as volume_flush
+        if (qout) *qout = next_seed(kv);
+        queue(x, w, kv);
+        this->pend = false;
+        if (this->pend_m == 0) return mul(this->pend_w, v3(0.0f, 0.0f, 0.0f));
+        return this->finish_pending(sv, add);
+      }
+    }
+    if (seed == kNoSeed || !seeded_count(gv, x, window_seed(seed, caph), caph, kv)) {
       this->hp.size = 0;
       this->heap_pass(x, caph, caph, kv);
     }
-    if (qout) *qout = next_seed(kv, caph);
+    if (qout) *qout = next_seed(kv);
     queue(x, w, kv);
-    V3f c = v3(0.0f, 0.0f, 0.0f);
     (void)volume_flush(wo, c);
     return c;
   }
