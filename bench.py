@@ -243,6 +243,15 @@ def run_cache(steps, train_frames, stride, thresholds, batch, rows, near_groups=
         for i in range(extra_error_frames):
             frame()
             more_rows.append(rgb.download(np.float32, npx * 3).reshape(cfg.height, cfg.width, 3)[rows].copy())
+        # the same frames without the fallback's seeds from the previous frame
+        # (as when the photon positions change every frame): heap passes
+        g.set_option(pmamd.PM_OPT_CACHE_SEEDS, 0)
+        t2 = time.perf_counter()
+        fb_noseed = []
+        for i in range(steps):
+            fb_noseed.append(frame().ms_fallback)
+        t3 = time.perf_counter()
+        g.set_option(pmamd.PM_OPT_CACHE_SEEDS, 1)
         m = np.mean(np.array(per), axis=0)
         # algorithmic FLOPs of the network per row: 2 x (216 x 128 + 3 x 128^2 + 128 x 3) on the full
         # layer 1; 2 x (144 x 128 + ...) for volume rows, whose 72 normal features are constants
@@ -253,6 +262,8 @@ def run_cache(steps, train_frames, stride, thresholds, batch, rows, near_groups=
             "breakdown_ms": {"record": round(m[0], 3), "train": round(m[1], 3), "infer": round(m[2], 3),
                              "fallback": round(m[3], 3), "accumulate": round(m[4], 3),
                              "render_total": round(m[5], 3)},
+            "without_fallback_seeds": {"ms_per_frame": round((t3 - t2) / steps * 1e3, 3),
+                                       "fallback_ms": round(float(np.mean(fb_noseed)), 3)},
             "cache_queries": int(cs.records), "fallback_queries": int(cs.fallbacks),
             "train_loss": round(cs.train_loss, 5),
             "mlp_inference_mfma": {"bound": "mfma", "achieved": round(infer_flop / (m[2] * 1e-3) / 1e12, 2),
@@ -618,6 +629,8 @@ def main():
                            err_rows, args.cache_near, args.cache_error_frames, args.cache_ema)
         for st in ns["settings"]:
             st["speedup_vs_plain_fp32_frame"] = round(out["ms_per_step"] / st["ms_per_frame"], 3)
+            ws = st["without_fallback_seeds"]
+            ws["speedup_vs_plain_fp32_frame"] = round(out["ms_per_step"] / ws["ms_per_frame"], 3)
             st["rmse_vs_plain_fp32_same_samples"] = {
                 k: v for k, v in row_errors(st["_rows"], plain_rows.astype(np.float64), cfg.exposure).items()
                 if k.startswith("rmse")}

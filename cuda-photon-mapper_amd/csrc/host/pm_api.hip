@@ -610,6 +610,10 @@ int pm_set_option(pm_context* c, int option, int value) {
       c->cache.ema_decay = (float)value / 1000.0f;
       c->cache.ema_stale = true;
       return PM_OK;
+    case PM_OPT_CACHE_SEEDS:
+      if (value != 0 && value != 1) return PM_ERR_INVALID;
+      c->cache.use_seeds = value;
+      return PM_OK;
     case PM_OPT_CACHE_NEAR_GROUPS:
       if (value < 0 || value > 16) return PM_ERR_INVALID;
       c->cache.near_groups = value;

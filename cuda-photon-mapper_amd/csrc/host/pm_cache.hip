@@ -507,8 +507,10 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   // the maps' fingerprints, read with the fallback lists' lengths below
   HIPCHK(k.map_hash.ensure(16));
   unsigned long long* mh = (unsigned long long*)k.map_hash.p;
-  HIPCHK(launch_point_hash(c->ms.m.tpts, c->ms.m.n, mh, st));
-  HIPCHK(launch_point_hash(c->mv.m.tpts, c->mv.m.n, mh + 1, st));
+  if (mode7 && k.use_seeds) {
+    HIPCHK(launch_point_hash(c->ms.m.tpts, c->ms.m.n, mh, st));
+    HIPCHK(launch_point_hash(c->mv.m.tpts, c->mv.m.n, mh + 1, st));
+  }
   CacheListRule fbr;
   fbr.fallback = 1;
   fbr.cache_surface = k.cache_surface;
@@ -524,7 +526,7 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   // values, but seeds from other photon positions miss often and cost more
   // than the heap passes they replace, so they are dropped.
   uint16_t* seeds = nullptr;
-  if (mode7 && nfb > 0) {
+  if (mode7 && nfb > 0 && k.use_seeds) {
     unsigned long long h[2] = {0ull, 0ull};
     HIPCHK(hipMemcpy(h, mh, 16, hipMemcpyDeviceToHost));
     if (k.rec_q_n != N || !k.rec_q.p || h[0] != k.rec_q_hash[0] || h[1] != k.rec_q_hash[1]) {

@@ -76,6 +76,7 @@ struct CacheState {
   DevBuf loss_part, loss_acc;  // partial sums of the training loss, and its sum
   DevBuf rec_q;               // per record, the k-th neighbour's bucket of its last exact gather (seeds)
   uint64_t rec_q_n = 0;       // the record count rec_q was kept for
+  int use_seeds = 1;          // PM_OPT_CACHE_SEEDS
   DevBuf map_hash;            // fingerprints of the two maps' point positions (this frame)
   unsigned long long rec_q_hash[2] = {0ull, 0ull};  // ... and of the maps rec_q was kept for
   uint64_t infer_rows = 0;    // rows of the last inference, and of them on the 5-step layer 1

@@ -40,6 +40,7 @@ PM_OPT_VOXEL_GRID = 8
 PM_OPT_TOP_LEVELS = 9
 PM_OPT_CACHE_NEAR_GROUPS = 10
 PM_OPT_CACHE_EMA = 11
+PM_OPT_CACHE_SEEDS = 12
 VOXEL_TABLE_ENTRIES = 3 * 33 ** 3  # summed-volume table of one map's 32^3 voxel grid
 
 

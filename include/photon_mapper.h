@@ -239,6 +239,11 @@ int pm_set_render(pm_context* ctx, int width, int height, int spp, int k_surface
  *                      decay d / 1000 per Adam step (default 950; training
  *                      itself is unchanged; setting the parameters or a reset
  *                      restarts the average);
+ * PM_OPT_CACHE_SEEDS   1 = the cache render's exact fallback (gather modes 7
+ *                      and auto) starts each record from its k-th neighbour's
+ *                      bucket of the previous call while the records and the
+ *                      photon positions are unchanged (default), 0 = heap
+ *                      passes every call; the values are the same bits;
  * PM_OPT_TOP_LEVELS    gather mode 7 (also when auto picks it): the top 0..7
  *                      levels of both kd-trees (2^levels - 1 nodes, 48 B each)
  *                      staged in each wave's LDS for the traversals to read,
@@ -267,7 +272,8 @@ enum {
   PM_OPT_VOXEL_GRID = 8,
   PM_OPT_TOP_LEVELS = 9,
   PM_OPT_CACHE_NEAR_GROUPS = 10,
-  PM_OPT_CACHE_EMA = 11
+  PM_OPT_CACHE_EMA = 11,
+  PM_OPT_CACHE_SEEDS = 12
 };
 int pm_set_option(pm_context* ctx, int option, int value);
 

@@ -63,7 +63,7 @@ def test_options_and_accessors_reject_bad_arguments_without_gpu():
     lib = pmamd.load()
     # no context: every option call is refused before any GPU work
     for opt in (pmamd.PM_OPT_GATHER_MODE, pmamd.PM_OPT_VOXEL_GRID, pmamd.PM_OPT_TOP_LEVELS,
-                pmamd.PM_OPT_CACHE_NEAR_GROUPS, pmamd.PM_OPT_CACHE_EMA):
+                pmamd.PM_OPT_CACHE_NEAR_GROUPS, pmamd.PM_OPT_CACHE_EMA, pmamd.PM_OPT_CACHE_SEEDS):
         assert lib.pm_set_option(None, opt, 0) == pmamd.PM_ERR_INVALID
     t = (ctypes.c_uint64 * 8)()
     assert lib.pm_get_voxel_table(None, 0, t, 8) == pmamd.PM_ERR_INVALID

@@ -89,10 +89,17 @@ def test_seeded_fallback_equals_plain():
     g.cache_reset(1)
     g.cache_configure(threshold=0.0, train_samples=16384, batch=16384)
     ref = g.render().astype(np.float64)
+    frames = []
     for frame in range(2):
         img = g.render_cache()
+        frames.append(img)
         err = np.abs(img.astype(np.float64) - ref)
         assert (err <= 1e-4 * np.abs(ref) + 1e-6).all(), (frame, float(err.max()))
+    # with the seeds switched off the same frame comes from heap passes: the same bits
+    g.set_option(pmamd.PM_OPT_CACHE_SEEDS, 0)
+    assert np.array_equal(g.render_cache(), frames[1])
+    g.set_option(pmamd.PM_OPT_CACHE_SEEDS, 1)
+    assert np.array_equal(g.render_cache(), frames[1])
     n_records = g.cache_stats().records
     # another map: every record moved by up to 0.02 (the seeds are dropped)
     rng = np.random.default_rng(3)

@@ -56,10 +56,12 @@ def test_option_ranges():
     c.n_emit = 1000
     g = pmamd.PhotonMapper(c)
     for opt, bad in ((pmamd.PM_OPT_TOP_LEVELS, -1), (pmamd.PM_OPT_VOXEL_GRID, 3), (pmamd.PM_OPT_CACHE_NEAR_GROUPS, 17),
-                     (pmamd.PM_OPT_CACHE_EMA, 1000), (pmamd.PM_OPT_CACHE_EMA, -1), (99, 0)):
+                     (pmamd.PM_OPT_CACHE_EMA, 1000), (pmamd.PM_OPT_CACHE_EMA, -1), (pmamd.PM_OPT_CACHE_SEEDS, 2),
+                     (99, 0)):
         with pytest.raises(pmamd.PmError):
             g.set_option(opt, bad)
     for opt, ok in ((pmamd.PM_OPT_TOP_LEVELS, 7), (pmamd.PM_OPT_VOXEL_GRID, 2), (pmamd.PM_OPT_CACHE_NEAR_GROUPS, 0),
-                    (pmamd.PM_OPT_CACHE_EMA, 0), (pmamd.PM_OPT_CACHE_EMA, 999)):
+                    (pmamd.PM_OPT_CACHE_EMA, 0), (pmamd.PM_OPT_CACHE_EMA, 999), (pmamd.PM_OPT_CACHE_SEEDS, 0),
+                    (pmamd.PM_OPT_CACHE_SEEDS, 1)):
         g.set_option(opt, ok)
     g.close()
