@@ -328,7 +328,10 @@ typedef struct pm_cache_stats {
 /* Render the crop (x0, y0, w, h) of the configured frame through the cache:
  * record all radiance queries, train on train_samples of them (exact kNN
  * targets), predict the rest, re-gather exactly where the cell error is above
- * the threshold, accumulate.  Same outputs as pm_render_crop. */
+ * the threshold, accumulate.  Same outputs as pm_render_crop.  While the
+ * records and the photon positions stay the same from one call to the next,
+ * the exact re-gathers start from each record's k-th-neighbour bucket of the
+ * previous call (PM_OPT_CACHE_SEEDS; the same values, fewer passes). */
 int pm_render_cache_crop(pm_context* ctx, int x0, int y0, int w, int h, float* rgb_dev, pm_uchar4* out_dev);
 int pm_cache_stats_get(pm_context* ctx, pm_cache_stats* out);
 
