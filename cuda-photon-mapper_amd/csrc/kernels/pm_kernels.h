@@ -247,14 +247,14 @@ enum : int { CACHE_GRID = 16 };  // error-tracker cells per axis (x 2 kinds)
 // wave in the record lists) come from a compact patch of the image.  The
 // per-pixel counts and offsets are indexed by this order over the crop padded
 // to whole blocks (cache_order_size slots; padding slots hold no records).
-PM_HD inline uint32_t cache_pixel_order(int x, int y, int w) {
+PM_HD uint32_t cache_pixel_order(int x, int y, int w) {
   const uint32_t bx = (uint32_t)((w + 3) / 4);
   return ((uint32_t)(y >> 2) * bx + (uint32_t)(x >> 2)) * 16u + (uint32_t)((y & 3) * 4 + (x & 3));
 }
-PM_HD inline size_t cache_order_size(int w, int h) { return (size_t)((w + 3) / 4) * (size_t)((h + 3) / 4) * 16u; }
+PM_HD size_t cache_order_size(int w, int h) { return (size_t)((w + 3) / 4) * (size_t)((h + 3) / 4) * 16u; }
 // The error-tracker cell of a record at (x, y, z): its 16^3 cell (clamped to
 // the grid), plus 4096 for volume records.
-PM_HD inline uint32_t err_cell_of(float x, float y, float z, bool surf) {
+PM_HD uint32_t err_cell_of(float x, float y, float z, bool surf) {
   const float p[3] = {x, y, z};
   int c[3];
   for (int a = 0; a < 3; ++a) {
@@ -265,7 +265,7 @@ PM_HD inline uint32_t err_cell_of(float x, float y, float z, bool surf) {
   return (uint32_t)((((surf ? 0 : 1) * CACHE_GRID + c[2]) * CACHE_GRID + c[1]) * CACHE_GRID + c[0]);
 }
 // The cell of a query row (position, direction, normal or 0).
-PM_HD inline uint32_t err_cell(const float* p) {
+PM_HD uint32_t err_cell(const float* p) {
   return err_cell_of(p[0], p[1], p[2], !(p[6] == 0.0f && p[7] == 0.0f && p[8] == 0.0f));
 }
 struct CacheRecordLaunch {
