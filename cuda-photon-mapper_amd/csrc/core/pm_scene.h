@@ -214,6 +214,24 @@ PM_HD T phase_hg(T cos_theta, T g) {
   T den = T(1) + g * g - T(2) * g * cos_theta;
   return inv4pi * (T(1) - g * g) / (den * sqrt_(den));
 }
+// FP32: den^(-3/2) from y = den^(-1/2) by three Newton steps
+// y <- y (3/2 - den/2 y^2), seeded by halving the exponent in the integer
+// bits (C. Lomont, "Fast Inverse Square Root", 2003, constant 0x5f375a86:
+// relative error below 1.8e-3, then 4.7e-6, then the rounding floor).  The
+// result is within 6e-7 of den^(-3/2) for every den in [2^-48, 4] (|g| < 1),
+// against 1.5e-7 for the correctly rounded sqrt and division it replaces, at
+// about half their instructions on the GPU; plain multiplies and subtractions
+// only, so the CPU and the GPU give the same bits.
+PM_HD float phase_hg(float cos_theta, float g) {
+  const float inv4pi = 0.0795774715459476678844418816863f;
+  const float den = 1.0f + g * g - 2.0f * g * cos_theta;
+  const float h = 0.5f * den;
+  float y = u2f(0x5f375a86u - (f2u(den) >> 1));
+  y = y * (1.5f - h * (y * y));
+  y = y * (1.5f - h * (y * y));
+  y = y * (1.5f - h * (y * y));
+  return inv4pi * (1.0f - g * g) * ((y * y) * y);
+}
 
 // Unpolarised Fresnel reflectance for a dielectric interface; cos_i > 0 is the
 // cosine on the incident side, eta = n_incident / n_transmitted.  Returns 1 on

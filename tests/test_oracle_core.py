@@ -132,6 +132,27 @@ def test_phase_hg_normalised(g):
     assert abs(integral - 1) < 1e-4
 
 
+@pytest.mark.parametrize("g", [-0.95, -0.6, -0.1, 0.0, 0.3, 0.6, 0.9])
+def test_phase_hg_values(g):
+    # The FP32 phase takes den = 1 + g^2 - 2 g mu and 1 - g^2 in FP32, then
+    # den^-3/2 by Newton steps.  Against the closed form in FP64 on those
+    # FP32 values: within 6e-7 plus the two final products' roundings.
+    f = np.float32
+    mu = np.concatenate([np.linspace(-1, 1, 4001), [-1.0, 1.0, 0.0]]).astype(f)
+    g32 = f(g)
+    got = np.array([O.phase_hg(float(m), float(g32)) for m in mu])
+    den = ((f(1) + g32 * g32) - (f(2) * g32) * mu).astype(np.float64)
+    num = float(f(0.0795774715459476678844418816863) * (f(1) - g32 * g32))
+    rel = np.abs(got / (num * den ** -1.5) - 1)
+    assert rel.max() < 6e-7 + 2.0 ** -23, (rel.max(), mu[rel.argmax()])
+    # and against the exact phase at the FP32 inputs, where |g| keeps den away from cancellation
+    if abs(g) <= 0.6:
+        m = mu.astype(np.float64)
+        gg = float(g32)
+        exact = (1 - gg * gg) / (4 * np.pi * (1 + gg * gg - 2 * gg * m) ** 1.5)
+        assert np.abs(got / exact - 1).max() < 2e-6
+
+
 def test_fresnel_closed_forms():
     n = 1.5
     assert abs(O.fresnel(1.0, 1 / n) - ((n - 1) / (n + 1)) ** 2) < 1e-6   # normal incidence, 0.04
