@@ -478,19 +478,21 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
     }
     // without surface caching the subset is drawn from the volume records only
     const uint32_t ns_sel = k.cache_surface ? nk[0] : 0u;
-    HIPCHK(launch_cache_select(k.rec_in.as<float>(), list_s, ns_sel, list_v, nk[1], S, c->cfg.seed, k.frame,
-                               k.sel_in.as<float>(), st));
-    CacheValueLaunch vl;
-    memset(&vl, 0, sizeof(vl));
-    vl.gs = gs; vl.gv = gv; vl.hg_g = c->scene.med.g;
-    vl.in = k.sel_in.as<float>();
-    vl.n = S;
-    vl.out = k.sel_t.as<float>();
-    vl.mode7 = mode7;
-    HIPCHK(launch_cache_values(vl, st));
-    int r = cache_train_dev(c, k.sel_in.as<float>(), k.sel_t.as<float>(), S, k.batch, k.lr, nullptr,
-                            k.err_sum.as<unsigned long long>(), k.err_cnt.as<unsigned int>(), &loss_terms);
-    if (r != PM_OK) return r;
+    if (ns_sel > 0 || nk[1] > 0) {  // else there is nothing the network answers, and nothing to train on
+      HIPCHK(launch_cache_select(k.rec_in.as<float>(), list_s, ns_sel, list_v, nk[1], S, c->cfg.seed, k.frame,
+                                 k.sel_in.as<float>(), st));
+      CacheValueLaunch vl;
+      memset(&vl, 0, sizeof(vl));
+      vl.gs = gs; vl.gv = gv; vl.hg_g = c->scene.med.g;
+      vl.in = k.sel_in.as<float>();
+      vl.n = S;
+      vl.out = k.sel_t.as<float>();
+      vl.mode7 = mode7;
+      HIPCHK(launch_cache_values(vl, st));
+      int r = cache_train_dev(c, k.sel_in.as<float>(), k.sel_t.as<float>(), S, k.batch, k.lr, nullptr,
+                              k.err_sum.as<unsigned long long>(), k.err_cnt.as<unsigned int>(), &loss_terms);
+      if (r != PM_OK) return r;
+    }
   }
   HIPCHK(hipEventRecord(ev[2], st));
   // 5: inference over the records the cache may answer
