@@ -323,6 +323,19 @@ def run_batch(steps, tune_stride, n_scenes=64, seed=1):
     st = b.stats()
     q = st.surface_queries + st.volume_queries
     ms = float(np.median(per))
+    # the dominant kernel's roofline: the same frame launched kernel by
+    # kernel, with events around its render (one launch, every scene's tiles)
+    ms_emit, ms_build, ms_render = b.time_phases()
+    log("batch phases, kernel by kernel: emit %.1f ms, build %.1f ms, render %.1f ms" % (ms_emit, ms_build, ms_render))
+    bf16 = cfgs[0].storage == pmamd.PM_STORE_BF16
+    ks = {c.k_surface for c in cfgs} | {c.k_volume for c in cfgs}
+    bpq = gather_bytes_per_query(max(ks), bf16)
+    achieved = bpq * q / (ms_render * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(achieved / 8000.0, 4), "traffic": None,
+            "kernel": "k_render_batch (persistent work queue over every scene's tiles)",
+            "bytes_per_query": bpq, "queries_per_launch": int(q), "kernel_ms": round(ms_render, 3),
+            "phases_ms": {"emit": round(ms_emit, 3), "build": round(ms_build, 3), "render": round(ms_render, 3)}}
     b.close()
     alone = 0.0
     # the one-scene gather with the same heap keys (or mode 7)
@@ -351,6 +364,7 @@ def run_batch(steps, tune_stride, n_scenes=64, seed=1):
                    "autotune_s": round(t_tune, 2)},
         "graph_nodes": int(st.graph_nodes), "work_items": int(st.work_items),
         "scenes_one_at_a_time_ms": round(alone, 3), "speedup_vs_one_at_a_time": round(alone / ms, 3),
+        "roofline": roof,
     }
 
 

@@ -799,6 +799,30 @@ int pm_batch_submit(pm_batch* b) {
   return PM_OK;
 }
 
+int pm_batch_time_phases(pm_batch* b, double* ms_emit, double* ms_build, double* ms_render) {
+  if (!b || !ms_emit || !ms_build || !ms_render) return PM_ERR_INVALID;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() {
+      for (int i = 0; i < 4; ++i)
+        if (e[i]) (void)hipEventDestroy(e[i]);
+    }
+  } guard{ev};
+  for (int i = 0; i < 4; ++i) HIPCHK(hipEventCreate(&ev[i]));
+  // the graph's kernels, launched one by one on the batch stream
+  HIPCHK(hipEventRecord(ev[0], b->stream));
+  HIPCHK(enqueue_frame(b, b->items_d.as<uint2>(), (uint32_t)b->items_h.size(), ev[1], ev[2]));
+  HIPCHK(hipEventRecord(ev[3], b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  float t[3] = {0.0f, 0.0f, 0.0f};
+  for (int i = 0; i < 3; ++i) HIPCHK(hipEventElapsedTime(&t[i], ev[i], ev[i + 1]));
+  *ms_emit = t[0];
+  *ms_build = t[1];
+  *ms_render = t[2];
+  return PM_OK;
+}
+
 int pm_batch_image(pm_batch* b, int scene, float* rgb_dev, pm_uchar4* out_dev) {
   if (!scene_ok(b, scene)) return PM_ERR_INVALID;
   const pm_config& c = b->cfg[scene];

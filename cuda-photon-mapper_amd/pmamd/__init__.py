@@ -227,6 +227,8 @@ _SIGS = {
     "pm_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(PmBatchTuning),
                                          ctypes.POINTER(ctypes.c_double)]),
     "pm_batch_submit": (ctypes.c_int, [ctypes.c_void_p]),
+    "pm_batch_time_phases": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "pm_batch_image": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
     "pm_batch_scene_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(PmStats)]),
     "pm_batch_stats_get": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PmBatchStats)]),
@@ -515,6 +517,12 @@ class Batch:
 
     def submit(self):
         check(load().pm_batch_submit(self.h), "pm_batch_submit")
+
+    def time_phases(self):
+        """One frame launched kernel by kernel: (emit, build, render) in ms."""
+        t = [ctypes.c_double() for _ in range(3)]
+        check(load().pm_batch_time_phases(self.h, *[ctypes.byref(x) for x in t]), "pm_batch_time_phases")
+        return tuple(x.value for x in t)
 
     def stats(self):
         s = PmBatchStats()

@@ -391,6 +391,13 @@ int pm_batch_set_tuning(pm_batch* b, const pm_batch_tuning* t);
 int pm_batch_autotune(pm_batch* b, int sample_stride, pm_batch_tuning* best_out, double* best_ms_out);
 /* One frame of every scene (one graph launch); blocking. */
 int pm_batch_submit(pm_batch* b);
+/* The same frame with its kernels launched one by one on the batch's stream
+ * instead of through the graph, timed by events between its phases: the
+ * emission (trace, scans, tally), the forest build (with node boxes and the
+ * tree-ordered payload) and the render.  Blocking; leaves the same images,
+ * records and trees as pm_batch_submit, and no statistics.  No counterpart in
+ * the reference (its single scene is timed by simpleGLMain.cpp:34-49). */
+int pm_batch_time_phases(pm_batch* b, double* ms_emit, double* ms_build, double* ms_render);
 /* Scene `scene`'s image of the last frame: rgb_dev w*h*3 FP32 and/or out_dev
  * w*h uchar4 (either may be NULL). */
 int pm_batch_image(pm_batch* b, int scene, float* rgb_dev, pm_uchar4* out_dev);

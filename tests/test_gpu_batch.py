@@ -91,6 +91,24 @@ def test_batch_equals_scenes_run_alone(batch):
     assert st.surface_queries + st.volume_queries == tot_q
 
 
+def test_batch_time_phases_same_frame(batch):
+    # the frame launched kernel by kernel leaves the graph frame's images and
+    # records, and times three phases that fit inside one frame
+    cfgs, b = batch
+    b.submit()
+    imgs = [b.image(i) for i in range(len(cfgs))]
+    recs = [b.records(i, pmamd.PM_MAP_SURFACE)[0] for i in range(len(cfgs))]
+    ms = b.time_phases()
+    say("phases (emit, build, render) ms: %s" % (ms,))
+    assert all(t > 0.0 for t in ms)
+    for i in range(len(cfgs)):
+        assert np.array_equal(bits(imgs[i]), bits(b.image(i))), i
+        assert np.array_equal(bits(recs[i]), bits(b.records(i, pmamd.PM_MAP_SURFACE)[0])), i
+    b.submit()  # the graph still runs after the eager frame
+    for i in range(len(cfgs)):
+        assert np.array_equal(bits(imgs[i]), bits(b.image(i))), i
+
+
 def assert_same_radiance(ref, img, bitwise, what):
     if bitwise:
         assert np.array_equal(bits(ref), bits(img)), what
