@@ -331,8 +331,9 @@ def run_batch(steps, tune_stride, n_scenes=64, seed=1):
     ks = {c.k_surface for c in cfgs} | {c.k_volume for c in cfgs}
     bpq = gather_bytes_per_query(max(ks), bf16)
     achieved = bpq * q / (ms_render * 1e-3) / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-            "frac": round(achieved / 8000.0, 4), "traffic": None,
+    tb = read_profile("traffic_batch_cfg4.json")
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": (tb.get("bytes_per_launch") if tb else None),
             "kernel": "k_render_batch (persistent work queue over every scene's tiles)",
             "bytes_per_query": bpq, "queries_per_launch": int(q), "kernel_ms": round(ms_render, 3),
             "phases_ms": {"emit": round(ms_emit, 3), "build": round(ms_build, 3), "render": round(ms_render, 3)}}
