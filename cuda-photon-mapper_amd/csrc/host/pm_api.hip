@@ -111,10 +111,11 @@ int do_emit(pm_context* c) {
   uint32_t N = fixed ? c->cfg.n_emit
                      : (uint32_t)std::min<uint64_t>(std::min<uint64_t>(std::max<uint64_t>(cap_s, 1u << 16), 1u << 30),
                                                     max_photons);
-  // caps mode: start from the photon count the last emission's loop ended at.
-  // The records are those of the first N_total photons whatever N the loop
-  // reaches, so a larger start only traces more photons through the count
-  // pass, and the same scene needs one pass instead of a doubling sequence.
+  // caps mode: start from the photon count the last emission needed (its
+  // N_total).  The records are those of the first N_total photons whatever N
+  // the loop reaches, so a larger start only traces more photons through the
+  // count pass, and the same scene needs one pass instead of a doubling
+  // sequence.
   if (!fixed && c->caps_N > N) N = (uint32_t)std::min<uint64_t>(c->caps_N, max_photons);
   uint32_t done = 0;
   uint32_t tot[2] = {0, 0};
@@ -137,7 +138,6 @@ int do_emit(pm_context* c) {
     if (N >= max_photons) return PM_ERR_CAPACITY;
     N = (uint32_t)std::min<uint64_t>((uint64_t)N * 2u, max_photons);
   }
-  if (!fixed) c->caps_N = N;
   uint32_t n_s = (uint32_t)std::min<uint64_t>(tot[0], cap_s);
   uint32_t n_v = media ? (uint32_t)std::min<uint64_t>(tot[1], cap_v) : 0u;
   uint32_t N_s = N, N_v = N, N_total = N;
@@ -155,6 +155,8 @@ int do_emit(pm_context* c) {
     if (N_s == 0xffffffffu || (media && N_v == 0xffffffffu)) return PM_ERR_STATE;
     N_total = media ? std::max(N_s, N_v) : N_s;
     if (!media) N_v = N_total;
+    // the next emission starts its count pass at the photon that met both caps
+    c->caps_N = N_total;
   }
   int r;
   const int bf16 = c->cfg.storage == PM_STORE_BF16;

@@ -123,7 +123,7 @@ struct pm_context {
   pm::MapBufs ms, mv;
   pm::DevBuf cnt_s, cnt_v, off_s, off_v, scratch, tally_part, tally9, small, counters;
   uint32_t n_emit_total = 0;
-  uint32_t caps_N = 0;  // caps mode: the photon count the last emission's doubling loop ended at
+  uint32_t caps_N = 0;  // caps mode: the photons the last emission needed to meet both caps (its N_total)
   double tally_h[9];
   bool emitted = false, built = false;
   int gather_mode = 5;  // auto
