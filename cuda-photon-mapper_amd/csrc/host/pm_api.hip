@@ -221,7 +221,8 @@ void pm::shape_tables(uint32_t n, uint32_t L, std::vector<uint32_t>& lo, std::ve
 
 namespace {
 
-int build_map(pm_context* c, MapBufs& mb) {
+// Builds mb's tree on stream st with temp as its scratch.
+int build_map(pm_context* c, MapBufs& mb, DevBuf& temp, hipStream_t st) {
   DevMap& m = mb.m;
   if (!mb.shape.p || mb.shape_n != m.n || mb.shape_L != m.L) {
     // the shape tables depend on (n, L) only: computed and uploaded once per size
@@ -235,14 +236,14 @@ int build_map(pm_context* c, MapBufs& mb) {
       t[2 * stride + i] = lc[i];
       t[3 * stride + i] = lb[i];
     }
-    HIPCHK(hipStreamSynchronize(c->stream));  // no build still reads the old tables
+    HIPCHK(hipStreamSynchronize(st));  // no build still reads the old tables
     HIPCHK(mb.shape.ensure(4ull * t.size()));
     HIPCHK(hipMemcpy(mb.shape.p, t.data(), 4ull * t.size(), hipMemcpyHostToDevice));
     mb.shape_n = m.n;
     mb.shape_L = m.L;
   }
-  HIPCHK(c->scratch.ensure(4ull * kdbuild_temp_words(m.n, m.L)));
-  HIPCHK(kd_build_impl(m, c->scratch.as<uint32_t>(), mb.shape.as<uint32_t>(), c->stream));
+  HIPCHK(temp.ensure(4ull * kdbuild_temp_words(m.n, m.L)));
+  HIPCHK(kd_build_impl(m, temp.as<uint32_t>(), mb.shape.as<uint32_t>(), st));
   return PM_OK;
 }
 
@@ -365,6 +366,9 @@ int pm_create(const pm_config* cfg, pm_context** out) {
   memset(&c->stats, 0, sizeof(c->stats));
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreate(&c->e0);
   if (e == hipSuccess) e = hipEventCreate(&c->e1);
   if (e == hipSuccess) e = hipEventCreate(&c->e2);
@@ -385,16 +389,20 @@ int pm_create(const pm_config* cfg, pm_context** out) {
 int pm_destroy(pm_context* c) {
   if (!c) return PM_ERR_INVALID;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   cache_release(c->cache);
   c->ms.release();
   c->mv.release();
   for (DevBuf* b : {&c->spheres_d, &c->density_d, &c->cnt_s, &c->cnt_v, &c->off_s, &c->off_v, &c->scratch,
-                    &c->tally_part, &c->tally9, &c->small, &c->counters, &c->vox_grid_s, &c->vox_grid_v,
+                    &c->scratch2, &c->tally_part, &c->tally9, &c->small, &c->counters, &c->vox_grid_s, &c->vox_grid_v,
                     &c->vox_sat_s, &c->vox_sat_v})
     b->release();
   if (c->e0) (void)hipEventDestroy(c->e0);
   if (c->e1) (void)hipEventDestroy(c->e1);
   if (c->e2) (void)hipEventDestroy(c->e2);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return PM_OK;
@@ -409,9 +417,16 @@ int pm_build(pm_context* c) {
   if (!c) return PM_ERR_INVALID;
   if (!c->emitted) return PM_ERR_STATE;
   HIPCHK(hipEventRecord(c->e0, c->stream));
+  // the two maps' builds are independent and each is a chain of small,
+  // latency-bound level passes: the volume map's runs on a second stream
+  // with its own scratch, and the first stream waits for it
+  HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+  HIPCHK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
   int r;
-  if ((r = build_map(c, c->ms)) != PM_OK) return r;
-  if ((r = build_map(c, c->mv)) != PM_OK) return r;
+  if ((r = build_map(c, c->ms, c->scratch, c->stream)) != PM_OK) return r;
+  if ((r = build_map(c, c->mv, c->scratch2, c->stream2)) != PM_OK) return r;
+  HIPCHK(hipEventRecord(c->ev_join, c->stream2));
+  HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
   HIPCHK(hipEventRecord(c->e1, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->stats.ms_build = elapsed_ms(c->e0, c->e1);

@@ -115,13 +115,14 @@ struct pm_context {
   pm_config cfg;
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // the volume map's kd-build, beside the surface map's on `stream`
   std::vector<pm::Sphere> spheres_h;
   pm::DevBuf spheres_d, density_d;
   pm::Scene scene;
   pm::TraceParams tp;
   pm::RenderParams rp;
   pm::MapBufs ms, mv;
-  pm::DevBuf cnt_s, cnt_v, off_s, off_v, scratch, tally_part, tally9, small, counters;
+  pm::DevBuf cnt_s, cnt_v, off_s, off_v, scratch, scratch2, tally_part, tally9, small, counters;
   uint32_t n_emit_total = 0;
   uint32_t caps_N = 0;  // caps mode: the photons the last emission needed to meet both caps (its N_total)
   double tally_h[9];
@@ -137,5 +138,6 @@ struct pm_context {
   pm::DevBuf vox_grid_s, vox_grid_v, vox_sat_s, vox_sat_v;
   pm_stats stats;
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;  // the two-stream kd-build's ordering
   pm::CacheState cache;
 };
