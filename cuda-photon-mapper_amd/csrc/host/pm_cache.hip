@@ -139,7 +139,7 @@ void cache_release(CacheState& k) {
   for (DevBuf* b : {&k.params, &k.grads, &k.adam_m, &k.adam_v, &k.params_bf16, &k.params_ema, &k.ema_bf16, &k.X, &k.y, &k.dy,
                     &k.row_err, &k.rgb, &k.err_sum, &k.err_cnt, &k.rec_cnt, &k.rec_off, &k.rec_in,
                     &k.rec_wt, &k.rec_code, &k.rec_val, &k.sel_in, &k.sel_t, &k.list, &k.list_cnt, &k.cell_fb,
-                    &k.row_code, &k.row_list, &k.loss_part, &k.loss_acc, &k.rec_q, &k.map_hash,
+                    &k.row_code, &k.row_list, &k.loss_part, &k.loss_acc, &k.rec_q, &k.map_hash, &k.list_fb,
                     &k.XT, &k.dT[0], &k.dT[1], &k.dT[2], &k.dT[3], &k.dyT, &k.tpart})
     b->release();
   for (int l = 0; l < MLP_HIDDEN_LAYERS; ++l) {
@@ -460,6 +460,39 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   // skips the surface records, whose values the exact fallback writes
   const uint32_t* infer_list = nullptr;
   uint32_t n_infer = 0;
+  // seeds of the counting first pass: each fallback record's bucket from the
+  // previous frame, kept while the records and the photon positions are
+  // unchanged (the same record count and map fingerprints; a static scene
+  // re-emitted with the same seed, or a reused map).  Any seed gives the same
+  // values, but seeds from other photon positions miss often and cost more
+  // than the heap passes they replace, so they are dropped.
+  uint16_t* seeds = nullptr;
+  bool seeds_checked = false;
+  auto prepare_seeds = [&]() -> int {
+    if (seeds_checked || !mode7 || !k.use_seeds) return PM_OK;
+    seeds_checked = true;
+    HIPCHK(k.map_hash.ensure(16));
+    unsigned long long* mh = (unsigned long long*)k.map_hash.p;
+    HIPCHK(launch_point_hash(c->ms.m.tpts, c->ms.m.n, mh, st));
+    HIPCHK(launch_point_hash(c->mv.m.tpts, c->mv.m.n, mh + 1, st));
+    unsigned long long hh[2] = {0ull, 0ull};
+    HIPCHK(hipMemcpyAsync(hh, mh, 16, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (k.rec_q_n != N || !k.rec_q.p || hh[0] != k.rec_q_hash[0] || hh[1] != k.rec_q_hash[1]) {
+      HIPCHK(k.rec_q.ensure(2ull * std::max<uint64_t>(N, 1)));
+      HIPCHK(hipMemsetAsync(k.rec_q.p, 0xff, 2ull * N, st));
+      k.rec_q_n = N;
+      k.rec_q_hash[0] = hh[0];
+      k.rec_q_hash[1] = hh[1];
+    }
+    seeds = k.rec_q.as<uint16_t>();
+    return PM_OK;
+  };
+  // when every surface record falls back (surfaces not cached), their exact
+  // values do not depend on the network: they are gathered on the second
+  // stream while the network trains and infers, from the surface list of the
+  // kind lists (k.list, not rewritten in this frame after that point)
+  bool surface_early = false;
   if (S > 0) {
     HIPCHK(k.sel_in.ensure(4ull * MLP_IN * S));
     HIPCHK(k.sel_t.ensure(12ull * S));
@@ -475,6 +508,26 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
     if (!k.cache_surface) {
       infer_list = list_v;
       n_infer = nk[1];
+    }
+    if (mode7 && !k.cache_surface && nk[0] > 0) {
+      int rs = prepare_seeds();
+      if (rs != PM_OK) return rs;
+      HIPCHK(hipEventRecord(c->ev_fork, st));
+      HIPCHK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+      CacheValueLaunch vs;
+      memset(&vs, 0, sizeof(vs));
+      vs.gs = gs; vs.gv = gv; vs.hg_g = c->scene.med.g;
+      vs.in = k.rec_in.as<float>();
+      vs.list = list_s;
+      vs.n = nk[0];
+      vs.out = k.rec_val.as<float>();
+      vs.out_by_list = 1;
+      vs.mode7 = mode7;
+      vs.kind = 1;
+      vs.seed_q = seeds;
+      HIPCHK(launch_cache_values(vs, c->stream2));
+      HIPCHK(hipEventRecord(c->ev_join, c->stream2));
+      surface_early = true;
     }
     // without surface caching the subset is drawn from the volume records only
     const uint32_t ns_sel = k.cache_surface ? nk[0] : 0u;
@@ -506,47 +559,26 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
   HIPCHK(k.cell_fb.ensure(CACHE_CELLS));
   HIPCHK(launch_cache_cell_fb(k.err_sum.as<unsigned long long>(), k.err_cnt.as<unsigned int>(), k.threshold,
                               k.cell_fb.as<uint8_t>(), st));
-  // the maps' fingerprints, read with the fallback lists' lengths below
-  HIPCHK(k.map_hash.ensure(16));
-  unsigned long long* mh = (unsigned long long*)k.map_hash.p;
-  if (mode7 && k.use_seeds) {
-    HIPCHK(launch_point_hash(c->ms.m.tpts, c->ms.m.n, mh, st));
-    HIPCHK(launch_point_hash(c->mv.m.tpts, c->mv.m.n, mh + 1, st));
-  }
   CacheListRule fbr;
   fbr.fallback = 1;
   fbr.cache_surface = k.cache_surface;
   fbr.cell_fb = k.cell_fb.as<uint8_t>();
   uint32_t nk2[2] = {0, 0};
-  r = build_lists(c, k.rec_code.as<uint16_t>(), N, fbr, k.list.as<uint32_t>(), nk2);
+  HIPCHK(k.list_fb.ensure(8ull * std::max<uint64_t>(N, 1)));
+  r = build_lists(c, k.rec_code.as<uint16_t>(), N, fbr, k.list_fb.as<uint32_t>(), nk2);
   if (r != PM_OK) return r;
   const uint32_t nfb = nk2[0] + nk2[1];
-  // seeds of the counting first pass: each fallback record's bucket from the
-  // previous frame, kept while the records and the photon positions are
-  // unchanged (the same record count and map fingerprints; a static scene
-  // re-emitted with the same seed, or a reused map).  Any seed gives the same
-  // values, but seeds from other photon positions miss often and cost more
-  // than the heap passes they replace, so they are dropped.
-  uint16_t* seeds = nullptr;
-  if (mode7 && nfb > 0 && k.use_seeds) {
-    unsigned long long h[2] = {0ull, 0ull};
-    HIPCHK(hipMemcpy(h, mh, 16, hipMemcpyDeviceToHost));
-    if (k.rec_q_n != N || !k.rec_q.p || h[0] != k.rec_q_hash[0] || h[1] != k.rec_q_hash[1]) {
-      HIPCHK(k.rec_q.ensure(2ull * std::max<uint64_t>(N, 1)));
-      HIPCHK(hipMemsetAsync(k.rec_q.p, 0xff, 2ull * N, st));
-      k.rec_q_n = N;
-      k.rec_q_hash[0] = h[0];
-      k.rec_q_hash[1] = h[1];
-    }
-    seeds = k.rec_q.as<uint16_t>();
+  if (mode7 && nfb > 0) {
+    r = prepare_seeds();
+    if (r != PM_OK) return r;
   }
   for (int kind = 1; kind <= 2; ++kind) {  // one launch per kind of row
-    if (nk2[kind - 1] == 0) continue;
+    if (nk2[kind - 1] == 0 || (kind == 1 && surface_early)) continue;
     CacheValueLaunch vl;
     memset(&vl, 0, sizeof(vl));
     vl.gs = gs; vl.gv = gv; vl.hg_g = c->scene.med.g;
     vl.in = k.rec_in.as<float>();
-    vl.list = k.list.as<uint32_t>() + (kind == 2 ? nk2[0] : 0u);
+    vl.list = k.list_fb.as<uint32_t>() + (kind == 2 ? nk2[0] : 0u);
     vl.n = nk2[kind - 1];
     vl.out = k.rec_val.as<float>();
     vl.out_by_list = 1;
@@ -555,6 +587,7 @@ int pm_render_cache_crop(pm_context* c, int x0, int y0, int w, int h, float* rgb
     vl.seed_q = seeds;
     HIPCHK(launch_cache_values(vl, st));
   }
+  if (surface_early) HIPCHK(hipStreamWaitEvent(st, c->ev_join, 0));
   HIPCHK(hipEventRecord(ev[4], st));
   // 7: per-pixel accumulation
   HIPCHK(launch_cache_accumulate(k.rec_off.as<uint32_t>(), k.rec_cnt.as<uint32_t>(), w, h, k.rec_wt.as<float>(),

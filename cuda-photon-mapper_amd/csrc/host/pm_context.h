@@ -88,6 +88,7 @@ struct CacheState {
   float ema_decay = 0.95f;  // inference on the moving average of the weights (decay per Adam step), 0: off
   bool ema_stale = true;    // the average must restart at the current weights before its next use
   DevBuf rec_cnt, rec_off, rec_in, rec_wt, rec_code, rec_val, sel_in, sel_t, list, list_cnt, cell_fb;
+  DevBuf list_fb;  // the fallback lists (list keeps the kind lists, which the early surface gather reads)
   pm_cache_stats st;
 };
 

@@ -319,6 +319,9 @@ typedef struct pm_cache_stats {
   uint64_t records;        /* radiance queries of the frame */
   uint64_t fallbacks;      /* of which gathered exactly (kNN) */
   double train_loss;       /* mean loss of this frame's training steps */
+  /* phase times on the frame's stream; when surface records are not cached,
+   * their exact gather runs on a second stream from the start of training,
+   * so it overlaps ms_train and ms_infer and ms_fallback holds the rest */
   double ms_record, ms_train, ms_infer, ms_fallback, ms_accumulate, ms_total;
   uint32_t frame;          /* frames rendered with this cache */
   uint32_t inferred_volume; /* of the inferred rows, volume records on the 5-step layer 1 */
